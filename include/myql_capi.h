@@ -1,0 +1,228 @@
+/*
+ * myql_capi.h -- C-ABI of the MI355X Monte Carlo engine (libmyql_hip.so).
+ *
+ * Plain C: POD structs, raw pointers and sizes, integer status codes. No
+ * C++ or torch types cross this boundary. The C++20 drop-in headers under
+ * include/myql/ (MonteCarloPricer, MultiLevelMonteCarloPricer, ...) marshal
+ * their template parameters into these structs and rethrow status codes as
+ * the exceptions the reference library throws.
+ *
+ * Each entry point replaces one reference interface:
+ *   myql_mc_calculate        MonteCarloPricer<...>::calculate
+ *                            (reference: include/myql/pricers/montecarlo/
+ *                             MonteCarloPricer.hpp:95-481)
+ *   myql_mc_calculate_batch  several MonteCarloPricer::calculate calls (one
+ *                            per job) fused into one persistent launch; the
+ *                            reference runs them one after another
+ *                            (MonteCarloPricer.hpp:95, called per maturity in
+ *                             benchmarks/test_smile_suite.cpp:72-73)
+ *   myql_mlmc_calculate      MultiLevelMonteCarloPricer<...>::calculate
+ *                            (reference: include/myql/pricers/mlmc/
+ *                             MultiLevelMonteCarloPricer.hpp:46-241)
+ *   myql_create/destroy      (no counterpart: the reference has no device
+ *                            state; a context owns a HIP stream, cached
+ *                            stream states and NCI tables)
+ *
+ * Ownership: the caller owns every pointer it passes; the library never
+ * frees or retains caller memory past the call. Calls are synchronous on the
+ * context's HIP stream. One context per host thread; different contexts may
+ * be used concurrently.
+ */
+#ifndef MYQL_CAPI_H
+#define MYQL_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  MYQL_OK = 0,
+  MYQL_E_INVALID_ARG = 1, /* std::invalid_argument in the C++ layer */
+  MYQL_E_NV_FELLER = 2,   /* std::runtime_error("SchemeNV violation: ...") */
+  MYQL_E_HIP = 3,         /* HIP runtime failure (launch, copy, device) */
+  MYQL_E_OOM = 4,         /* device allocation failed */
+  MYQL_E_UNSUPPORTED = 5  /* combination not implemented */
+};
+
+/* ---- enums mirroring the reference's template parameters --------------- */
+enum { MYQL_GREEK_NONE = 0, MYQL_GREEK_ESSENTIAL = 1, MYQL_GREEK_FULL = 2 };
+/* VolSchemes.hpp: NullVolScheme, SchemeNV, SchemeNCI, SchemeExact */
+enum {
+  MYQL_SCHEME_NULL = 0,
+  MYQL_SCHEME_NV = 1,
+  MYQL_SCHEME_NCI = 2,
+  MYQL_SCHEME_EXACT = 3
+};
+/* JumpPolicies.hpp: NoJumps, MertonJump, KouJump */
+enum { MYQL_JUMP_NONE = 0, MYQL_JUMP_MERTON = 1, MYQL_JUMP_KOU = 2 };
+/* instruments/options: European, Asian (Geo/Arith tracker), Lookback, Barrier */
+enum {
+  MYQL_INSTR_EUROPEAN = 0,
+  MYQL_INSTR_GEO_ASIAN = 1,
+  MYQL_INSTR_ARITH_ASIAN = 2,
+  MYQL_INSTR_LOOKBACK = 3,
+  MYQL_INSTR_BARRIER = 4
+};
+/* Payoffs.hpp: PayoffVanilla, PayoffCashOrNothing, PayoffAssetOrNothing */
+enum { MYQL_PAYOFF_VANILLA = 0, MYQL_PAYOFF_CASH = 1, MYQL_PAYOFF_ASSET = 2 };
+enum { MYQL_CALL = 0, MYQL_PUT = 1 };
+enum { MYQL_BARRIER_UP = 0, MYQL_BARRIER_DOWN = 1 };
+enum { MYQL_BARRIER_IN = 0, MYQL_BARRIER_OUT = 1 };
+enum {
+  MYQL_REBATE_NONE = 0,
+  MYQL_REBATE_AT_HIT = 1,
+  MYQL_REBATE_AT_MATURITY = 2
+};
+/* Arithmetic type of the path state. Sums are always FP64. */
+enum { MYQL_FP64 = 0, MYQL_FP32 = 1 };
+/* Stream layout: how the S per-lane xoshiro256++ streams are derived.
+ *  JUMP   stream s = splitmix64-seeded state of `seed`, jumped s * 2^128.
+ *  SEEDED stream s = state seeded with seed + s * 999983, the reference's
+ *         per-OpenMP-thread seeding (MonteCarloPricer.hpp:191); with S equal
+ *         to OMP_NUM_THREADS the draws equal the reference's.
+ * In both layouts stream s prices the contiguous block of paths that an
+ * OpenMP static schedule over S threads would give thread s. */
+enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
+
+/* ---- model (ModelParams.hpp:7-47, ASVJmodel.hpp:14-87) ----------------- */
+typedef struct {
+  double kappa, theta, sigma, rho, v0; /* HestonParams */
+} myql_heston_params;
+
+typedef struct {
+  int32_t n_factors;   /* 0: Zero-, 1: Single-, 2: DoubleFactorModel */
+  int32_t jump_kind;   /* MYQL_JUMP_* */
+  double vol;          /* ZeroFactorModel::vol */
+  myql_heston_params heston[2];
+  /* MertonParams {lambda, mu, delta, -}; KouParams {lambda, p_up, eta1, eta2} */
+  double jump[4];
+} myql_model;
+
+/* ---- instrument (European.hpp, Asian.hpp, Lookback.hpp, Barrier.hpp) ---- */
+typedef struct {
+  int32_t kind;           /* MYQL_INSTR_* */
+  int32_t payoff;         /* MYQL_PAYOFF_* */
+  int32_t option_type;    /* MYQL_CALL / MYQL_PUT */
+  int32_t fixed_strike;   /* Asian/Lookback: 1 fixed strike, 0 floating */
+  int32_t barrier_dir;    /* MYQL_BARRIER_UP / DOWN */
+  int32_t barrier_action; /* MYQL_BARRIER_IN / OUT */
+  int32_t rebate_timing;  /* MYQL_REBATE_* */
+  int32_t is_strip;       /* 0: StrikeContainer=double, 1: std::vector */
+  double maturity;
+  double barrier, rebate, rebate_rate; /* BarrierOption B, rebate, r */
+  double payoff_eps;      /* digital smoothing bandwidth; < 0 = automatic */
+  double past_avg_log, past_avg_S, t_elapsed; /* in-flight Asian */
+  double hist_min_S, hist_max_S;              /* in-flight Lookback (0 = none) */
+  const double *strikes;  /* n_strikes strikes (host memory) */
+  int64_t n_strikes;
+} myql_instrument;
+
+/* ---- MonteCarloConfig (MonteCarloPricer.hpp:16-26) + engine options ---- */
+typedef struct {
+  uint64_t num_paths;
+  uint64_t time_steps;
+  uint64_t seed;
+  double fd_bump, vol_bump, t_bump, r_bump;
+  int32_t precision;     /* MYQL_FP64 / MYQL_FP32 */
+  int32_t stream_layout; /* MYQL_STREAMS_* */
+  uint64_t num_streams;  /* S; 0 = library default (65536, at most num_paths) */
+} myql_mc_config;
+
+/* ---- MonteCarloResult<Mode, T> (PricingTypes.hpp:46-97) ----------------
+ * Every pointer is a caller-owned array of n_strikes doubles, or NULL to
+ * skip that field. Fields the Greek mode does not produce are left alone. */
+typedef struct {
+  double *price, *price_std_err;
+  double *delta, *delta_std_err;
+  double *gamma, *gamma_std_err;
+  double *vega[2], *vega_std_err[2];
+  double *theta, *theta_std_err;
+  double *rho, *rho_std_err;
+  double *vanna[2], *vanna_std_err[2];
+  double *charm, *charm_std_err;
+  uint64_t total_steps;    /* simulated path-steps, all Greek variants */
+  double device_ms;        /* GPU time of the pricing kernels (HIP events) */
+} myql_mc_result;
+
+/* One MonteCarloPricer::calculate call inside a batch. */
+typedef struct {
+  const myql_model *model;
+  int32_t schemes[2];
+  int32_t greek_mode;
+  const myql_instrument *instrument;
+  const myql_mc_config *config;
+  double S0, r, q;
+  myql_mc_result *result;
+} myql_mc_job;
+
+/* ---- MLMCConfig (MultiLevelMonteCarloPricer.hpp:16-30) ----------------- */
+typedef struct {
+  double epsilon;
+  uint64_t L_min, L_max, N0, base_steps;
+  double alpha, beta;
+  int32_t adaptive_rates;
+  uint64_t seed;
+  double fd_bump;
+  int32_t precision;     /* path arithmetic; level sums are FP64 */
+  int32_t stream_layout;
+  uint64_t num_streams;
+} myql_mlmc_config;
+
+/* MlmcResult<None, double> (PricingTypes.hpp:102-115). The three arrays are
+ * caller-owned with L_max + 1 entries; n_levels reports how many are used. */
+typedef struct {
+  double price, price_std_err;
+  uint64_t total_steps;
+  int32_t n_levels;
+  uint64_t *paths_per_level;
+  double *mean_diffs, *var_diffs;
+  double alpha_estimated, beta_estimated;
+  double device_ms;
+} myql_mlmc_result;
+
+typedef struct myql_ctx myql_ctx;
+
+/* Context on HIP device `device`; NULL on failure. */
+myql_ctx *myql_create(int device);
+void myql_destroy(myql_ctx *ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char *myql_last_error(const myql_ctx *ctx);
+
+int myql_mc_calculate(myql_ctx *ctx, const myql_model *model,
+                      const int32_t schemes[2], const myql_instrument *instr,
+                      int32_t greek_mode, const myql_mc_config *cfg,
+                      double S0, double r, double q, myql_mc_result *res);
+
+int myql_mc_calculate_batch(myql_ctx *ctx, const myql_mc_job *jobs,
+                            int64_t n_jobs);
+
+int myql_mlmc_calculate(myql_ctx *ctx, const myql_model *model,
+                        const int32_t schemes[2], const myql_instrument *instr,
+                        int32_t greek_mode, const myql_mlmc_config *cfg,
+                        double S0, double r, double q, myql_mlmc_result *res);
+
+/* ---- diagnostics used by the parity tests ------------------------------
+ * First n_per_stream raw xoshiro256++ outputs of streams [0, n_streams) for
+ * (seed, layout), written stream-major into out (host memory). The draws
+ * come from the same device code the pricing kernels use. */
+int myql_debug_stream_draws(myql_ctx *ctx, uint64_t seed, int32_t layout,
+                            uint64_t n_streams, uint64_t n_per_stream,
+                            uint64_t *out);
+/* Device Acklam inverse normal (FP64 and FP32 paths) on n host inputs. */
+int myql_debug_normcdfinv(myql_ctx *ctx, const double *p, int64_t n,
+                          double *out64, float *out32);
+/* Device Giles inverse Poisson CDF on n (lambda, U) pairs (FP64). */
+int myql_debug_poissinv(myql_ctx *ctx, const double *lambda, const double *u,
+                        int64_t n, double *out);
+/* Device chi-squared quantile (inverse regularised incomplete gamma). */
+int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
+                             int64_t n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MYQL_CAPI_H */

@@ -1,0 +1,282 @@
+"""Python host-side mirror of the engine's C-ABI: builders for the model,
+instrument and config structs, result marshalling into numpy, and `Engine`,
+a thin owner of one `myql_ctx` on a HIP device.
+
+Names follow the reference library (HestonParams, MertonParams, KouParams,
+MonteCarloConfig, MLMCConfig, GreekMode, EuropeanOption, ...): reference
+include/myql/models/asvj/data/ModelParams.hpp:7-47,
+include/myql/pricers/montecarlo/MonteCarloPricer.hpp:16-26,
+include/myql/pricers/mlmc/MultiLevelMonteCarloPricer.hpp:16-30.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import capi
+from .capi import (Model, Instrument, MCConfig, MCResult, MLMCConfig, MLMCResult,
+                   HestonParams)
+
+RESULT_FIELDS = ["price", "delta", "gamma", "theta", "rho", "charm"]
+PAIR_FIELDS = ["vega", "vanna"]
+
+
+@dataclass
+class Heston:
+    kappa: float
+    theta: float
+    sigma: float
+    rho: float
+    v0: float
+
+
+def make_model(n_factors: int, heston: Sequence[Heston] = (), vol: float = 0.0,
+               jump_kind: int = capi.JUMP_NONE, jump: Sequence[float] = (0, 0, 0, 0)) -> Model:
+    m = Model()
+    m.n_factors = n_factors
+    m.jump_kind = jump_kind
+    m.vol = vol
+    for i, h in enumerate(heston):
+        m.heston[i] = HestonParams(h.kappa, h.theta, h.sigma, h.rho, h.v0)
+    for i, x in enumerate(list(jump) + [0.0] * (4 - len(jump))):
+        m.jump[i] = x
+    return m
+
+
+def merton(lam: float, mu: float, delta: float):
+    return capi.JUMP_MERTON, (lam, mu, delta, 0.0)
+
+
+def kou(lam: float, p_up: float, eta1: float, eta2: float):
+    return capi.JUMP_KOU, (lam, p_up, eta1, eta2)
+
+
+@dataclass
+class InstrumentSpec:
+    """Owns the strike buffer the C struct points to."""
+    kind: int = capi.INSTR_EUROPEAN
+    payoff: int = capi.PAYOFF_VANILLA
+    option_type: int = capi.CALL
+    strikes: Sequence[float] = (100.0,)
+    maturity: float = 1.0
+    is_strip: Optional[bool] = None
+    fixed_strike: bool = True
+    barrier_dir: int = capi.BARRIER_UP
+    barrier_action: int = capi.BARRIER_OUT
+    rebate_timing: int = capi.REBATE_NONE
+    barrier: float = 0.0
+    rebate: float = 0.0
+    rebate_rate: float = 0.0
+    payoff_eps: float = -1.0
+    past_avg_log: float = 0.0
+    past_avg_S: float = 0.0
+    t_elapsed: float = 0.0
+    hist_min_S: float = 0.0
+    hist_max_S: float = 0.0
+    _k: np.ndarray = field(default=None, repr=False)
+    _s: Instrument = field(default=None, repr=False)
+
+    def struct(self) -> Instrument:
+        self._k = np.ascontiguousarray(np.asarray(self.strikes, dtype=np.float64).reshape(-1))
+        s = Instrument()
+        s.kind, s.payoff, s.option_type = self.kind, self.payoff, self.option_type
+        s.fixed_strike = int(self.fixed_strike)
+        s.barrier_dir, s.barrier_action = self.barrier_dir, self.barrier_action
+        s.rebate_timing = self.rebate_timing
+        strip = self.is_strip if self.is_strip is not None else len(self._k) > 1
+        s.is_strip = int(strip)
+        s.maturity, s.barrier, s.rebate = self.maturity, self.barrier, self.rebate
+        s.rebate_rate, s.payoff_eps = self.rebate_rate, self.payoff_eps
+        s.past_avg_log, s.past_avg_S, s.t_elapsed = self.past_avg_log, self.past_avg_S, self.t_elapsed
+        s.hist_min_S, s.hist_max_S = self.hist_min_S, self.hist_max_S
+        s.strikes = self._k.ctypes.data_as(C.POINTER(C.c_double))
+        s.n_strikes = len(self._k)
+        self._s = s
+        return s
+
+
+def mc_config(num_paths: int, time_steps: int, seed: int = 42, fd_bump: float = 1e-4,
+              vol_bump: float = 1e-4, t_bump: float = 1e-4, r_bump: float = 1e-4,
+              precision: int = capi.FP64, stream_layout: int = capi.STREAMS_JUMP,
+              num_streams: int = 0) -> MCConfig:
+    return MCConfig(num_paths, time_steps, seed, fd_bump, vol_bump, t_bump, r_bump, precision,
+                    stream_layout, num_streams)
+
+
+def mlmc_config(epsilon: float = 0.005, L_min: int = 2, L_max: int = 15, N0: int = 10000,
+                base_steps: int = 1, alpha: float = 2.0, beta: float = 1.0,
+                adaptive_rates: bool = True, seed: int = 42, fd_bump: float = 1e-4,
+                precision: int = capi.FP64, stream_layout: int = capi.STREAMS_JUMP,
+                num_streams: int = 0) -> MLMCConfig:
+    return MLMCConfig(epsilon, L_min, L_max, N0, base_steps, alpha, beta, int(adaptive_rates),
+                      seed, fd_bump, precision, stream_layout, num_streams)
+
+
+class ResultBuffers:
+    """numpy arrays that an MCResult points into."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.arr = {}
+        r = MCResult()
+        pd = C.POINTER(C.c_double)
+        for f in RESULT_FIELDS:
+            for suffix in ("", "_std_err"):
+                a = np.full(n, np.nan)
+                self.arr[f + suffix] = a
+                setattr(r, f + suffix, a.ctypes.data_as(pd))
+        for f in PAIR_FIELDS:
+            for suffix in ("", "_std_err"):
+                a = np.full((2, n), np.nan)
+                self.arr[f + suffix] = a
+                arrp = getattr(r, f + suffix)
+                arrp[0] = a[0].ctypes.data_as(pd)
+                arrp[1] = a[1].ctypes.data_as(pd)
+        self.struct = r
+
+    def as_dict(self) -> dict:
+        d = {k: v.copy() for k, v in self.arr.items()}
+        d["total_steps"] = int(self.struct.total_steps)
+        d["device_ms"] = float(self.struct.device_ms)
+        return d
+
+
+class MLMCBuffers:
+    def __init__(self, L_max: int):
+        self.ppl = np.zeros(L_max + 1, dtype=np.uint64)
+        self.mean = np.full(L_max + 1, np.nan)
+        self.var = np.full(L_max + 1, np.nan)
+        r = MLMCResult()
+        r.paths_per_level = self.ppl.ctypes.data_as(C.POINTER(C.c_uint64))
+        r.mean_diffs = self.mean.ctypes.data_as(C.POINTER(C.c_double))
+        r.var_diffs = self.var.ctypes.data_as(C.POINTER(C.c_double))
+        self.struct = r
+
+    def as_dict(self) -> dict:
+        r = self.struct
+        n = int(r.n_levels)
+        return dict(price=r.price, price_std_err=r.price_std_err, total_steps=int(r.total_steps),
+                    n_levels=n, paths_per_level=self.ppl[:n].copy(),
+                    mean_diffs=self.mean[:n].copy(), var_diffs=self.var[:n].copy(),
+                    alpha_estimated=r.alpha_estimated, beta_estimated=r.beta_estimated,
+                    device_ms=r.device_ms)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def schemes_array(schemes: Sequence[int]):
+    s = list(schemes) + [capi.SCHEME_NULL] * (2 - len(schemes))
+    return (C.c_int32 * 2)(*s)
+
+
+class Engine:
+    """Owner of one device context of libmyql_hip.so."""
+
+    def __init__(self, device: int = 0, lib_path: str = capi.LIB_PATH):
+        self.lib = capi.load_library(lib_path)
+        self.ctx = self.lib.myql_create(device)
+        if not self.ctx:
+            raise EngineError(capi.E_HIP, "myql_create failed (no usable HIP device?)")
+
+    def close(self):
+        if self.ctx:
+            self.lib.myql_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != capi.OK:
+            msg = self.lib.myql_last_error(self.ctx).decode()
+            raise EngineError(rc, msg)
+
+    def mc(self, model: Model, schemes, instr: InstrumentSpec, mode: int, cfg: MCConfig,
+           S0: float, r: float, q: float) -> dict:
+        s = instr.struct()
+        buf = ResultBuffers(s.n_strikes)
+        rc = self.lib.myql_mc_calculate(self.ctx, C.byref(model), schemes_array(schemes),
+                                        C.byref(s), mode, C.byref(cfg), S0, r, q,
+                                        C.byref(buf.struct))
+        self._check(rc)
+        return buf.as_dict()
+
+    def mc_batch(self, jobs: list) -> list:
+        """jobs: list of (model, schemes, instr_spec, mode, cfg, S0, r, q)."""
+        keep, arr = [], (capi.MCJob * len(jobs))()
+        bufs = []
+        for i, (model, schemes, instr, mode, cfg, S0, r, q) in enumerate(jobs):
+            s = instr.struct()
+            b = ResultBuffers(s.n_strikes)
+            keep.append((model, s, cfg, instr))
+            bufs.append(b)
+            j = arr[i]
+            j.model = C.pointer(model)
+            j.schemes = schemes_array(schemes)
+            j.greek_mode = mode
+            j.instrument = C.pointer(s)
+            j.config = C.pointer(cfg)
+            j.S0, j.r, j.q = S0, r, q
+            j.result = C.pointer(b.struct)
+        rc = self.lib.myql_mc_calculate_batch(self.ctx, arr, len(jobs))
+        self._check(rc)
+        return [b.as_dict() for b in bufs]
+
+    def mlmc(self, model: Model, schemes, instr: InstrumentSpec, mode: int, cfg: MLMCConfig,
+             S0: float, r: float, q: float) -> dict:
+        s = instr.struct()
+        buf = MLMCBuffers(int(cfg.L_max))
+        rc = self.lib.myql_mlmc_calculate(self.ctx, C.byref(model), schemes_array(schemes),
+                                          C.byref(s), mode, C.byref(cfg), S0, r, q,
+                                          C.byref(buf.struct))
+        self._check(rc)
+        return buf.as_dict()
+
+    def stream_draws(self, seed: int, layout: int, n_streams: int, n_per: int) -> np.ndarray:
+        out = np.zeros(n_streams * n_per, dtype=np.uint64)
+        rc = self.lib.myql_debug_stream_draws(self.ctx, seed, layout, n_streams, n_per,
+                                              out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        self._check(rc)
+        return out.reshape(n_streams, n_per)
+
+    def normcdfinv(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        o64 = np.zeros_like(p)
+        o32 = np.zeros(p.shape, dtype=np.float32)
+        pd = C.POINTER(C.c_double)
+        rc = self.lib.myql_debug_normcdfinv(self.ctx, p.ctypes.data_as(pd), p.size,
+                                            o64.ctypes.data_as(pd),
+                                            o32.ctypes.data_as(C.POINTER(C.c_float)))
+        self._check(rc)
+        return o64, o32
+
+    def poissinv(self, lam: np.ndarray, u: np.ndarray) -> np.ndarray:
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(lam)
+        pd = C.POINTER(C.c_double)
+        rc = self.lib.myql_debug_poissinv(self.ctx, lam.ctypes.data_as(pd), u.ctypes.data_as(pd),
+                                          lam.size, out.ctypes.data_as(pd))
+        self._check(rc)
+        return out
+
+    def chi2_quantile(self, dof: np.ndarray, p: np.ndarray) -> np.ndarray:
+        dof = np.ascontiguousarray(dof, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.zeros_like(dof)
+        pd = C.POINTER(C.c_double)
+        rc = self.lib.myql_debug_chi2_quantile(self.ctx, dof.ctypes.data_as(pd),
+                                               p.ctypes.data_as(pd), dof.size,
+                                               out.ctypes.data_as(pd))
+        self._check(rc)
+        return out
