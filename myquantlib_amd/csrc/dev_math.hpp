@@ -1,0 +1,405 @@
+// Device-side random numbers and special functions for the path kernels
+// (gfx950, 64-wide wavefronts). One xoshiro256++ stream lives in four 64-bit
+// registers of a lane; every transform below consumes draws exactly as the
+// reference's Boost distributions do (MonteCarloPricer.hpp:86,
+// ASVJstepper.hpp:137, JumpPolicies.hpp:38-39), so the number of draws per
+// path -- and hence the stream position of every later path -- matches the
+// FP64 oracle bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace myql {
+
+struct Rng {
+  uint64_t s0, s1, s2, s3;
+};
+
+__device__ __forceinline__ uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+// xoshiro256++: out = rotl(s0 + s3, 23) + s0 (Blackman & Vigna).
+__device__ __forceinline__ uint64_t next_u64(Rng &g) {
+  const uint64_t out = rotl(g.s0 + g.s3, 23) + g.s0;
+  const uint64_t t = g.s1 << 17;
+  g.s2 ^= g.s0;
+  g.s3 ^= g.s1;
+  g.s1 ^= g.s2;
+  g.s0 ^= g.s3;
+  g.s2 ^= t;
+  g.s3 = rotl(g.s3, 45);
+  return out;
+}
+
+// advance the state only (jump-ahead inner loop)
+__device__ __forceinline__ void step_state(Rng &g) {
+  const uint64_t t = g.s1 << 17;
+  g.s2 ^= g.s0;
+  g.s3 ^= g.s1;
+  g.s1 ^= g.s2;
+  g.s0 ^= g.s3;
+  g.s2 ^= t;
+  g.s3 = rotl(g.s3, 45);
+}
+
+// uniform_real_distribution<double>(0,1): u = double(x) / 2^64 rounded to
+// nearest, redrawn while u == 1 (probability 2^-54).
+__device__ __forceinline__ double uniform(Rng &g) {
+  for (;;) {
+    const double u = static_cast<double>(next_u64(g)) * 0x1p-64;
+    if (u < 1.0) return u;
+  }
+}
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t &x) {
+  x += 0x9E3779B97F4A7C15ull;
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ Rng seed_rng(uint64_t seed) {
+  Rng g;
+  uint64_t x = seed;
+  g.s0 = splitmix(x);
+  g.s1 = splitmix(x);
+  g.s2 = splitmix(x);
+  g.s3 = splitmix(x);
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Acklam's inverse normal CDF (reference Numerics.hpp:12-47). The branch is
+// always decided on the FP64 uniform so FP32 and FP64 paths take the same
+// region for the same draw; the FP32 form evaluates the rational functions in
+// single precision with hardware log2/sqrt/rcp.
+// ---------------------------------------------------------------------------
+#define MYQL_ACKLAM_A0 -3.969683028665376e+01
+#define MYQL_ACKLAM_A1 2.209460984245205e+02
+#define MYQL_ACKLAM_A2 -2.759285104469687e+02
+#define MYQL_ACKLAM_A3 1.383577518672690e+02
+#define MYQL_ACKLAM_A4 -3.066479806614716e+01
+#define MYQL_ACKLAM_A5 2.506628277459239e+00
+#define MYQL_ACKLAM_B0 -5.447609879822406e+01
+#define MYQL_ACKLAM_B1 1.615858368580409e+02
+#define MYQL_ACKLAM_B2 -1.556989798598866e+02
+#define MYQL_ACKLAM_B3 6.680131188771972e+01
+#define MYQL_ACKLAM_B4 -1.328068155288572e+01
+#define MYQL_ACKLAM_C0 -7.784894002430293e-03
+#define MYQL_ACKLAM_C1 -3.223964580411365e-01
+#define MYQL_ACKLAM_C2 -2.400758277161838e+00
+#define MYQL_ACKLAM_C3 -2.549732539343734e+00
+#define MYQL_ACKLAM_C4 4.374664141464968e+00
+#define MYQL_ACKLAM_C5 2.938163982698783e+00
+#define MYQL_ACKLAM_D0 7.784695709041462e-03
+#define MYQL_ACKLAM_D1 3.224671290700398e-01
+#define MYQL_ACKLAM_D2 2.445134137142996e+00
+#define MYQL_ACKLAM_D3 3.754408661907416e+00
+
+__device__ __forceinline__ double normcdfinv_d(double p) {
+  if (p > 0.02425 && p < 1.0 - 0.02425) {
+    const double q = p - 0.5, r = q * q;
+    const double num =
+        (((((MYQL_ACKLAM_A0 * r + MYQL_ACKLAM_A1) * r + MYQL_ACKLAM_A2) * r + MYQL_ACKLAM_A3) * r +
+          MYQL_ACKLAM_A4) * r + MYQL_ACKLAM_A5) * q;
+    const double den =
+        ((((MYQL_ACKLAM_B0 * r + MYQL_ACKLAM_B1) * r + MYQL_ACKLAM_B2) * r + MYQL_ACKLAM_B3) * r +
+         MYQL_ACKLAM_B4) * r + 1.0;
+    return num / den;
+  }
+  const bool lower = p <= 0.02425;
+  const double q = lower ? p : 1.0 - p;
+  const double r = sqrt(-2.0 * log(q));
+  const double x =
+      (((((MYQL_ACKLAM_C0 * r + MYQL_ACKLAM_C1) * r + MYQL_ACKLAM_C2) * r + MYQL_ACKLAM_C3) * r +
+        MYQL_ACKLAM_C4) * r + MYQL_ACKLAM_C5) /
+      ((((MYQL_ACKLAM_D0 * r + MYQL_ACKLAM_D1) * r + MYQL_ACKLAM_D2) * r + MYQL_ACKLAM_D3) * r + 1.0);
+  return lower ? x : -x;
+}
+
+__device__ __forceinline__ float normcdfinv_f(double p) {
+  if (p > 0.02425 && p < 1.0 - 0.02425) {
+    const float q = static_cast<float>(p - 0.5), r = q * q;
+    const float num =
+        (((((float(MYQL_ACKLAM_A0) * r + float(MYQL_ACKLAM_A1)) * r + float(MYQL_ACKLAM_A2)) * r +
+           float(MYQL_ACKLAM_A3)) * r + float(MYQL_ACKLAM_A4)) * r + float(MYQL_ACKLAM_A5)) * q;
+    const float den =
+        ((((float(MYQL_ACKLAM_B0) * r + float(MYQL_ACKLAM_B1)) * r + float(MYQL_ACKLAM_B2)) * r +
+          float(MYQL_ACKLAM_B3)) * r + float(MYQL_ACKLAM_B4)) * r + 1.0f;
+    return num * __builtin_amdgcn_rcpf(den);
+  }
+  const bool lower = p <= 0.02425;
+  // tail argument formed in FP64 so 1 - p keeps its precision near 1
+  const float q = static_cast<float>(lower ? p : 1.0 - p);
+  const float r = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(q));
+  const float x =
+      (((((float(MYQL_ACKLAM_C0) * r + float(MYQL_ACKLAM_C1)) * r + float(MYQL_ACKLAM_C2)) * r +
+         float(MYQL_ACKLAM_C3)) * r + float(MYQL_ACKLAM_C4)) * r + float(MYQL_ACKLAM_C5)) *
+      __builtin_amdgcn_rcpf(
+          (((float(MYQL_ACKLAM_D0) * r + float(MYQL_ACKLAM_D1)) * r + float(MYQL_ACKLAM_D2)) * r +
+           float(MYQL_ACKLAM_D3)) * r + 1.0f);
+  return lower ? x : -x;
+}
+
+template <typename R>
+__device__ __forceinline__ R normcdfinv(double p);
+template <>
+__device__ __forceinline__ double normcdfinv<double>(double p) { return normcdfinv_d(p); }
+template <>
+__device__ __forceinline__ float normcdfinv<float>(double p) { return normcdfinv_f(p); }
+
+// ---------------------------------------------------------------------------
+// Giles' inverse Poisson CDF (reference Numerics.hpp:49-272; ACM TOMS Alg.
+// 955) in FP64. Decides the integer N that fixes the CIR mixture of the NCI
+// and Exact schemes, so it runs in double on both path precisions.
+// ---------------------------------------------------------------------------
+__device__ static double poissinv(double lam, double U) {
+  if (U <= 0.0) return 0.0;
+  if (U >= 1.0) return __builtin_nan("");
+  const double V = 1.0 - U;
+  const double ilam = 1.0 / lam;
+  double X = 0.0, Del = 0.0;
+  if (lam > 4.0) {
+    double S;
+    double W = normcdfinv_d(fmin(U, V));
+    if (U > V) W = -W;
+    if (fabs(W) < 3.0) {
+      const double sl = sqrt(lam);
+      S = sl * W + (1.0 / 3.0 + (1.0 / 6.0) * W * W) * (1.0 - W / (12.0 * sl));
+      Del = 1.0 / 160.0;
+      Del = 1.0 / 80.0 + Del * (W * W);
+      Del = 1.0 / 40.0 + Del * (W * W);
+      Del = Del * ilam;
+      S = lam + (S + Del);
+    } else {
+      S = W / sqrt(lam);
+      double R = 1.0 + S;
+      if (R < 0.1) R = 0.1;
+      double Rp;
+      do {
+        const double Tl = log(R);
+        Rp = R;
+        double S2 = sqrt(2.0 * ((1.0 - R) + R * Tl));
+        if (R < 1.0) S2 = -S2;
+        R = Rp - (S2 - S) * S2 / Tl;
+        if (R < 0.1 * Rp) R = 0.1 * Rp;
+      } while (fabs(R - Rp) > 1e-8);
+      const double Tl = log(R);
+      S = lam * R + log(sqrt(2.0 * R * ((1.0 - R) + R * Tl)) / fabs(R - 1.0)) / Tl;
+      S = S - 0.0218 / (S + 0.065 * lam);
+      Del = 0.01 / S;
+      S = S + Del;
+    }
+    X = floor(S);
+    if (S > 10.0 && S < X + 2.0 * Del) {
+      if (X > 0.5 * lam && X < 2.0 * lam) {
+        const double Xi = 1.0 / X;
+        double Eta = X * ilam;
+        Eta = sqrt(2.0 * (1.0 - Eta + Eta * log(Eta)) / Eta);
+        if (X > lam) Eta = -Eta;
+        // Temme expansion coefficients: odd / even Horner chains in 1/X
+        const double b1[13] = {8.0995211567045583e-16,  -5.1391118342426808e-16, -1.3923887224181616e-13,
+                               1.1004392031956284e-13,  2.4361948020667402e-11,  -2.5514193994946487e-11,
+                               -4.3820360184533521e-09, 6.7078535434015332e-09,  8.2967113409530833e-07,
+                               -2.1854485106799979e-06, -0.00017875514403292177, 0.0011574074074074078,
+                               0.083333333333333329};
+        const double b0[13] = {-1.9752288294349411e-15, 2.8534893807047458e-14,  3.3717632624009806e-13,
+                               -5.0276692801141763e-12, -5.8307721325504166e-11, 9.1476995822367933e-10,
+                               1.0261809784240299e-08,  -1.7665952736826086e-07, -1.8540622107151585e-06,
+                               3.9192631785224383e-05,  0.00035273368606701921,  -0.014814814814814815,
+                               -0.33333333333333331};
+        double B1 = b1[0], Sm = B1, B0 = b0[0];
+        Sm = B0 + Sm * Eta;
+        double mult = 25.0;
+#pragma unroll
+        for (int j = 1; j < 13; ++j) {
+          B1 = b1[j] + mult * B1 * Xi;
+          Sm = B1 + Sm * Eta;
+          mult -= 1.0;
+          B0 = b0[j] + mult * B0 * Xi;
+          Sm = B0 + Sm * Eta;
+          mult -= 1.0;
+        }
+        Sm = Sm / (1.0 + B1 * Xi);
+        Sm = Sm * exp(-0.5 * X * Eta * Eta) / sqrt(2.0 * 3.141592653589793 * X);
+        if (X < lam) {
+          Sm += 0.5 * erfc(Eta * sqrt(0.5 * X));
+          if (Sm > U) X -= 1.0;
+        } else {
+          Sm -= 0.5 * erfc(-Eta * sqrt(0.5 * X));
+          if (Sm > -V) X -= 1.0;
+        }
+      } else {
+        double Xi = 1.0 / X;
+        double Sm = -(691.0 / 360360.0);
+        Sm = (1.0 / 1188.0) + Sm * Xi * Xi;
+        Sm = -(1.0 / 1680.0) + Sm * Xi * Xi;
+        Sm = (1.0 / 1260.0) + Sm * Xi * Xi;
+        Sm = -(1.0 / 360.0) + Sm * Xi * Xi;
+        Sm = (1.0 / 12.0) + Sm * Xi * Xi;
+        Sm = Sm * Xi;
+        Sm = (X - lam) - X * log(X * ilam) - Sm;
+        if (X < lam) {
+          double Tt = exp(-0.5 * Sm);
+          Sm = 1.0 - Tt * (U * Tt) * sqrt(2.0 * 3.141592653589793 * Xi) * lam;
+          Tt = 1.0;
+          Xi = X;
+          for (int i = 1; i < 50; ++i) {
+            Xi -= 1.0;
+            Tt *= Xi * ilam;
+            Sm += Tt;
+          }
+          if (Sm > 0.0) X -= 1.0;
+        } else {
+          const double Tt = exp(-0.5 * Sm);
+          Sm = 1.0 - Tt * (V * Tt) * sqrt(2.0 * 3.141592653589793 * X);
+          Xi = X;
+          for (int i = 0; i < 50; ++i) {
+            Xi += 1.0;
+            Sm = Sm * Xi * ilam + 1.0;
+          }
+          if (Sm < 0.0) X -= 1.0;
+        }
+      }
+    }
+  }
+  if (X < 10.0) {
+    X = 0.0;
+    double Tt = exp(0.5 * lam);
+    Del = 0.0;
+    if (U > 0.5) Del = Tt * (1e-13 * Tt);
+    double Sm = 1.0 - Tt * (U * Tt) + Del;
+    while (Sm < 0.0) {
+      X += 1.0;
+      Tt = X * ilam;
+      Del = Tt * Del;
+      Sm = Tt * Sm + 1.0;
+    }
+    if (Sm < 2.0 * Del) {
+      Del = 1e13 * Del;
+      Tt = 1e17 * Del;
+      Del = V * Del;
+      while (Del < Tt) {
+        X += 1.0;
+        Del *= X * ilam;
+      }
+      Sm = Del;
+      Tt = 1.0;
+      while (Sm > 0.0) {
+        Tt *= X * ilam;
+        Sm -= Tt;
+        X -= 1.0;
+      }
+    }
+  }
+  return X;
+}
+
+// Poisson(mean) by sequential inversion on one uniform (Boost's method for
+// means below 10; JumpPolicies.hpp:60,116).
+__device__ __forceinline__ int poisson_count(double mean, double exp_mean, double u) {
+  double p = exp_mean;
+  int x = 0;
+  while (u > p) {
+    u -= p;
+    ++x;
+    p = mean * p / x;
+    if (p == 0.0) break;
+  }
+  return x;
+}
+
+// Exponential(rate) by inversion (replaces Boost's ziggurat: one draw each).
+template <typename R>
+__device__ __forceinline__ R exponential(R rate, Rng &g) {
+  return static_cast<R>(-log1p(-uniform(g))) / rate;
+}
+
+// ---------------------------------------------------------------------------
+// Regularised incomplete gamma and its inverse (chi-squared quantile of the
+// NCI scheme: table rows and the out-of-table fallback, VolSchemes.hpp:282,
+// 368-369). Power series below a+1, Legendre continued fraction (Lentz)
+// above; Halley steps inside a shrinking bracket.
+// ---------------------------------------------------------------------------
+__device__ static double igam_series(double a, double x) {
+  double sum = 1.0, term = 1.0, ap = a;
+  for (int n = 1; n < 2000; ++n) {
+    ap += 1.0;
+    term *= x / ap;
+    sum += term;
+    if (term < sum * 1e-17) break;
+  }
+  return exp(a * log(x) - x - lgamma(a + 1.0)) * sum;
+}
+
+__device__ static double igamc_cf(double a, double x) {
+  const double tiny = 1e-300;
+  double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int i = 1; i < 2000; ++i) {
+    const double an = -i * (i - a);
+    b += 2.0;
+    d = an * d + b;
+    if (fabs(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < 1e-16) break;
+  }
+  return exp(a * log(x) - x - lgamma(a)) * h;
+}
+
+__device__ static double chi2_quantile(double dof, double p) {
+  if (p <= 0.0) return 0.0;
+  if (p >= 1.0) return __builtin_inf();
+  const double a = 0.5 * dof;
+  const bool upper = p > 0.5;
+  const double q = 1.0 - p;
+  const double lga = lgamma(a);
+  double x;
+  if (a > 1.0) {
+    const double z = normcdfinv_d(p);
+    const double t = 1.0 / (9.0 * a);
+    const double w = 1.0 - t + z * sqrt(t);
+    x = (w > 0.0) ? a * w * w * w : 0.5 * a;
+  } else {
+    const double xl = exp((log(p) + lgamma(a + 1.0)) / a);
+    if (!upper && xl < 1.0) {
+      x = xl;
+    } else {
+      double xx = -log(q) - lga;
+      if (xx < 1.0) xx = 1.0;
+      for (int i = 0; i < 4; ++i) xx = -log(q) - lga + (a - 1.0) * log(xx);
+      x = (xx > 0.0) ? xx : 1.0;
+    }
+  }
+  if (!(x > 0.0)) x = 1e-300;
+  double lo = 0.0, hi = __builtin_inf();
+  for (int it = 0; it < 200; ++it) {
+    double f;
+    if (upper) f = q - ((x < a + 1.0) ? 1.0 - igam_series(a, x) : igamc_cf(a, x));
+    else f = ((x < a + 1.0) ? igam_series(a, x) : 1.0 - igamc_cf(a, x)) - p;
+    if (f > 0.0) hi = x; else lo = x;
+    if (f == 0.0) break;
+    const double pdf = exp((a - 1.0) * log(x) - x - lga);
+    double xn = -1.0;
+    if (pdf > 0.0 && isfinite(pdf)) {
+      const double r = f / pdf;
+      double den = 1.0 - 0.5 * r * ((a - 1.0) / x - 1.0);
+      if (den <= 0.1) den = 1.0;
+      xn = x - r / den;
+    }
+    if (!(xn > lo && xn < hi)) {
+      if (isinf(hi)) xn = (x > 0.0 ? 2.0 * x : 1.0);
+      else if (lo > 0.0 && hi / lo > 4.0) xn = sqrt(lo * hi);
+      else xn = 0.5 * (lo + hi);
+    }
+    const double dx = fabs(xn - x);
+    x = xn;
+    if (dx <= 2e-16 * x) break;
+  }
+  return 2.0 * x;
+}
+
+}  // namespace myql

@@ -1,0 +1,1233 @@
+// Host side of the MI355X Monte Carlo engine: the C-ABI of
+// include/myql_capi.h, device context and caches, and the translation of a
+// MonteCarloPricer::calculate call (reference MonteCarloPricer.hpp:95-185)
+// into kernel parameter blocks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <bitset>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/myql_capi.h"
+#include "dev_math.hpp"
+#include "engine_types.hpp"
+#include "mc_launch.hpp"
+#include "mlmc_kernel.hpp"
+
+using namespace myql;
+
+namespace {
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                     \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw Fail(e_ == hipErrorOutOfMemory ? MYQL_E_OOM : MYQL_E_HIP,                      \
+                 std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+// growable device buffer
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  void reserve(size_t bytes) {
+    if (bytes <= n) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    HIPCHECK(hipMalloc(&p, bytes));
+    n = bytes;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Jump-ahead polynomials of xoshiro256: J_k(x) = x^(2^(128+k)) mod P(x),
+// P the characteristic polynomial of the state transition, found from a bit
+// sequence of the generator by the Berlekamp-Massey recurrence over GF(2).
+// ---------------------------------------------------------------------------
+using Bits = std::bitset<512>;
+
+void host_step(uint64_t s[4]) {
+  const uint64_t t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = (s[3] << 45) | (s[3] >> 19);
+}
+
+Bits transition_charpoly() {
+  const int n = 512;
+  uint64_t s[4] = {0x243F6A8885A308D3ull, 0x13198A2E03707344ull, 0xA4093822299F31D0ull,
+                   0x082EFA98EC4E6C89ull};
+  Bits seq;
+  for (int i = 0; i < n; ++i) {
+    seq[i] = (s[3] >> 7) & 1;
+    host_step(s);
+  }
+  Bits c, b;  // connection polynomial and its last copy
+  c[0] = b[0] = 1;
+  int L = 0, shift = 1;
+  for (int i = 0; i < n; ++i) {
+    bool disc = seq[i];
+    for (int j = 1; j <= L; ++j) disc ^= (c[j] && seq[i - j]);
+    if (!disc) {
+      ++shift;
+      continue;
+    }
+    const Bits prev = c;
+    c ^= (b << shift);
+    if (2 * L <= i) {
+      L = i + 1 - L;
+      b = prev;
+      shift = 1;
+    } else {
+      ++shift;
+    }
+  }
+  Bits P;  // reciprocal of c: x^L c(1/x)
+  for (int j = 0; j <= L; ++j) P[j] = c[L - j];
+  return P;
+}
+
+Bits sqr_mod(const Bits &a, const Bits &P, int deg) {
+  Bits r;
+  for (int i = 0; i < deg; ++i)
+    if (a[i]) r[2 * i] = 1;
+  for (int i = 2 * deg - 2; i >= deg; --i)
+    if (r[i]) r ^= (P << (i - deg));
+  return r;
+}
+
+std::vector<uint64_t> host_jump_polys(int nk) {
+  const Bits P = transition_charpoly();
+  int deg = 511;
+  while (deg > 0 && !P[deg]) --deg;
+  Bits r;
+  r[1] = 1;
+  for (int i = 0; i < 128; ++i) r = sqr_mod(r, P, deg);
+  std::vector<uint64_t> out(4 * nk);
+  for (int k = 0; k < nk; ++k) {
+    for (int w = 0; w < 4; ++w) {
+      uint64_t x = 0;
+      for (int b = 0; b < 64; ++b)
+        if (r[64 * w + b]) x |= (1ull << b);
+      out[4 * k + w] = x;
+    }
+    r = sqr_mod(r, P, deg);
+  }
+  return out;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+namespace myql {
+
+__device__ void apply_jump(Rng &g, const uint64_t *__restrict__ J) {
+  Rng a = {0, 0, 0, 0};
+  for (int w = 0; w < 4; ++w) {
+    const uint64_t word = J[w];
+    for (int b = 0; b < 64; ++b) {
+      if ((word >> b) & 1ull) {
+        a.s0 ^= g.s0;
+        a.s1 ^= g.s1;
+        a.s2 ^= g.s2;
+        a.s3 ^= g.s3;
+      }
+      step_state(g);
+    }
+  }
+  g = a;
+}
+
+__global__ void stream_states_kernel(uint64_t seed, int layout, uint64_t S, const uint64_t *__restrict__ polys,
+                                     uint64_t *__restrict__ out) {
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  Rng g;
+  if (layout == MYQL_STREAMS_SEEDED) {
+    g = seed_rng(seed + s * 999983ull);
+  } else {
+    g = seed_rng(seed);
+    for (int k = 0; k < 64; ++k) {
+      if ((s >> k) == 0) break;
+      if ((s >> k) & 1ull) apply_jump(g, polys + 4 * k);
+    }
+  }
+  out[4 * s + 0] = g.s0;
+  out[4 * s + 1] = g.s1;
+  out[4 * s + 2] = g.s2;
+  out[4 * s + 3] = g.s3;
+}
+
+__global__ void nci_quantiles_kernel(double d, int rows, int grid, double dx, double *__restrict__ y) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<int64_t>(rows) * grid) return;
+  const int k = static_cast<int>(e / grid), i = static_cast<int>(e % grid);
+  y[e] = chi2_quantile(d + 2.0 * k, static_cast<double>(i) * dx);
+}
+
+// Fritsch-Butland slopes (VolSchemes.hpp:292-310) and the interleaved
+// {y, dx*m} layout the path kernels gather from.
+template <typename R>
+__global__ void nci_pack_kernel(const double *__restrict__ y, int rows, int grid, double dx, double inv_dx,
+                                R *__restrict__ tab) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<int64_t>(rows) * grid) return;
+  const int i = static_cast<int>(e % grid);
+  const double *r = y + (e - i);
+  double m;
+  if (i == 0) m = (r[1] - r[0]) * inv_dx;
+  else if (i == grid - 1) m = (r[i] - r[i - 1]) * inv_dx;
+  else {
+    const double d1 = (r[i] - r[i - 1]) * inv_dx, d2 = (r[i + 1] - r[i]) * inv_dx;
+    m = (d1 * d2 > 0.0) ? 2.0 * d1 * d2 / (d1 + d2) : 0.0;
+  }
+  tab[2 * e] = static_cast<R>(r[i]);
+  tab[2 * e + 1] = static_cast<R>(dx * m);
+}
+
+// Sum of the unit partials of each job in unit order.
+__global__ void reduce_partials_kernel(const double *__restrict__ partial, double *__restrict__ sums,
+                                       const long long *__restrict__ base, const int *__restrict__ n_slots,
+                                       const int *__restrict__ nk, const int *__restrict__ out_off,
+                                       int n_jobs) {
+  const int job = blockIdx.y;
+  if (job >= n_jobs) return;
+  const int width = 2 * NSTAT * nk[job];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= width) return;
+  double t = 0.0;
+  const double *p = partial + base[job] + e;
+  for (int i = 0; i < n_slots[job]; ++i) t += p[static_cast<size_t>(i) * width];
+  sums[out_off[job] + e] = t;
+}
+
+
+__global__ void debug_draws_kernel(const uint64_t *__restrict__ states, uint64_t S, uint64_t n,
+                                   uint64_t *__restrict__ out) {
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  Rng g = {states[4 * s], states[4 * s + 1], states[4 * s + 2], states[4 * s + 3]};
+  for (uint64_t i = 0; i < n; ++i) out[s * n + i] = next_u64(g);
+}
+
+__global__ void debug_normcdfinv_kernel(const double *__restrict__ p, int64_t n, double *__restrict__ o64,
+                                        float *__restrict__ o32) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  o64[i] = normcdfinv_d(p[i]);
+  o32[i] = normcdfinv_f(p[i]);
+}
+
+__global__ void debug_poissinv_kernel(const double *__restrict__ lam, const double *__restrict__ u, int64_t n,
+                                      double *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = poissinv(lam[i], u[i]);
+}
+
+__global__ void debug_chi2_kernel(const double *__restrict__ dof, const double *__restrict__ p, int64_t n,
+                                  double *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = chi2_quantile(dof[i], p[i]);
+}
+
+}  // namespace myql
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct myql_ctx {
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::vector<uint64_t> jump_polys;
+  DevBuf d_polys;
+  std::map<std::tuple<uint64_t, int, uint64_t>, std::unique_ptr<DevBuf>> streams;
+  std::map<std::tuple<uint64_t, int, int>, std::unique_ptr<DevBuf>> nci;  // (d bits, rows, prec)
+  DevBuf jobs, units, partial, sums, meta, counter, tmp;
+  DevBuf mlmc_streams, mlmc_sums, mlmc_job;
+};
+
+namespace {
+
+void set_device(myql_ctx *c) { HIPCHECK(hipSetDevice(c->device)); }
+
+// S stream states for (seed, layout), cached per context.
+const uint64_t *stream_states(myql_ctx *c, uint64_t seed, int layout, uint64_t S) {
+  auto key = std::make_tuple(seed, layout, S);
+  auto it = c->streams.find(key);
+  if (it != c->streams.end()) return it->second->as<uint64_t>();
+  if (c->streams.size() > 16) c->streams.clear();
+  auto buf = std::make_unique<DevBuf>();
+  buf->reserve(S * 32);
+  const int tb = 256;
+  const unsigned nb = static_cast<unsigned>((S + tb - 1) / tb);
+  hipLaunchKernelGGL(stream_states_kernel, dim3(nb), dim3(tb), 0, c->stream, seed, layout, S,
+                     c->d_polys.as<uint64_t>(), buf->as<uint64_t>());
+  HIPCHECK(hipGetLastError());
+  const uint64_t *p = buf->as<uint64_t>();
+  c->streams[key] = std::move(buf);
+  return p;
+}
+
+struct NciInfo {
+  const void *tab;
+  int rows, grid;
+  double dx, inv_dx;
+};
+
+constexpr int NCI_GRID = 10000;
+
+// NCI row count for one workspace (VolSchemes.hpp:245-255)
+int nci_nmax(const myql_heston_params &p, double dt) {
+  const double v_max = std::max(p.v0, 3.0 * p.theta);
+  const double ex = std::exp(-p.kappa * dt);
+  const double psi = (std::fabs(p.kappa) < 1e-10) ? dt : (1.0 - ex) / p.kappa;
+  const double lam_max = 2.0 * v_max * ex / (p.sigma * p.sigma * psi);
+  double nm = std::ceil(lam_max + 4.0 * std::sqrt(lam_max));
+  size_t n = static_cast<size_t>(nm);
+  n = std::max<size_t>(n, 23);
+  n = std::min<size_t>(n, 1023);
+  return static_cast<int>(n);
+}
+
+NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec) {
+  const int rows = nmax + 1;
+  uint64_t dbits;
+  std::memcpy(&dbits, &d, 8);
+  const double dx = 0.999 / static_cast<double>(NCI_GRID - 1);
+  const double inv_dx = 1.0 / dx;
+  // a cached table with at least as many rows serves as well
+  for (auto &kv : c->nci)
+    if (std::get<0>(kv.first) == dbits && std::get<2>(kv.first) == prec && std::get<1>(kv.first) >= rows)
+      return {kv.second->p, std::get<1>(kv.first), NCI_GRID, dx, inv_dx};
+  if (c->nci.size() > 32) c->nci.clear();
+  const int64_t n = static_cast<int64_t>(rows) * NCI_GRID;
+  c->tmp.reserve(n * sizeof(double));
+  auto buf = std::make_unique<DevBuf>();
+  buf->reserve(n * 2 * (prec == MYQL_FP32 ? 4 : 8));
+  const int tb = 256;
+  const unsigned nb = static_cast<unsigned>((n + tb - 1) / tb);
+  hipLaunchKernelGGL(nci_quantiles_kernel, dim3(nb), dim3(tb), 0, c->stream, d, rows, NCI_GRID, dx,
+                     c->tmp.as<double>());
+  HIPCHECK(hipGetLastError());
+  if (prec == MYQL_FP32)
+    hipLaunchKernelGGL(nci_pack_kernel<float>, dim3(nb), dim3(tb), 0, c->stream, c->tmp.as<double>(), rows,
+                       NCI_GRID, dx, inv_dx, buf->as<float>());
+  else
+    hipLaunchKernelGGL(nci_pack_kernel<double>, dim3(nb), dim3(tb), 0, c->stream, c->tmp.as<double>(), rows,
+                       NCI_GRID, dx, inv_dx, buf->as<double>());
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));  // the scratch rows are reused by the next table
+  const void *p = buf->p;
+  c->nci[std::make_tuple(dbits, rows, prec)] = std::move(buf);
+  return {p, rows, NCI_GRID, dx, inv_dx};
+}
+
+// ---------------------------------------------------------------------------
+// job building
+// ---------------------------------------------------------------------------
+double expected_avg_variance(const myql_model &m, double T) {
+  auto h = [&](const myql_heston_params &p) {
+    if (std::fabs(p.kappa * T) > 1e-8) return p.theta + (p.v0 - p.theta) * (1.0 - std::exp(-p.kappa * T)) / (p.kappa * T);
+    return p.v0;
+  };
+  double v = 0.0;
+  if (m.n_factors == 0) v += m.vol * m.vol;
+  else if (m.n_factors == 1) v += h(m.heston[0]);
+  else v += h(m.heston[0]) + h(m.heston[1]);
+  if (m.jump_kind == MYQL_JUMP_MERTON) v += m.jump[0] * (m.jump[1] * m.jump[1] + m.jump[2] * m.jump[2]);
+  else if (m.jump_kind == MYQL_JUMP_KOU)
+    v += m.jump[0] * (m.jump[1] / (m.jump[2] * m.jump[2]) + (1.0 - m.jump[1]) / (m.jump[3] * m.jump[3]));
+  return v;
+}
+
+void validate(const myql_model *m, const int32_t *schemes, const myql_instrument *in, const myql_mc_config *cfg) {
+  if (!m || !schemes || !in || !cfg) throw Fail(MYQL_E_INVALID_ARG, "null argument");
+  if (m->n_factors < 0 || m->n_factors > 2) throw Fail(MYQL_E_INVALID_ARG, "n_factors must be 0, 1 or 2");
+  for (int i = 0; i < m->n_factors; ++i)
+    if (schemes[i] < MYQL_SCHEME_NV || schemes[i] > MYQL_SCHEME_EXACT)
+      throw Fail(MYQL_E_INVALID_ARG, "scheme must be NV, NCI or Exact for each variance factor");
+  if (m->jump_kind < MYQL_JUMP_NONE || m->jump_kind > MYQL_JUMP_KOU) throw Fail(MYQL_E_INVALID_ARG, "bad jump kind");
+  if (in->n_strikes < 1 || !in->strikes) throw Fail(MYQL_E_INVALID_ARG, "instrument needs strikes");
+  if (in->kind < MYQL_INSTR_EUROPEAN || in->kind > MYQL_INSTR_BARRIER) throw Fail(MYQL_E_INVALID_ARG, "bad instrument kind");
+  if (!(in->maturity > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "maturity must be positive");
+  if (cfg->num_paths < 1 || cfg->time_steps < 1) throw Fail(MYQL_E_INVALID_ARG, "num_paths and time_steps must be >= 1");
+  if (cfg->time_steps > (1ull << 30)) throw Fail(MYQL_E_INVALID_ARG, "time_steps too large");
+  if (cfg->precision != MYQL_FP64 && cfg->precision != MYQL_FP32) throw Fail(MYQL_E_INVALID_ARG, "bad precision");
+}
+
+// Stepper constants of one variant (ASVJstepper.hpp:85-117, init_cir_impl
+// :262-283, Scheme::prepare, JumpPolicy::prepare).
+template <typename R>
+void fill_variant(VarParams<R> &V, const myql_model &m, const int32_t *schemes, double dt, double r, double q,
+                  double total_time, uint64_t steps, bool pathdep, const int nmax[2]) {
+  std::memset(&V, 0, sizeof(V));
+  V.dt = static_cast<R>(dt);
+  V.total_time = static_cast<R>(static_cast<double>(steps) * dt);
+  if (m.n_factors == 0) {
+    const double var = m.vol * m.vol;
+    const double gd = (r - q - 0.5 * var) * dt, gv = m.vol * std::sqrt(dt);
+    V.gdrift = static_cast<R>(gd);
+    V.gvol = static_cast<R>(gv);
+    V.gdrift_n = static_cast<R>(gd * static_cast<double>(steps));
+    V.gvol_n = static_cast<R>(gv * std::sqrt(static_cast<double>(steps)));
+  } else {
+    const double r_part = (r - q) / static_cast<double>(m.n_factors);
+    for (int f = 0; f < m.n_factors; ++f) {
+      const myql_heston_params &p = m.heston[f];
+      const double h = dt;
+      const double C0 = (r_part - p.rho * p.kappa * p.theta / p.sigma) * h;
+      const double term = (p.kappa * p.rho / p.sigma - 0.5) * h * 0.5;
+      V.C[f][0] = static_cast<R>(C0);
+      V.C[f][1] = static_cast<R>(term - p.rho / p.sigma);
+      V.C[f][2] = static_cast<R>(term + p.rho / p.sigma);
+      V.C[f][3] = static_cast<R>(h * (1.0 - p.rho * p.rho) * 0.5);
+      V.v0[f] = static_cast<R>(p.v0);
+      double K[3] = {0, 0, 0};
+      if (schemes[f] == MYQL_SCHEME_NV) {
+        if (p.sigma * p.sigma > 4.0 * p.kappa * p.theta)
+          throw Fail(MYQL_E_NV_FELLER, "SchemeNV violation: sigma^2 > 4*kappa*theta");
+        const double ex = std::exp(-p.kappa * h * 0.5);
+        const double psi = (std::fabs(p.kappa) < 1e-10) ? h * 0.5 : (1.0 - ex) / p.kappa;
+        K[0] = ex;
+        K[1] = (p.kappa * p.theta - p.sigma * p.sigma * 0.25) * psi;
+        K[2] = p.sigma * std::sqrt(h) * 0.5;
+      } else {
+        const double ex = std::exp(-p.kappa * h);
+        const double psi = (std::fabs(p.kappa) < 1e-10) ? h : (1.0 - ex) / p.kappa;
+        K[0] = (2.0 / (p.sigma * p.sigma * psi)) * ex;
+        if (schemes[f] == MYQL_SCHEME_NCI) {
+          K[1] = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
+          K[2] = 0.25 * p.sigma * p.sigma * psi;
+        } else {
+          K[1] = 2.0 * p.kappa * p.theta / (p.sigma * p.sigma);
+          K[2] = 0.5 * p.sigma * p.sigma * psi;
+        }
+      }
+      for (int i = 0; i < 3; ++i) {
+        V.Kd[f][i] = K[i];
+        V.Kr[f][i] = static_cast<R>(K[i]);
+      }
+      V.nmax[f] = nmax[f];
+    }
+  }
+  const double span = pathdep ? dt : total_time;
+  if (m.jump_kind == MYQL_JUMP_MERTON) {
+    const double lam = m.jump[0], mu = m.jump[1], dl = m.jump[2];
+    V.jmean = lam * span;
+    V.jexpm = std::exp(-V.jmean);
+    V.jdrift = static_cast<R>(-lam * (std::exp(mu + 0.5 * dl * dl) - 1.0) * span);
+  } else if (m.jump_kind == MYQL_JUMP_KOU) {
+    const double lam = m.jump[0], p = m.jump[1], e1 = m.jump[2], e2 = m.jump[3];
+    const double k = (p * e1) / (e1 - 1.0) + ((1.0 - p) * e2) / (e2 + 1.0) - 1.0;
+    V.jmean = lam * span;
+    V.jexpm = std::exp(-V.jmean);
+    V.jdrift = static_cast<R>(-lam * k * span);
+  } else {
+    V.jmean = 0.0;
+    V.jexpm = 1.0;
+  }
+}
+
+struct JobSpec {
+  const myql_model *model;
+  const int32_t *schemes;
+  const myql_instrument *instr;
+  int mode;
+  const myql_mc_config *cfg;
+  double S0, r, q;
+  myql_mc_result *res;
+  int k0, nk;  // strike chunk [k0, k0+nk) of the instrument
+};
+
+struct BuiltJob {
+  int nf, nv;        // kernel instantiation
+  int n_slots;       // work units
+  uint64_t cost;     // per-unit cost estimate
+  int nk;
+  JobSpec spec;
+  double df;
+  uint64_t variants_alg;
+};
+
+int pow2ceil(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// Model, instrument and tracker fields of a job (everything but the
+// variants): get_tracker_config<Mode> and the payoff, with the automatic
+// digital smoothing of MonteCarloPricer.hpp:51-74 when `auto_smooth`.
+template <typename R>
+void fill_instrument(Job<R> &J, const myql_model &M0, const int32_t *schemes, const myql_instrument &in, int mode,
+                     double S0, double r, double fd_bump, bool auto_smooth) {
+  const bool ess = mode != MYQL_GREEK_NONE;
+  const int nf = M0.n_factors;
+  const bool pd = in.kind != MYQL_INSTR_EUROPEAN;
+  const double T = in.maturity;
+  const double h = S0 * fd_bump;
+  (void)r;
+  std::memset(&J, 0, sizeof(J));
+  J.nf = nf;
+  J.scheme[0] = nf > 0 ? schemes[0] : 0;
+  J.scheme[1] = nf > 1 ? schemes[1] : 0;
+  J.jump_kind = M0.jump_kind;
+  J.pathdep = pd;
+  J.jmu = static_cast<R>(M0.jump[1]);
+  J.jdelta = static_cast<R>(M0.jump[2]);
+  J.kp = static_cast<R>(M0.jump[1]);
+  J.keta1 = static_cast<R>(M0.jump[2]);
+  J.keta2 = static_cast<R>(M0.jump[3]);
+  J.tracker = in.kind;
+  J.payoff = in.payoff;
+  J.call = in.option_type == MYQL_CALL;
+  J.fixed_strike = in.fixed_strike;
+  J.barrier_up = in.barrier_dir == MYQL_BARRIER_UP;
+  J.barrier_in = in.barrier_action == MYQL_BARRIER_IN;
+  J.rebate_timing = in.rebate_timing;
+  J.mode = mode;
+  J.S0 = S0;
+  J.logS0 = std::log(S0);
+  J.mult_up = ess ? (S0 + h) / S0 : 1.0;
+  J.mult_dn = ess ? (S0 - h) / S0 : 1.0;
+  J.inv2h = 1.0 / (2.0 * h);
+  J.invh2 = 1.0 / (h * h);
+  if (in.kind == MYQL_INSTR_BARRIER) {
+    const double bb = std::log(in.barrier);
+    J.b_base = static_cast<R>(bb);
+    J.b_up = static_cast<R>(ess ? std::log(in.barrier * S0 / (S0 + h)) : bb);
+    J.b_dn = static_cast<R>(ess ? std::log(in.barrier * S0 / (S0 - h)) : bb);
+  }
+  J.init_sumlog = in.past_avg_log * in.t_elapsed;
+  J.init_sumS = in.past_avg_S * in.t_elapsed;
+  J.hist_min_logS = in.hist_min_S > 0.0 ? std::log(in.hist_min_S) : 1e300;
+  J.hist_max_logS = in.hist_max_S > 0.0 ? std::log(in.hist_max_S) : -1e300;
+  J.t_elapsed = in.t_elapsed;
+  J.maturity = T;
+  J.rebate = in.rebate;
+  J.rebate_rate = in.rebate_rate;
+  // automatic digital smoothing (MonteCarloPricer.hpp:51-74)
+  double eps = in.payoff_eps;
+  if (auto_smooth && ess && in.payoff != MYQL_PAYOFF_VANILLA && eps < 0.0) {
+    const double sig = std::sqrt(expected_avg_variance(M0, T));
+    const double scale = in.is_strip ? S0 : in.strikes[0];
+    eps = sig * scale * std::sqrt(T) / 10.0;
+  }
+  J.eps_scale = 0.70710678118654752440 / eps;
+  J.n_fields = in.kind == MYQL_INSTR_EUROPEAN ? 1
+               : in.kind != MYQL_INSTR_BARRIER ? 2
+               : (in.rebate_timing == MYQL_REBATE_AT_HIT ? NFIELD : 3);
+}
+
+template <typename R>
+BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double> &strike_store) {
+  const myql_model &M0 = *js.model;
+  const myql_instrument &in = *js.instr;
+  const myql_mc_config &cfg = *js.cfg;
+  const int mode = js.mode;
+  const bool full = mode == MYQL_GREEK_FULL;
+  const int nf = M0.n_factors;
+  const bool pd = in.kind != MYQL_INSTR_EUROPEAN;
+  const double S0 = js.S0, r = js.r, q = js.q;
+  const double T = in.maturity;
+  const uint64_t steps = cfg.time_steps, M = cfg.num_paths;
+  const double dt = T / static_cast<double>(steps);
+
+  fill_instrument(J, M0, js.schemes, in, mode, S0, r, cfg.fd_bump, true);
+
+  // Greek bump constants (MonteCarloPricer.hpp:297-300,384-399)
+  J.inv2dv = 1.0 / (2.0 * cfg.vol_bump);
+  J.inv2dT = 1.0 / (2.0 * cfg.t_bump);
+  J.inv2dr = 1.0 / (2.0 * cfg.r_bump);
+  J.cv[0] = nf > 0 ? 2.0 * std::sqrt(M0.heston[0].v0) : 1.0;
+  J.cv[1] = nf > 1 ? 2.0 * std::sqrt(M0.heston[1].v0) : 1.0;
+  J.dfTu = std::exp(-r * cfg.t_bump);
+  J.dfTd = std::exp(r * cfg.t_bump);
+  J.dfru = std::exp(-cfg.r_bump * T);
+  J.dfrd = std::exp(cfg.r_bump * T);
+  J.erb_up = std::exp(cfg.r_bump * T);
+  J.erb_dn = std::exp(-cfg.r_bump * T);
+  J.nvega = std::max(nf, 1);
+
+  // variants: models, step sizes, rates, maturities
+  struct VDef {
+    int id;
+    myql_model m;
+    double dt, r, T;
+  };
+  std::vector<VDef> vd;
+  vd.push_back({V_BASE, M0, dt, r, T});
+  if (full) {
+    for (int f = 0; f < J.nvega; ++f) {
+      myql_model mu = M0, md = M0;
+      if (nf == 0) {
+        mu.vol += cfg.vol_bump;
+        md.vol -= cfg.vol_bump;
+      } else {
+        mu.heston[f].v0 += cfg.vol_bump;
+        md.heston[f].v0 -= cfg.vol_bump;
+      }
+      vd.push_back({V_V0UP + 2 * f, mu, dt, r, T});
+      vd.push_back({V_V0DN + 2 * f, md, dt, r, T});
+    }
+    const double Tu = T + cfg.t_bump, Td = T - cfg.t_bump;
+    vd.push_back({V_TUP, M0, Tu / steps, r, Tu});
+    vd.push_back({V_TDN, M0, Td / steps, r, Td});
+    if (pd) {
+      vd.push_back({V_RUP, M0, dt, r + cfg.r_bump, T});
+      vd.push_back({V_RDN, M0, dt, r - cfg.r_bump, T});
+    }
+  }
+  J.derive_r = full && !pd;
+
+  // NCI tables: one per factor, rows for the largest workspace among variants
+  int nmax_max[2] = {0, 0};
+  std::vector<std::array<int, 2>> nmaxv(vd.size(), {0, 0});
+  for (size_t i = 0; i < vd.size(); ++i)
+    for (int f = 0; f < nf; ++f)
+      if (js.schemes[f] == MYQL_SCHEME_NCI) {
+        nmaxv[i][f] = nci_nmax(vd[i].m.heston[f], vd[i].dt);
+        nmax_max[f] = std::max(nmax_max[f], nmaxv[i][f]);
+      }
+  for (int f = 0; f < nf; ++f)
+    if (js.schemes[f] == MYQL_SCHEME_NCI) {
+      const myql_heston_params &p = M0.heston[f];
+      const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
+      NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision);
+      J.nci[f] = ni.tab;
+      J.nci_grid = ni.grid;
+      J.nci_dx = ni.dx;
+      J.nci_inv_dx = ni.inv_dx;
+    }
+  for (size_t i = 0; i < vd.size(); ++i)
+    fill_variant(J.var[vd[i].id], vd[i].m, js.schemes, vd[i].dt, vd[i].r, q, vd[i].T, steps, pd,
+                 nmaxv[i].data());
+
+  // grouping: lockstep when every variant consumes the same draws
+  bool lockstep = true;
+  for (int f = 0; f < nf; ++f)
+    if (js.schemes[f] == MYQL_SCHEME_EXACT) lockstep = false;
+  if (pd && M0.jump_kind != MYQL_JUMP_NONE) lockstep = false;
+  const int nsim = static_cast<int>(vd.size());
+  for (int i = 0; i < nsim; ++i) {
+    J.var_list[i] = vd[i].id;
+    J.staged[vd[i].id] = 1;
+  }
+  int nv;
+  if (lockstep && nsim > 1) {
+    nv = nsim;
+    J.n_groups = 1;
+    J.group_first[0] = 0;
+    J.group_size[0] = nsim;
+  } else {
+    nv = 1;
+    J.n_groups = nsim;
+    for (int i = 0; i < nsim; ++i) {
+      J.group_first[i] = i;
+      J.group_size[i] = 1;
+    }
+  }
+  if (!mc_supported(nf, nv)) throw Fail(MYQL_E_UNSUPPORTED, "no kernel for this factor count / variant set");
+  uint64_t S = cfg.num_streams ? cfg.num_streams : std::min<uint64_t>(65536, M);
+  if (S > M) S = M;
+  J.M = M;
+  J.S = S;
+  J.steps = steps;
+  J.streams = stream_states(c, cfg.seed, cfg.stream_layout, S);
+
+  // strikes of this chunk
+  J.nk = js.nk;
+  J.nkp = pow2ceil(js.nk);
+  const size_t off = strike_store.size();
+  for (int k = 0; k < js.nk; ++k) strike_store.push_back(in.strikes[js.k0 + k]);
+  J.strikes = reinterpret_cast<const double *>(off);  // rebased after upload
+
+  BuiltJob b;
+  b.nf = nf;
+  b.nv = nv;
+  b.n_slots = static_cast<int>((S + BLOCK - 1) / BLOCK);
+  const uint64_t per_stream = (M + S - 1) / S;
+  b.cost = per_stream * steps * static_cast<uint64_t>(nsim) * (nf == 0 ? 1 : nf) + per_stream * js.nk;
+  b.nk = js.nk;
+  b.spec = js;
+  b.df = std::exp(-r * T);
+  b.variants_alg = full ? (nf == 2 ? 9 : 7) : 1;
+  return b;
+}
+
+// Run a set of jobs of one path precision. Jobs with the same kernel
+// instantiation share one persistent launch.
+template <typename R>
+void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms) {
+  std::vector<Job<R>> jobs(specs.size());
+  std::vector<BuiltJob> built;
+  std::vector<double> strikes;
+  for (size_t i = 0; i < specs.size(); ++i) built.push_back(build_job<R>(c, specs[i], jobs[i], strikes));
+
+  // slots and sums layout
+  int soff = 0;
+  std::vector<int> n_slots(jobs.size()), nks(jobs.size()), out_off(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    n_slots[i] = built[i].n_slots;
+    nks[i] = built[i].nk;
+    out_off[i] = soff;
+    soff += 2 * NSTAT * built[i].nk;
+  }
+  size_t partial_elems = 0;
+  for (size_t i = 0; i < jobs.size(); ++i)
+    partial_elems += static_cast<size_t>(n_slots[i]) * 2 * NSTAT * nks[i];
+  // lay partials out per job contiguously: offsets in units of that job's width
+  std::vector<size_t> pbase(jobs.size());
+  {
+    size_t o = 0;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+      pbase[i] = o;
+      o += static_cast<size_t>(n_slots[i]) * 2 * NSTAT * nks[i];
+    }
+  }
+  c->partial.reserve(std::max<size_t>(partial_elems, 1) * sizeof(double));
+  c->sums.reserve(std::max(soff, 1) * sizeof(double));
+  const size_t strike_bytes = strikes.size() * sizeof(double);
+  const size_t jobs_bytes = jobs.size() * sizeof(Job<R>);
+  c->jobs.reserve(jobs_bytes + strike_bytes + 256);
+  double *d_strikes = reinterpret_cast<double *>(static_cast<char *>(c->jobs.p) + ((jobs_bytes + 255) / 256) * 256);
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    jobs[i].strikes = d_strikes + reinterpret_cast<size_t>(jobs[i].strikes);
+    // each job indexes its partials from its own base with first_slot = 0
+    jobs[i].partial = c->partial.as<double>() + pbase[i];
+    jobs[i].first_slot = 0;
+  }
+  HIPCHECK(hipMemcpyAsync(c->jobs.p, jobs.data(), jobs_bytes, hipMemcpyHostToDevice, c->stream));
+  if (strike_bytes)
+    HIPCHECK(hipMemcpyAsync(d_strikes, strikes.data(), strike_bytes, hipMemcpyHostToDevice, c->stream));
+  const Job<R> *d_jobs = c->jobs.as<Job<R>>();
+
+  // units grouped by kernel instantiation, most expensive first
+  std::map<std::pair<int, int>, std::vector<Unit>> groups;
+  for (size_t i = 0; i < jobs.size(); ++i)
+    for (int b = 0; b < n_slots[i]; ++b) groups[{built[i].nf, built[i].nv}].push_back({static_cast<int>(i), b});
+  size_t max_units = 0;
+  for (auto &kv : groups) max_units = std::max(max_units, kv.second.size());
+  c->units.reserve(groups.size() * max_units * sizeof(Unit) + 64);
+  c->counter.reserve(groups.size() * 64);
+  HIPCHECK(hipMemsetAsync(c->counter.p, 0, groups.size() * 64, c->stream));
+  std::vector<Unit> all_units;
+  std::vector<std::pair<size_t, size_t>> ranges;
+  for (auto &kv : groups) {
+    auto &u = kv.second;
+    std::stable_sort(u.begin(), u.end(), [&](const Unit &a, const Unit &b) { return built[a.job].cost > built[b.job].cost; });
+    ranges.push_back({all_units.size(), u.size()});
+    all_units.insert(all_units.end(), u.begin(), u.end());
+  }
+  HIPCHECK(hipMemcpyAsync(c->units.p, all_units.data(), all_units.size() * sizeof(Unit), hipMemcpyHostToDevice,
+                          c->stream));
+
+  // meta for the reduction: per job the element offset of its partials,
+  // its unit count, strike count and output offset
+  const size_t nj = jobs.size();
+  c->meta.reserve(nj * (sizeof(long long) + 3 * sizeof(int)) + 64);
+  std::vector<long long> mbase(pbase.begin(), pbase.end());
+  std::vector<int> mint;
+  mint.insert(mint.end(), n_slots.begin(), n_slots.end());
+  mint.insert(mint.end(), nks.begin(), nks.end());
+  mint.insert(mint.end(), out_off.begin(), out_off.end());
+  long long *d_base = c->meta.as<long long>();
+  int *d_mint = reinterpret_cast<int *>(d_base + nj);
+  HIPCHECK(hipMemcpyAsync(d_base, mbase.data(), nj * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_mint, mint.data(), mint.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  size_t gi = 0;
+  for (auto &kv : groups) {
+    const int nf = kv.first.first, nv = kv.first.second;
+    int nfields = 1;
+    for (size_t i = 0; i < jobs.size(); ++i)
+      if (built[i].nf == nf && built[i].nv == nv) nfields = std::max(nfields, jobs[i].n_fields);
+    const size_t smem = static_cast<size_t>(nfields) * MAXV * BLOCK * sizeof(R);
+    if (smem > 64 * 1024) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds 64 KiB of LDS");
+    const int bpc = mc_blocks_per_cu<R>(nf, nv, smem);
+    const int n_units = static_cast<int>(ranges[gi].second);
+    const int grid = std::max(1, std::min(n_units, bpc * c->n_cu));
+    HIPCHECK(launch_mc<R>(nf, nv, d_jobs, c->units.as<Unit>() + ranges[gi].first, n_units,
+                          c->counter.as<unsigned>() + 16 * gi, smem, grid, c->stream));
+    ++gi;
+  }
+  {
+    int maxw = 0;
+    for (int k : nks) maxw = std::max(maxw, 2 * NSTAT * k);
+    const int tb = 128;
+    dim3 grid((maxw + tb - 1) / tb, static_cast<unsigned>(jobs.size()));
+    const int nji = static_cast<int>(nj);
+    hipLaunchKernelGGL(reduce_partials_kernel, grid, dim3(tb), 0, c->stream, c->partial.as<double>(),
+                       c->sums.as<double>(), d_base, d_mint, d_mint + nj, d_mint + 2 * nj, nji);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  std::vector<double> sums(soff);
+  HIPCHECK(hipMemcpyAsync(sums.data(), c->sums.p, soff * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  device_ms += ms;
+
+  // statistics (MonteCarloPricer.hpp:443-478)
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    const JobSpec &js = built[i].spec;
+    myql_mc_result *res = js.res;
+    const int nk = nks[i];
+    const double *sm = sums.data() + out_off[i];
+    const double invM = 1.0 / static_cast<double>(js.cfg->num_paths);
+    const double df = built[i].df;
+    auto put = [&](int st, double *val, double *err) {
+      for (int k = 0; k < nk; ++k) {
+        const double mean = sm[(2 * st) * nk + k] * invM;
+        const double var = sm[(2 * st + 1) * nk + k] * invM - mean * mean;
+        if (val) val[js.k0 + k] = mean * df;
+        if (err) err[js.k0 + k] = std::sqrt(var * invM) * df;
+      }
+    };
+    put(ST_PRICE, res->price, res->price_std_err);
+    if (js.mode != MYQL_GREEK_NONE) {
+      put(ST_DELTA, res->delta, res->delta_std_err);
+      put(ST_GAMMA, res->gamma, res->gamma_std_err);
+    }
+    if (js.mode == MYQL_GREEK_FULL) {
+      put(ST_VEGA0, res->vega[0], res->vega_std_err[0]);
+      put(ST_VEGA1, res->vega[1], res->vega_std_err[1]);
+      put(ST_THETA, res->theta, res->theta_std_err);
+      put(ST_RHO, res->rho, res->rho_std_err);
+      put(ST_VANNA0, res->vanna[0], res->vanna_std_err[0]);
+      put(ST_VANNA1, res->vanna[1], res->vanna_std_err[1]);
+      put(ST_CHARM, res->charm, res->charm_std_err);
+    }
+    if (js.k0 == 0) res->total_steps = js.cfg->num_paths * js.cfg->time_steps * built[i].variants_alg;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// MLMC: Giles' adaptive driver on the host (MultiLevelMonteCarloPricer.hpp
+// :46-241) over level kernels that keep every stream's state on the device.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct MlmcRun {
+  myql_ctx *c;
+  const myql_model &m;
+  const int32_t *schemes;
+  const myql_instrument &in;
+  int mode;
+  const myql_mlmc_config &cfg;
+  double S0, r, q;
+  uint64_t S;
+  double device_ms = 0.0;
+
+  void level(size_t l, uint64_t dN, double &sum, double &sq) {
+    Job<R> J;
+    fill_instrument(J, m, schemes, in, mode, S0, r, cfg.fd_bump, false);
+    const bool pd = in.kind != MYQL_INSTR_EUROPEAN;
+    const double T = in.maturity;
+    const int nf = m.n_factors;
+    const bool coupled = l > 0;
+    const uint64_t fs = cfg.base_steps * (1ull << l);
+    const uint64_t steps = coupled ? fs / 2 : cfg.base_steps;
+    const double dtf = coupled ? T / static_cast<double>(fs) : T / static_cast<double>(cfg.base_steps);
+    int nmax[2] = {0, 0};
+    for (int f = 0; f < nf; ++f)
+      if (schemes[f] == MYQL_SCHEME_NCI) {
+        const myql_heston_params &p = m.heston[f];
+        nmax[f] = nci_nmax(p, dtf);
+        NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nmax[f], cfg.precision);
+        J.nci[f] = ni.tab;
+        J.nci_grid = ni.grid;
+        J.nci_dx = ni.dx;
+        J.nci_inv_dx = ni.inv_dx;
+      }
+    fill_variant(J.var[0], m, schemes, dtf, r, q, T, coupled ? fs : steps, pd, nmax);
+    if (coupled) fill_variant(J.var[1], m, schemes, 2.0 * dtf, r, q, T, steps, pd, nmax);
+    J.var_list[0] = 0;
+    J.var_list[1] = 1;
+    J.n_groups = 1;
+    J.group_size[0] = 1;
+    J.M = dN;
+    J.S = S;
+    J.steps = steps;
+    J.nk = 1;
+    J.nkp = 1;
+    const size_t jb = (sizeof(Job<R>) + 255) / 256 * 256;
+    c->mlmc_job.reserve(jb + 64);
+    double *d_k = reinterpret_cast<double *>(static_cast<char *>(c->mlmc_job.p) + jb);
+    J.strikes = d_k;
+    HIPCHECK(hipMemcpyAsync(c->mlmc_job.p, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(d_k, in.strikes, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const unsigned nb = static_cast<unsigned>((S + BLOCK - 1) / BLOCK);
+    const size_t smem = static_cast<size_t>(J.n_fields) * MAXV * BLOCK * sizeof(R);
+    if (smem > 64 * 1024) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds 64 KiB of LDS");
+    HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    const Job<R> *dj = c->mlmc_job.as<Job<R>>();
+    uint64_t *st = c->mlmc_streams.as<uint64_t>();
+    double *bs = c->mlmc_sums.as<double>();
+    const int cp = coupled ? 1 : 0;
+    if (nf == 0)
+      hipLaunchKernelGGL((mlmc_level_kernel<R, 0>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dN, cp, bs);
+    else if (nf == 1)
+      hipLaunchKernelGGL((mlmc_level_kernel<R, 1>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dN, cp, bs);
+    else
+      hipLaunchKernelGGL((mlmc_level_kernel<R, 2>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dN, cp, bs);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev1, c->stream));
+    std::vector<double> h(2 * nb);
+    HIPCHECK(hipMemcpyAsync(h.data(), bs, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    device_ms += ms;
+    for (unsigned b = 0; b < nb; ++b) {
+      sum += h[2 * b];
+      sq += h[2 * b + 1];
+    }
+  }
+
+  void run(myql_mlmc_result &res) {
+    // persistent per-stream generators (the reference's rng_pool, :55-61)
+    const uint64_t *init = stream_states(c, cfg.seed, cfg.stream_layout, S);
+    c->mlmc_streams.reserve(S * 32);
+    HIPCHECK(hipMemcpyAsync(c->mlmc_streams.p, init, S * 32, hipMemcpyDeviceToDevice, c->stream));
+    c->mlmc_sums.reserve(2 * ((S + BLOCK - 1) / BLOCK) * sizeof(double) + 64);
+    const size_t Lmax = cfg.L_max;
+    size_t L = cfg.L_min;
+    std::vector<uint64_t> N(Lmax + 1, 0);
+    std::vector<double> sum(Lmax + 1, 0.0), sq(Lmax + 1, 0.0), V(Lmax + 1, 0.0), cost(Lmax + 1, 0.0);
+    for (size_t l = 0; l <= Lmax; ++l) cost[l] = static_cast<double>(cfg.base_steps * (1ull << l));
+    double alpha = cfg.alpha, beta = cfg.beta;
+    const double eps = cfg.epsilon;
+    bool converged = false;
+    while (!converged) {
+      for (size_t l = 0; l <= L; ++l) {
+        double sumVC = 0.0;
+        for (size_t k = 0; k <= L; ++k) sumVC += std::sqrt(V[k] * cost[k]);
+        uint64_t opt = 0;
+        if (V[l] > 1e-18)
+          opt = static_cast<uint64_t>(std::ceil(2.0 * std::pow(eps, -2.0) * std::sqrt(V[l] / cost[l]) * sumVC));
+        const uint64_t pilot = std::max<uint64_t>(20, cfg.N0 / (1ull << l));
+        const uint64_t target = std::max(pilot, opt);
+        if (N[l] < target) {
+          const uint64_t dN = target - N[l];
+          level(l, dN, sum[l], sq[l]);
+          N[l] += dN;
+          if (N[l] > 1) {
+            const double mean = sum[l] / N[l];
+            V[l] = std::max(1e-18, sq[l] / N[l] - mean * mean);
+          }
+        }
+      }
+      if (cfg.adaptive_rates && L >= 2) {
+        auto order = [&](const std::vector<double> &vals) {
+          double sl = 0, sy = 0, sly = 0, sl2 = 0;
+          int cnt = 0;
+          const size_t l0 = L - 2;
+          for (size_t l = l0; l <= L; ++l)
+            if (vals[l] > 1e-16) {
+              const double y = std::log(vals[l]), x = static_cast<double>(l - l0);
+              sl += x;
+              sy += y;
+              sly += x * y;
+              sl2 += x * x;
+              ++cnt;
+            }
+          if (cnt < 2) return -1.0;
+          return -((cnt * sly - sl * sy) / (cnt * sl2 - sl * sl)) / std::log(2.0);
+        };
+        std::vector<double> am(L + 1);
+        for (size_t l = 0; l <= L; ++l) am[l] = std::fabs(sum[l] / N[l]);
+        const double ea = order(am), eb = order(V);
+        if (ea > 0.3) alpha = std::min(2.0, ea);
+        if (eb > 0.3) beta = std::min(2.0, eb);
+      }
+      double sumVC = 0.0;
+      for (size_t l = 0; l <= L; ++l) sumVC += std::sqrt(V[l] * cost[l]);
+      bool more = false;
+      for (size_t l = 0; l <= L; ++l) {
+        const uint64_t opt =
+            static_cast<uint64_t>(std::ceil(2.0 * std::pow(eps, -2.0) * std::sqrt(V[l] / cost[l]) * sumVC));
+        if (opt > N[l]) more = true;
+      }
+      if (more) continue;
+      double eff = std::fabs(sum[L] / N[L]);
+      if (L > 0) eff = std::max(eff, 0.5 * std::fabs(sum[L - 1] / N[L - 1]));
+      const double bias = eff / (std::pow(2.0, alpha) - 1.0);
+      if (bias > eps / 1.4142135623730951) {
+        if (L < Lmax) ++L;
+        else converged = true;
+      } else {
+        converged = true;
+      }
+    }
+    const double df = std::exp(-r * in.maturity);
+    res.price = 0.0;
+    double var_total = 0.0;
+    res.total_steps = 0;
+    res.alpha_estimated = alpha;
+    res.beta_estimated = beta;
+    res.n_levels = static_cast<int32_t>(L + 1);
+    for (size_t l = 0; l <= L; ++l) {
+      const uint64_t spp = (l == 0) ? cfg.base_steps : (cfg.base_steps * (1ull << l) * 3 / 2);
+      res.total_steps += N[l] * spp;
+      const double mean = sum[l] / N[l];
+      if (res.paths_per_level) res.paths_per_level[l] = N[l];
+      if (res.mean_diffs) res.mean_diffs[l] = mean * df;
+      if (res.var_diffs) res.var_diffs[l] = V[l] * df * df;
+      res.price += mean * df;
+      var_total += V[l] / N[l];
+    }
+    res.price_std_err = std::sqrt(var_total) * df;
+    res.device_ms = device_ms;
+  }
+};
+
+void mlmc_validate(const myql_model *m, const int32_t *schemes, const myql_instrument *in, const myql_mlmc_config *cfg) {
+  myql_mc_config probe{};
+  probe.num_paths = 1;
+  probe.time_steps = 1;
+  probe.precision = cfg ? cfg->precision : MYQL_FP64;
+  validate(m, schemes, in, &probe);
+  if (cfg->L_max > 30 || cfg->L_min > cfg->L_max) throw Fail(MYQL_E_INVALID_ARG, "need L_min <= L_max <= 30");
+  if (cfg->base_steps < 1) throw Fail(MYQL_E_INVALID_ARG, "base_steps must be >= 1");
+  if (!(cfg->epsilon > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "epsilon must be positive");
+}
+
+int fail(myql_ctx *c, const Fail &e) {
+  if (c) c->err = e.what();
+  return e.code;
+}
+
+// Split jobs into strike chunks of at most 64 and run them by precision.
+void run_specs(myql_ctx *c, const std::vector<JobSpec> &in_specs) {
+  std::vector<JobSpec> s32, s64;
+  for (const JobSpec &s : in_specs) {
+    validate(s.model, s.schemes, s.instr, s.cfg);
+    if (!s.res) throw Fail(MYQL_E_INVALID_ARG, "null result");
+    const int nk = static_cast<int>(s.instr->n_strikes);
+    for (int k0 = 0; k0 < nk; k0 += STRIKE_CHUNK) {
+      JobSpec t = s;
+      t.k0 = k0;
+      t.nk = std::min(STRIKE_CHUNK, nk - k0);
+      (s.cfg->precision == MYQL_FP32 ? s32 : s64).push_back(t);
+    }
+    s.res->device_ms = 0.0;
+  }
+  double ms32 = 0.0, ms64 = 0.0;
+  if (!s32.empty()) run_jobs<float>(c, s32, ms32);
+  if (!s64.empty()) run_jobs<double>(c, s64, ms64);
+  for (const JobSpec &s : in_specs) s.res->device_ms = ms32 + ms64;
+}
+
+}  // namespace
+
+namespace myql {
+bool mc_supported(int nf, int nv) {
+  if (nv == 1) return nf >= 0 && nf <= 2;
+  if (nf == 0 || nf == 1) return nv == 5 || nv == 7;
+  if (nf == 2) return nv == 7 || nv == 9;
+  return false;
+}
+}  // namespace myql
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+myql_ctx *myql_create(int device) {
+  auto *c = new (std::nothrow) myql_ctx();
+  if (!c) return nullptr;
+  try {
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Fail(MYQL_E_INVALID_ARG, "no such device");
+    c->device = device;
+    set_device(c);
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount;
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&c->ev0));
+    HIPCHECK(hipEventCreate(&c->ev1));
+    c->jump_polys = host_jump_polys(64);
+    c->d_polys.reserve(c->jump_polys.size() * 8);
+    HIPCHECK(hipMemcpy(c->d_polys.p, c->jump_polys.data(), c->jump_polys.size() * 8, hipMemcpyHostToDevice));
+    return c;
+  } catch (const Fail &) {
+    delete c;
+    return nullptr;
+  }
+}
+
+void myql_destroy(myql_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  c->streams.clear();
+  c->nci.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *myql_last_error(const myql_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int myql_mc_calculate(myql_ctx *c, const myql_model *model, const int32_t schemes[2], const myql_instrument *instr,
+                      int32_t greek_mode, const myql_mc_config *cfg, double S0, double r, double q,
+                      myql_mc_result *res) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (greek_mode < MYQL_GREEK_NONE || greek_mode > MYQL_GREEK_FULL) throw Fail(MYQL_E_INVALID_ARG, "bad greek mode");
+    JobSpec s{model, schemes, instr, greek_mode, cfg, S0, r, q, res, 0, 0};
+    run_specs(c, {s});
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_mc_calculate_batch(myql_ctx *c, const myql_mc_job *jobs, int64_t n_jobs) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) throw Fail(MYQL_E_INVALID_ARG, "bad job list");
+    std::vector<JobSpec> specs;
+    for (int64_t i = 0; i < n_jobs; ++i) {
+      const myql_mc_job &j = jobs[i];
+      if (j.greek_mode < MYQL_GREEK_NONE || j.greek_mode > MYQL_GREEK_FULL) throw Fail(MYQL_E_INVALID_ARG, "bad greek mode");
+      specs.push_back({j.model, j.schemes, j.instrument, j.greek_mode, j.config, j.S0, j.r, j.q, j.result, 0, 0});
+    }
+    if (!specs.empty()) run_specs(c, specs);
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_mlmc_calculate(myql_ctx *c, const myql_model *model, const int32_t schemes[2], const myql_instrument *instr,
+                        int32_t greek_mode, const myql_mlmc_config *cfg, double S0, double r, double q,
+                        myql_mlmc_result *res) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!model || !schemes || !instr || !cfg || !res) throw Fail(MYQL_E_INVALID_ARG, "null argument");
+    mlmc_validate(model, schemes, instr, cfg);
+    const uint64_t S = cfg->num_streams ? cfg->num_streams : 65536;
+    if (cfg->precision == MYQL_FP32) {
+      MlmcRun<float> run{c, *model, schemes, *instr, greek_mode, *cfg, S0, r, q, S};
+      run.run(*res);
+    } else {
+      MlmcRun<double> run{c, *model, schemes, *instr, greek_mode, *cfg, S0, r, q, S};
+      run.run(*res);
+    }
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_debug_stream_draws(myql_ctx *c, uint64_t seed, int32_t layout, uint64_t n_streams, uint64_t n_per,
+                            uint64_t *out) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!out || n_streams == 0) throw Fail(MYQL_E_INVALID_ARG, "bad arguments");
+    const uint64_t *st = stream_states(c, seed, layout, n_streams);
+    DevBuf o;
+    o.reserve(n_streams * n_per * 8 + 8);
+    const unsigned nb = static_cast<unsigned>((n_streams + 255) / 256);
+    hipLaunchKernelGGL(debug_draws_kernel, dim3(nb), dim3(256), 0, c->stream, st, n_streams, n_per, o.as<uint64_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, o.p, n_streams * n_per * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+static int debug_map(myql_ctx *c, int which, const double *a, const double *b, int64_t n, double *o64, float *o32) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (n <= 0) return MYQL_OK;
+    DevBuf da, db, d64, d32;
+    da.reserve(n * 8);
+    db.reserve(n * 8);
+    d64.reserve(n * 8);
+    d32.reserve(n * 4);
+    HIPCHECK(hipMemcpyAsync(da.p, a, n * 8, hipMemcpyHostToDevice, c->stream));
+    if (b) HIPCHECK(hipMemcpyAsync(db.p, b, n * 8, hipMemcpyHostToDevice, c->stream));
+    const unsigned nb = static_cast<unsigned>((n + 255) / 256);
+    if (which == 0)
+      hipLaunchKernelGGL(debug_normcdfinv_kernel, dim3(nb), dim3(256), 0, c->stream, da.as<double>(), n,
+                         d64.as<double>(), d32.as<float>());
+    else if (which == 1)
+      hipLaunchKernelGGL(debug_poissinv_kernel, dim3(nb), dim3(256), 0, c->stream, da.as<double>(), db.as<double>(), n,
+                         d64.as<double>());
+    else
+      hipLaunchKernelGGL(debug_chi2_kernel, dim3(nb), dim3(256), 0, c->stream, da.as<double>(), db.as<double>(), n,
+                         d64.as<double>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(o64, d64.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (o32) HIPCHECK(hipMemcpyAsync(o32, d32.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_debug_normcdfinv(myql_ctx *c, const double *p, int64_t n, double *out64, float *out32) {
+  if (!p || !out64 || !out32) return MYQL_E_INVALID_ARG;
+  return debug_map(c, 0, p, nullptr, n, out64, out32);
+}
+
+int myql_debug_poissinv(myql_ctx *c, const double *lambda, const double *u, int64_t n, double *out) {
+  if (!lambda || !u || !out) return MYQL_E_INVALID_ARG;
+  return debug_map(c, 1, lambda, u, n, out, nullptr);
+}
+
+int myql_debug_chi2_quantile(myql_ctx *c, const double *dof, const double *p, int64_t n, double *out) {
+  if (!dof || !p || !out) return MYQL_E_INVALID_ARG;
+  return debug_map(c, 2, dof, p, n, out, nullptr);
+}
+
+}  // extern "C"

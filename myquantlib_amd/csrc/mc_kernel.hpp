@@ -1,0 +1,585 @@
+// Path-simulation kernels of the MI355X Monte Carlo engine.
+//
+// One lane owns one xoshiro256++ stream and walks the samples of that stream
+// in order (the OpenMP static partition the reference uses,
+// MonteCarloPricer.hpp:257,302). A sample's Greek variants either run in
+// lockstep in the lane's registers, sharing every draw (NV/NCI schemes:
+// fixed draw counts), or are replayed one after another from a register
+// copy of the stream state (Exact scheme, path-dependent jumps) -- both give
+// exactly the reference's common-random-number replay (:304, :328-405).
+//
+// Per block of 256 lanes (4 wavefronts) the terminal state of every variant
+// is staged in LDS; the strike loop then maps lanes to strikes, so each lane
+// keeps the FP64 sum / sum-of-squares of one strike in registers and no
+// atomics are used. Work units (job, 256 streams) are pulled from a queue by
+// a persistent grid; each unit writes its own partial sums, which a second
+// kernel adds in unit order, so results do not depend on scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dev_math.hpp"
+#include "engine_types.hpp"
+
+namespace myql {
+
+template <typename R>
+__device__ __forceinline__ R dsqrt(R x);
+template <>
+__device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+
+template <typename R>
+__device__ __forceinline__ R dexp(R x);
+template <>
+__device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
+
+template <typename R>
+__device__ __forceinline__ R dlog(R x);
+template <>
+__device__ __forceinline__ double dlog<double>(double x) { return log(x); }
+template <>
+__device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+
+template <typename R>
+struct Trk {
+  R sumlog, sumS, mn, mx, t;
+  int hits;  // bit 0 base barrier, bit 1 barrier for S0+h, bit 2 for S0-h
+  R ht[3];
+};
+
+// ---- trackers (PathTrackers.hpp:8-300) -----------------------------------
+template <typename R>
+__device__ __forceinline__ void trk_init(const Job<R> *__restrict__ J, Trk<R> &s, R x, R dt) {
+  s.sumlog = 0;
+  s.sumS = 0;
+  s.mn = x;
+  s.mx = x;
+  s.t = 0;
+  s.hits = 0;
+  s.ht[0] = s.ht[1] = s.ht[2] = 0;
+  switch (J->tracker) {
+    case 1:  // geometric Asian
+      s.sumlog = static_cast<R>(J->init_sumlog) + R(0.5) * x * dt;
+      break;
+    case 2:  // arithmetic Asian
+      s.sumlog = static_cast<R>(J->init_sumlog) + R(0.5) * x * dt;
+      s.sumS = static_cast<R>(J->init_sumS) + R(0.5) * dexp<R>(x) * dt;
+      break;
+    case 3:  // lookback, optionally seeded by the observed history
+      if (J->hist_min_logS < 1e300) s.mn = static_cast<R>(J->hist_min_logS);
+      if (J->hist_max_logS > -1e300) s.mx = static_cast<R>(J->hist_max_logS);
+      break;
+    case 4:  // barrier: day-0 breaches
+      if (J->barrier_up) {
+        s.hits = (x >= J->b_base ? 1 : 0) | (x >= J->b_up ? 2 : 0) | (x >= J->b_dn ? 4 : 0);
+      } else {
+        s.hits = (x <= J->b_base ? 1 : 0) | (x <= J->b_up ? 2 : 0) | (x <= J->b_dn ? 4 : 0);
+      }
+      break;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void mark(Trk<R> &s, int bit, int idx) {
+  if (!(s.hits & bit)) {
+    s.hits |= bit;
+    s.ht[idx] = s.t;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void trk_update(const Job<R> *__restrict__ J, Trk<R> &s, R x, R dt) {
+  switch (J->tracker) {
+    case 1:
+      s.sumlog += x * dt;
+      break;
+    case 2:
+      s.sumlog += x * dt;
+      s.sumS += dexp<R>(x) * dt;
+      break;
+    case 3:
+      s.mx = (x > s.mx) ? x : s.mx;
+      s.mn = (x < s.mn) ? x : s.mn;
+      break;
+    case 4:
+      s.t += dt;
+      // nested tests, easiest barrier first
+      if (J->barrier_up) {
+        if (x >= J->b_up) {
+          mark(s, 2, 1);
+          if (x >= J->b_base) {
+            mark(s, 1, 0);
+            if (x >= J->b_dn) mark(s, 4, 2);
+          }
+        }
+      } else {
+        if (x <= J->b_dn) {
+          mark(s, 4, 2);
+          if (x <= J->b_base) {
+            mark(s, 1, 0);
+            if (x <= J->b_up) mark(s, 2, 1);
+          }
+        }
+      }
+      break;
+  }
+}
+
+// Finalize and stage one variant's terminal data in LDS.
+template <typename R>
+__device__ __forceinline__ void finalize_stage(const Job<R> *__restrict__ J, const Trk<R> &s, R x, R dt,
+                                               R T, R *__restrict__ stage, int v, int tid) {
+  R aux = 0;
+  switch (J->tracker) {
+    case 1: {
+      const R sl = s.sumlog - R(0.5) * x * dt;
+      aux = dexp<R>(sl / (T + static_cast<R>(J->t_elapsed)));
+      break;
+    }
+    case 2: {
+      const R sS = s.sumS - R(0.5) * dexp<R>(x) * dt;
+      aux = sS / (T + static_cast<R>(J->t_elapsed));
+      break;
+    }
+    case 3: {
+      const R e = J->fixed_strike ? (J->call ? s.mx : s.mn) : (J->call ? s.mn : s.mx);
+      aux = dexp<R>(e);
+      break;
+    }
+  }
+  stage[(F_SPOT * MAXV + v) * BLOCK + tid] = dexp<R>(x);
+  if (J->n_fields > F_AUX) stage[(F_AUX * MAXV + v) * BLOCK + tid] = aux;
+  if (J->n_fields > F_FLAGS) stage[(F_FLAGS * MAXV + v) * BLOCK + tid] = static_cast<R>(s.hits);
+  if (J->n_fields > F_HT0) {
+    stage[(F_HT0 * MAXV + v) * BLOCK + tid] = s.ht[0];
+    stage[(F_HT1 * MAXV + v) * BLOCK + tid] = s.ht[1];
+    stage[(F_HT2 * MAXV + v) * BLOCK + tid] = s.ht[2];
+  }
+}
+
+// ---- gamma variates of SchemeExact (VolSchemes.hpp:129-203) ---------------
+template <typename R>
+__device__ static R gamma_draw(double alpha_base, int N, Rng &g) {
+  const double alpha = alpha_base + N;
+  if (alpha > 1.0) {
+    const double dd = alpha - 1.0 / 3.0;
+    const R d = static_cast<R>(dd);
+    const R c = static_cast<R>(1.0 / sqrt(9.0 * dd));
+    for (;;) {
+      R Z, v;
+      do {
+        Z = normcdfinv<R>(uniform(g));
+        v = R(1) + c * Z;
+      } while (v <= R(0));
+      v = v * v * v;
+      const R U = static_cast<R>(uniform(g));
+      if (U < R(1) - R(0.0331) * Z * Z * Z * Z) return d * v;
+      if (dlog<R>(U) < R(0.5) * Z * Z + d * (R(1) - v + dlog<R>(v))) return d * v;
+    }
+  }
+  // Ahrens-Dieter GS for alpha <= 1: one uniform picks the piece and is
+  // reused, rescaled, as the acceptance uniform
+  const double e = 2.71828182845904523536;
+  const R p = static_cast<R>(e / (alpha_base + e));
+  const R inv_p = static_cast<R>((alpha_base + e) / e);
+  const R one_m_p = R(1) - p;
+  const R one_m_a = static_cast<R>(1.0 - alpha_base);
+  const R inv_a = static_cast<R>(1.0 / alpha_base);
+  for (;;) {
+    const R u = static_cast<R>(uniform(g));
+    const R y = exponential<R>(R(1), g);
+    if (u < p) {
+      const R x = dexp<R>(-y * inv_a);
+      if (u * inv_p < R(1) - x) return x;
+      if (u < p * dexp<R>(-x)) return x;
+    } else {
+      const R x = R(1) + y;
+      const R inv_h = R(1) / (R(1) + one_m_a * y);
+      if (u < p + one_m_p * inv_h) return x;
+      if (u < p + one_m_p * dexp<R>(-one_m_a * dlog<R>(x))) return x;
+    }
+  }
+}
+
+// ---- one lane-local log jump (JumpPolicies.hpp:55-135) ---------------------
+template <typename R>
+__device__ __forceinline__ R log_jump(const Job<R> *__restrict__ J, const VarParams<R> *__restrict__ P,
+                                      Rng &g) {
+  const int n = poisson_count(P->jmean, P->jexpm, uniform(g));
+  if (n == 0) return P->jdrift;
+  if (J->jump_kind == 1) {
+    const R Z = normcdfinv<R>(uniform(g));
+    return P->jdrift + static_cast<R>(n) * J->jmu + dsqrt<R>(static_cast<R>(n)) * J->jdelta * Z;
+  }
+  R tot = 0;
+  for (int i = 0; i < n; ++i) {
+    const double u = uniform(g);
+    if (u < static_cast<double>(J->kp)) tot += exponential<R>(J->keta1, g);
+    else tot -= exponential<R>(J->keta2, g);
+  }
+  return P->jdrift + tot;
+}
+
+// European end-of-path jump for NV lockstep slots. All slots share the
+// Poisson uniform; slots needing more draws read them from a copy of the
+// stream, and the stream advances by what the base path (slot 0) consumed.
+template <typename R, int NV>
+__device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
+                                          const VarParams<R> *__restrict__ const *P, R *x, Rng &g) {
+  if (J->jump_kind == 0) return;
+  const double ua = uniform(g);
+  int n[NV];
+  int nmax = 0;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    n[k] = poisson_count(P[k]->jmean, P[k]->jexpm, ua);
+    nmax = n[k] > nmax ? n[k] : nmax;
+  }
+  R tot[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = 0;
+  if (nmax > 0) {
+    Rng t = g;
+    if (J->jump_kind == 1) {
+      const R Z = normcdfinv<R>(uniform(t));
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        if (n[k] > 0)
+          tot[k] = static_cast<R>(n[k]) * J->jmu + dsqrt<R>(static_cast<R>(n[k])) * J->jdelta * Z;
+      if (n[0] > 0) g = t;
+    } else {
+      for (int i = 0; i < nmax; ++i) {
+        const double u = uniform(t);
+        const R E = static_cast<R>(-log1p(-uniform(t)));
+        const R jsz = (u < static_cast<double>(J->kp)) ? E / J->keta1 : -E / J->keta2;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          if (i < n[k]) tot[k] += jsz;
+        if (i + 1 == n[0]) g = t;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) x[k] += P[k]->jdrift + tot[k];
+}
+
+// ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
+template <typename R, int NV, int NFX>
+__device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
+                                              const VarParams<R> *__restrict__ const *P, int f,
+                                              R (&v)[NV][NFX], R (&vn)[NV][NFX], Rng &g) {
+  const int sch = J->scheme[f];
+  if (sch == 1) {  // Ninomiya-Victoir
+    const R Z = normcdfinv<R>(uniform(g));
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const R root = dsqrt<R>(P[k]->Kr[f][0] * v[k][f] + P[k]->Kr[f][1]) + P[k]->Kr[f][2] * Z;
+      vn[k][f] = P[k]->Kr[f][0] * (root * root) + P[k]->Kr[f][1];
+    }
+  } else if (sch == 2) {  // NCI: Poisson mixture, tabulated chi2 inverse
+    const double u1 = uniform(g);
+    const double u2 = uniform(g);
+    const double pos = u2 * J->nci_inv_dx;
+    int i = static_cast<int>(pos);
+    if (i >= J->nci_grid - 1) i = J->nci_grid - 2;
+    const R t = static_cast<R>(pos - static_cast<double>(i));
+    const R t2 = t * t, t3 = t2 * t;
+    const R h00 = R(2) * t3 - R(3) * t2 + R(1), h10 = t3 - R(2) * t2 + t;
+    const R h01 = R(-2) * t3 + R(3) * t2, h11 = t3 - t2;
+    const R *__restrict__ tab = static_cast<const R *>(J->nci[f]);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double lam = P[k]->Kd[f][0] * static_cast<double>(v[k][f]);
+      const int N = static_cast<int>(poissinv(lam, u1));
+      if (N <= P[k]->nmax[f] && u2 < 0.999) {
+        const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
+        vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
+      } else {
+        vn[k][f] = static_cast<R>(chi2_quantile(P[k]->Kd[f][1] + 2.0 * N, u2) * P[k]->Kd[f][2]);
+      }
+    }
+  } else {  // Exact (Poisson-gamma mixture); variable draws, so never lockstep
+    if constexpr (NV == 1) {
+      const double lam = P[0]->Kd[f][0] * static_cast<double>(v[0][f]);
+      const int N = static_cast<int>(poissinv(lam, uniform(g)));
+      vn[0][f] = gamma_draw<R>(P[0]->Kd[f][1], N, g) * P[0]->Kr[f][2];
+    }
+  }
+}
+
+// Simulate one variant group for the lane's current sample and stage it.
+template <typename R, int NF, int NV>
+__device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfirst, Rng &g,
+                                          R *__restrict__ stage, int tid) {
+  constexpr int NFX = NF > 0 ? NF : 1;
+  const VarParams<R> *__restrict__ P[NV];
+  int vid[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    vid[k] = J->var_list[gfirst + k];
+    P[k] = &J->var[vid[k]];
+  }
+  R x[NV], v[NV][NFX];
+  Trk<R> tr[NV];
+  const R x0 = static_cast<R>(J->logS0);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    x[k] = x0;
+#pragma unroll
+    for (int f = 0; f < NFX; ++f) v[k][f] = P[k]->v0[f];
+    trk_init(J, tr[k], x0, P[k]->dt);
+  }
+  const bool pd = J->pathdep;
+  const unsigned steps = static_cast<unsigned>(J->steps);
+  if (NF == 0 && !pd) {
+    // zero-factor European: one step to maturity (ASVJstepper.hpp:145-161)
+    const R G = normcdfinv<R>(uniform(g));
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] += P[k]->gdrift_n + P[k]->gvol_n * G;
+    euro_jump<R, NV>(J, P, x, g);
+  } else {
+    for (unsigned step = 0; step < steps; ++step) {
+      const R G = normcdfinv<R>(uniform(g));
+      if constexpr (NF == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) x[k] += P[k]->gdrift + P[k]->gvol * G;
+      } else {
+        R vn[NV][NFX];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) evolve_factor<R, NV, NFX>(J, P, f, v, vn, g);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          R a = 0, d = 0;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            a += P[k]->C[f][3] * (v[k][f] + vn[k][f]);
+            d += P[k]->C[f][0] + P[k]->C[f][1] * v[k][f] + P[k]->C[f][2] * vn[k][f];
+          }
+          x[k] += d + dsqrt<R>(a) * G;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) v[k][f] = vn[k][f];
+        }
+      }
+      if (pd) {
+        if (J->jump_kind != 0) {
+          if constexpr (NV == 1) x[0] += log_jump(J, P[0], g);
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) trk_update(J, tr[k], x[k], P[k]->dt);
+      }
+    }
+    if (!pd && NF > 0) euro_jump<R, NV>(J, P, x, g);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    finalize_stage(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
+}
+
+// ---- payoffs (Payoffs.hpp:14-69) and per-variant payoff triples -----------
+template <typename R>
+__device__ __forceinline__ double payoff(const Job<R> *__restrict__ J, bool smooth, double spot, double K) {
+  const double diff = J->call ? spot - K : K - spot;
+  if (J->payoff == 0) return diff > 0.0 ? diff : 0.0;
+  const double w = smooth ? 0.5 * erfc(-diff * J->eps_scale) : (diff > 0.0 ? 1.0 : 0.0);
+  return J->payoff == 1 ? w : spot * w;
+}
+
+// base / up / dn payoffs of staged variant v at sample s (calculate_to_buffer)
+template <typename R>
+__device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
+                                      int s, double K, bool bumps, double &p, double &pu, double &pd) {
+  const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLOCK + s]);
+  const double mu = J->mult_up, md = J->mult_dn;
+  const bool smooth = J->mode != 0;
+  const int trk = J->tracker;
+  if (trk == 0) {
+    p = payoff(J, smooth, S, K);
+    if (bumps) {
+      pu = payoff(J, smooth, S * mu, K);
+      pd = payoff(J, smooth, S * md, K);
+    }
+  } else if (trk != 4) {
+    const double A = static_cast<double>(stage[(F_AUX * MAXV + v) * BLOCK + s]);
+    if (J->fixed_strike) {
+      p = payoff(J, smooth, A, K);
+      if (bumps) {
+        pu = payoff(J, smooth, A * mu, K);
+        pd = payoff(J, smooth, A * md, K);
+      }
+    } else {
+      p = payoff(J, smooth, S, A);
+      if (bumps) {
+        pu = payoff(J, smooth, S * mu, A * mu);
+        pd = payoff(J, smooth, S * md, A * md);
+      }
+    }
+  } else {
+    const int hits = static_cast<int>(stage[(F_FLAGS * MAXV + v) * BLOCK + s]);
+    auto bar = [&](double sp, int idx) {
+      const bool hit = (hits >> idx) & 1;
+      double reb = 0.0;
+      if (J->rebate_timing != 0) {
+        reb = J->rebate;
+        if (J->rebate_timing == 1 && hit) {
+          const double ht = static_cast<double>(stage[((F_HT0 + idx) * MAXV + v) * BLOCK + s]);
+          reb = J->rebate * exp(J->rebate_rate * (J->maturity - ht));
+        }
+      }
+      const double pv = payoff(J, false, sp, K);
+      if (J->barrier_in) return hit ? pv : reb;
+      return hit ? reb : pv;
+    };
+    p = bar(S, 0);
+    if (bumps) {
+      pu = bar(S * mu, 1);
+      pd = bar(S * md, 2);
+    }
+  }
+}
+
+// Strike loop over one staged batch: lane (j, grp) adds strike j of samples
+// [grp*nkp, grp*nkp + nkp) into its FP64 accumulators (MonteCarloPricer.hpp
+// :264-276 and :312-409 per-path estimators).
+template <typename R>
+__device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
+                                           const int *__restrict__ valid, int tid, double *acc) {
+  const int nkp = J->nkp;
+  const int j = tid & (nkp - 1);
+  const int grp = tid / nkp;
+  if (j >= J->nk) return;
+  const double K = J->strikes[j];
+  const int mode = J->mode;
+  const bool ess = mode != 0, full = mode == 2;
+  for (int s = grp * nkp; s < grp * nkp + nkp; ++s) {
+    if (!valid[s]) continue;
+    double p, pu = 0, pd = 0;
+    eval3(J, stage, V_BASE, s, K, ess, p, pu, pd);
+    acc[2 * ST_PRICE] += p;
+    acc[2 * ST_PRICE + 1] += p * p;
+    if (!ess) continue;
+    const double d = (pu - pd) * J->inv2h;
+    const double gm = (pu - p * 2.0 + pd) * J->invh2;
+    acc[2 * ST_DELTA] += d;
+    acc[2 * ST_DELTA + 1] += d * d;
+    acc[2 * ST_GAMMA] += gm;
+    acc[2 * ST_GAMMA + 1] += gm * gm;
+    if (!full) continue;
+    double ap, au = 0, ad = 0, bp, bu = 0, bd = 0;
+    for (int f = 0; f < J->nvega; ++f) {
+      eval3(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
+      eval3(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
+      const double vg = (ap - bp) * J->inv2dv * J->cv[f];
+      const double vn = ((au - ad) - (bu - bd)) * (J->inv2dv * J->inv2h) * J->cv[f];
+      acc[2 * (ST_VEGA0 + f)] += vg;
+      acc[2 * (ST_VEGA0 + f) + 1] += vg * vg;
+      acc[2 * (ST_VANNA0 + f)] += vn;
+      acc[2 * (ST_VANNA0 + f) + 1] += vn * vn;
+    }
+    eval3(J, stage, V_TUP, s, K, true, ap, au, ad);
+    eval3(J, stage, V_TDN, s, K, true, bp, bu, bd);
+    const double th = (ap * J->dfTu - bp * J->dfTd) * J->inv2dT;
+    const double ch = ((au - ad) * J->dfTu - (bu - bd) * J->dfTd) * (J->inv2dT * J->inv2h);
+    acc[2 * ST_THETA] += th;
+    acc[2 * ST_THETA + 1] += th * th;
+    acc[2 * ST_CHARM] += ch;
+    acc[2 * ST_CHARM + 1] += ch * ch;
+    if (J->derive_r) {
+      // European: the rate-bumped paths are the base path shifted by
+      // +-r_bump*T in log space (only C0 of the log step depends on r)
+      const double S = static_cast<double>(stage[(F_SPOT * MAXV + V_BASE) * BLOCK + s]);
+      ap = payoff(J, mode != 0, S * J->erb_up, K);
+      bp = payoff(J, mode != 0, S * J->erb_dn, K);
+    } else {
+      eval3(J, stage, V_RUP, s, K, false, ap, au, ad);
+      eval3(J, stage, V_RDN, s, K, false, bp, bu, bd);
+    }
+    const double rh = (ap * J->dfru - bp * J->dfrd) * J->inv2dr;
+    acc[2 * ST_RHO] += rh;
+    acc[2 * ST_RHO + 1] += rh * rh;
+  }
+}
+
+// Persistent work-queue kernel. `counter` must be zero at launch.
+template <typename R, int NF, int NV>
+__global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restrict__ jobs,
+                                                         const Unit *__restrict__ units, int n_units,
+                                                         unsigned *__restrict__ counter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_unit;
+  __shared__ int s_valid[BLOCK];
+  __shared__ double s_red[BLOCK];
+  R *stage = reinterpret_cast<R *>(smem);
+  const int tid = threadIdx.x;
+
+  for (;;) {
+    if (tid == 0) s_unit = static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    __syncthreads();
+    if (u >= n_units) break;
+    const int job = __builtin_amdgcn_readfirstlane(units[u].job);
+    const int blk = __builtin_amdgcn_readfirstlane(units[u].blk);
+    const Job<R> *__restrict__ J = jobs + job;
+
+    const uint64_t S = J->S, M = J->M;
+    const uint64_t s = static_cast<uint64_t>(blk) * BLOCK + tid;
+    const bool active = s < S;
+    Rng g;
+    if (active) {
+      g.s0 = J->streams[4 * s + 0];
+      g.s1 = J->streams[4 * s + 1];
+      g.s2 = J->streams[4 * s + 2];
+      g.s3 = J->streams[4 * s + 3];
+    } else {
+      g.s0 = g.s1 = g.s2 = g.s3 = 0;
+    }
+    const uint64_t q = M / S, rem = M % S;
+    const uint64_t count = active ? q + (s < rem ? 1 : 0) : 0;
+    const uint64_t n_it = q + (static_cast<uint64_t>(blk) * BLOCK < rem ? 1 : 0);
+
+    double acc[2 * NSTAT];
+#pragma unroll
+    for (int a = 0; a < 2 * NSTAT; ++a) acc[a] = 0.0;
+
+    for (uint64_t it = 0; it < n_it; ++it) {
+      const bool valid = it < count;
+      if (valid) {
+        const Rng snap = g;
+        Rng next = g;
+        for (int gi = 0; gi < J->n_groups; ++gi) {
+          Rng gg = snap;
+          sim_group<R, NF, NV>(J, J->group_first[gi], gg, stage, tid);
+          if (gi == 0) next = gg;
+        }
+        g = next;
+      }
+      s_valid[tid] = valid ? 1 : 0;
+      __syncthreads();
+      accumulate(J, stage, s_valid, tid, acc);
+      __syncthreads();
+    }
+
+    // reduce the sample groups of each strike: wave butterfly, then the 4
+    // waves in order; one FP64 partial per unit, accumulator and strike
+    const int nkp = J->nkp;
+    const int lane = tid & 63, wave = tid >> 6;
+    double *out = J->partial + static_cast<size_t>(J->first_slot + blk) * (2 * NSTAT) * J->nk;
+    for (int a = 0; a < 2 * NSTAT; ++a) {
+      double x = acc[a];
+      for (int off = nkp; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
+      s_red[tid] = x;
+      __syncthreads();
+      if (wave == 0 && lane < J->nk) {
+        double t = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += s_red[w * 64 + lane];
+        out[a * J->nk + lane] = t;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace myql

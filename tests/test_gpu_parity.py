@@ -1,0 +1,193 @@
+"""GPU parity: the HIP engine against the CPU FP64 oracle on identical draws.
+
+Every call goes through the C-ABI (libmyql_hip.so). Tolerances:
+  integer stream and Poisson counts: bit-exact;
+  FP64 paths: 1e-9 relative on prices and Greeks (the engine and the oracle
+    differ only by FMA contraction and libm-vs-device ulps);
+  FP32 paths (sums and central differences in FP64): stated per test.
+"""
+import numpy as np
+import pytest
+
+from myquantlib_amd import capi
+from myquantlib_amd.pricing import (Heston, InstrumentSpec, make_model, mc_config, merton, kou,
+                                    mlmc_config)
+
+pytestmark = pytest.mark.gpu
+
+H1 = Heston(2.0, 0.04, 0.3, -0.7, 0.04)          # Feller-compliant (NV allowed)
+H_HI = Heston(2.0, 0.04, 1.0, -0.7, 0.04)        # Feller violated (NCI / Exact)
+H2 = Heston(1.0, 0.03, 0.25, -0.5, 0.02)
+
+
+def rel_err(a, b, floor=1e-12):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def compare(gpu, cpu, fields, tol, floor=1e-8):
+    for f in fields:
+        g, c = np.asarray(gpu[f]), np.asarray(cpu[f])
+        mask = np.isfinite(c)
+        assert np.all(np.isfinite(g[mask])), f
+        e = rel_err(g[mask], c[mask], floor)
+        assert e <= tol, f"{f}: rel err {e:.3e} > {tol:.1e}\n gpu={g}\n cpu={c}"
+
+
+ESS = ["price", "price_std_err", "delta", "gamma"]
+FULL = ESS + ["vega", "theta", "rho", "vanna", "charm"]
+
+
+@pytest.mark.parametrize("layout", [capi.STREAMS_JUMP, capi.STREAMS_SEEDED])
+def test_stream_draws_bit_exact(engine, oracle, layout):
+    n_streams, n_per = 1000, 40
+    gpu = engine.stream_draws(42, layout, n_streams, n_per)
+    for s in [0, 1, 2, 3, 7, 255, 256, 511, 999]:
+        st = oracle.stream_state(42, layout, s)
+        cpu = oracle.next(st, n_per)
+        assert np.array_equal(gpu[s], cpu), f"stream {s}"
+
+
+def test_normcdfinv_device(engine, oracle):
+    p = np.concatenate([np.linspace(1e-12, 1 - 1e-12, 20001), 10.0 ** -np.arange(1, 18),
+                        1 - 10.0 ** -np.arange(1, 16), [0.02425, 0.97575]])
+    o64, o32 = engine.normcdfinv(p)
+    ref = np.array([oracle.normcdfinv(x) for x in p])
+    assert rel_err(o64, ref, 1e-300) < 1e-14
+    assert np.max(np.abs(o32 - ref)) < 2e-5 * np.maximum(1.0, np.abs(ref)).max()
+
+
+def test_poissinv_device_exact(engine, oracle):
+    rng = np.random.default_rng(1)
+    lam = np.concatenate([rng.uniform(0, 4, 3000), rng.uniform(4, 40, 3000),
+                          10 ** rng.uniform(1, 4, 2000), [0.0, 3.999, 4.0, 4.001]])
+    u = rng.uniform(0, 1, lam.size)
+    u[:50] = 1e-15
+    u[50:100] = 1 - 1e-12
+    out = engine.poissinv(lam, u)
+    ref = np.array([oracle.poissinvcdf(a, b) for a, b in zip(lam, u)])
+    assert np.array_equal(out, ref)
+
+
+def test_chi2_quantile_device(engine, oracle):
+    rng = np.random.default_rng(2)
+    dof = np.concatenate([rng.uniform(0.05, 2, 500), rng.uniform(2, 200, 500)])
+    p = rng.uniform(0, 1, dof.size)
+    p[:20] = 0.999
+    p[20:40] = 1 - 1e-9
+    p[40:60] = 1e-6
+    out = engine.chi2_quantile(dof, p)
+    ref = np.array([oracle.chi2_quantile(a, b) for a, b in zip(dof, p)])
+    assert rel_err(out, ref, 1e-300) < 1e-12
+
+
+def _run_both(engine, oracle, model, schemes, inst, mode, cfg, S0=100.0, r=0.02, q=0.0):
+    g = engine.mc(model, schemes, inst, mode, cfg, S0, r, q)
+    c = oracle.mc(model, schemes, inst, mode, cfg, S0, r, q)
+    return g, c
+
+
+def test_heston_nv_essential_fp64(engine, oracle):
+    """Configuration 1 shape (Heston, NV, European call, Essential) at 2^16 paths."""
+    m = make_model(1, [H1])
+    inst = InstrumentSpec(strikes=[100.0], maturity=1.0)
+    cfg = mc_config(1 << 16, 50, seed=42, num_streams=4096)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NV], inst, capi.GREEK_ESSENTIAL, cfg)
+    compare(g, c, ESS, 1e-9)
+    assert g["total_steps"] == (1 << 16) * 50
+
+
+@pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-9), (capi.FP32, 2e-3)])
+def test_bates_nci_full_strip(engine, oracle, prec, tol):
+    """Configuration 2 shape (Bates, NCI with Feller violated, Merton, Full
+    CRN Greeks on a strike strip) at reduced size."""
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
+    inst = InstrumentSpec(strikes=np.linspace(70, 130, 16), maturity=0.5)
+    cfg = mc_config(1 << 14, 50, seed=42, precision=prec, num_streams=1024)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg)
+    if prec == capi.FP64:
+        compare(g, c, FULL, tol)
+    else:
+        # FP32 paths: prices and first-order Greeks against the FP64 oracle
+        compare(g, c, ["price", "delta"], tol, floor=1e-2)
+
+
+@pytest.mark.parametrize("scheme", [capi.SCHEME_NV, capi.SCHEME_NCI, capi.SCHEME_EXACT])
+@pytest.mark.parametrize("mode", [capi.GREEK_NONE, capi.GREEK_FULL])
+def test_schemes_fp64(engine, oracle, scheme, mode):
+    h = H1 if scheme == capi.SCHEME_NV else H_HI
+    m = make_model(1, [h])
+    inst = InstrumentSpec(strikes=[80.0, 100.0, 120.0], maturity=1.0)
+    cfg = mc_config(1 << 13, 20, seed=7, num_streams=512)
+    g, c = _run_both(engine, oracle, m, [scheme], inst, mode, cfg)
+    compare(g, c, FULL if mode == capi.GREEK_FULL else ["price", "price_std_err"], 1e-9)
+
+
+@pytest.mark.parametrize("jump", ["none", "merton", "kou"])
+@pytest.mark.parametrize("nf", [0, 2])
+def test_factors_and_jumps_fp64(engine, oracle, jump, nf):
+    jk, jp = {"none": (capi.JUMP_NONE, (0, 0, 0, 0)), "merton": merton(0.5, -0.1, 0.15),
+              "kou": kou(1.0, 0.4, 10.0, 5.0)}[jump]
+    if nf == 0:
+        m = make_model(0, vol=0.2, jump_kind=jk, jump=jp)
+        schemes = []
+    else:
+        m = make_model(2, [H1, H2], jump_kind=jk, jump=jp)
+        schemes = [capi.SCHEME_NV, capi.SCHEME_NCI]
+    inst = InstrumentSpec(strikes=[90.0, 100.0, 110.0], maturity=1.0)
+    cfg = mc_config(1 << 13, 16, seed=3, num_streams=512)
+    g, c = _run_both(engine, oracle, m, schemes, inst, capi.GREEK_FULL, cfg)
+    compare(g, c, FULL, 1e-9)
+
+
+@pytest.mark.parametrize("kind", ["geo", "arith", "lookback_fixed", "lookback_float",
+                                  "barrier_uo", "barrier_di_rebate"])
+@pytest.mark.parametrize("jump", [False, True])
+def test_path_dependent_fp64(engine, oracle, kind, jump):
+    jk, jp = merton(0.5, -0.1, 0.15) if jump else (capi.JUMP_NONE, (0, 0, 0, 0))
+    m = make_model(1, [H1], jump_kind=jk, jump=jp)
+    spec = dict(strikes=[95.0, 105.0], maturity=1.0)
+    if kind == "geo":
+        inst = InstrumentSpec(kind=capi.INSTR_GEO_ASIAN, **spec)
+    elif kind == "arith":
+        inst = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, **spec)
+    elif kind == "lookback_fixed":
+        inst = InstrumentSpec(kind=capi.INSTR_LOOKBACK, **spec)
+    elif kind == "lookback_float":
+        inst = InstrumentSpec(kind=capi.INSTR_LOOKBACK, fixed_strike=False, **spec)
+    elif kind == "barrier_uo":
+        inst = InstrumentSpec(kind=capi.INSTR_BARRIER, barrier=125.0, **spec)
+    else:
+        inst = InstrumentSpec(kind=capi.INSTR_BARRIER, barrier=85.0, barrier_dir=capi.BARRIER_DOWN,
+                              barrier_action=capi.BARRIER_IN, rebate_timing=capi.REBATE_AT_HIT,
+                              rebate=2.0, rebate_rate=0.02, **spec)
+    cfg = mc_config(1 << 12, 24, seed=11, num_streams=256)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NV], inst, capi.GREEK_FULL, cfg)
+    compare(g, c, FULL, 1e-9)
+
+
+def test_batch_matches_single_calls(engine):
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
+    strikes = np.linspace(70, 130, 8)
+    insts = [InstrumentSpec(strikes=strikes, maturity=T) for T in (0.1, 0.5, 1.0)]
+    cfgs = [mc_config(1 << 12, int(round(100 * T)), seed=42, num_streams=512) for T in (0.1, 0.5, 1.0)]
+    jobs = [(m, [capi.SCHEME_NCI], i, capi.GREEK_FULL, c, 100.0, 0.02, 0.0) for i, c in zip(insts, cfgs)]
+    batch = engine.mc_batch(jobs)
+    for (mm, sch, i, mode, c, S0, r, q), b in zip(jobs, batch):
+        single = engine.mc(mm, sch, i, mode, c, S0, r, q)
+        for f in FULL:
+            assert np.array_equal(np.asarray(single[f]), np.asarray(b[f])), f
+
+
+def test_mlmc_fp64_matches_oracle(engine, oracle):
+    m = make_model(1, [H1])
+    inst = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=130.0, maturity=1.0)
+    cfg = mlmc_config(epsilon=0.05, L_min=2, L_max=5, N0=2000, base_steps=4, num_streams=1024)
+    g = engine.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE, cfg, 100.0, 0.02, 0.0)
+    c = oracle.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE, cfg, 100.0, 0.02, 0.0)
+    assert g["n_levels"] == c["n_levels"]
+    assert np.array_equal(g["paths_per_level"], c["paths_per_level"])
+    assert rel_err(g["price"], c["price"]) < 1e-9
+    assert rel_err(g["mean_diffs"], c["mean_diffs"], 1e-10) < 1e-8
