@@ -45,6 +45,7 @@ class Oracle:
         lib.oracle_jump_poly.argtypes = [C.c_int, u64p]
         lib.oracle_apply_poly.argtypes = [u64p, u64p]
         lib.oracle_x_pow_mod.argtypes = [C.c_uint64, u64p]
+        lib.oracle_x_pow2_mod.argtypes = [C.c_int, u64p]
         lib.oracle_stream_state.argtypes = [C.c_uint64, C.c_int, C.c_uint64, u64p]
         for f in ("oracle_normcdfinv",):
             getattr(lib, f).restype = C.c_double
@@ -100,6 +101,11 @@ class Oracle:
     def x_pow_mod(self, n: int) -> np.ndarray:
         a, p = self._u64(4)
         self.lib.oracle_x_pow_mod(n, p)
+        return a
+
+    def x_pow2_mod(self, e: int) -> np.ndarray:
+        a, p = self._u64(4)
+        self.lib.oracle_x_pow2_mod(e, p)
         return a
 
     def apply_poly(self, state: np.ndarray, poly: np.ndarray) -> np.ndarray:

@@ -137,6 +137,12 @@ void oracle_x_pow_mod(uint64_t n, uint64_t *out4) {
   std::memcpy(out4, r.w, 32);
 }
 
+// x^(2^e) mod P (the xoshiro jump / long-jump polynomials for e = 128 / 192)
+void oracle_x_pow2_mod(int e, uint64_t *out4) {
+  const Poly r = x_pow2_mod(e, char_poly());
+  std::memcpy(out4, r.w, 32);
+}
+
 void oracle_stream_state(uint64_t seed, int layout, uint64_t s, uint64_t *out4) {
   Xoshiro g = stream_state(seed, layout, s);
   std::memcpy(out4, g.s, 32);

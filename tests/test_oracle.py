@@ -1,0 +1,363 @@
+"""Pins of the CPU FP64 oracle (no GPU needed).
+
+Each check is chosen so a plausible slip in the restatement -- a wrong shift
+or rotation, a dropped term, a wrong sign or index -- fails it:
+  * RNG: a pure-Python big-int restatement (tests/golden), the published
+    xoshiro256 JUMP and LONG_JUMP polynomials (which fix the state
+    transition), brute-force powers of x, the redraw rule of the uniform.
+  * Special functions: scipy (ndtri, chi2.ppf, gammainc, Poisson CDF).
+  * Schemes: closed-form CIR raw moments (reference benchmarks/cir_test.cpp).
+  * Pricer: Gil-Pelaez Fourier prices and Greeks (z-tests, the reference's
+    benchmark_fourier_vs_mc / EssentialGreekTest / mc_full_greek_verifier
+    checks), and the exact common-random-number identities of
+    barrier_test.cpp (in + out = European) and asian_test.cpp (1-step
+    arithmetic Asian = half a European at 2K - S0; AM-GM ordering).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import special, stats
+
+from myquantlib_amd import capi
+from myquantlib_amd.pricing import (Heston, InstrumentSpec, make_model, mc_config, merton, kou,
+                                    mlmc_config)
+from oracle import fourier
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+H1 = Heston(2.0, 0.04, 0.3, -0.7, 0.04)
+H_HI = Heston(2.0, 0.04, 1.0, -0.7, 0.04)
+H2 = Heston(1.0, 0.03, 0.25, -0.5, 0.02)
+Z_MAX = 4.0
+
+
+def u64(lst):
+    return np.array([int(x) for x in lst], dtype=np.uint64)
+
+
+# ---------------------------------------------------------------- RNG ----
+@pytest.mark.parametrize("seed", [0, 42, 999983, 2 ** 63 + 5])
+def test_xoshiro_golden(oracle, seed):
+    g = GOLDEN[f"xoshiro_seed_{seed}"]
+    st = oracle.seed_state(seed)
+    assert np.array_equal(st, u64(g["state"]))
+    assert np.array_equal(oracle.next(st, 64), u64(g["outputs"]))
+
+
+def test_jump_polynomials_are_published_constants(oracle):
+    assert np.array_equal(oracle.jump_poly(0), u64(GOLDEN["jump_2_128"]))
+    assert np.array_equal(oracle.x_pow2_mod(192), u64(GOLDEN["jump_2_192"]))
+    cp = oracle.char_poly()
+    assert int(cp[4]) == 1 and not np.any(cp[5:])  # degree exactly 256
+
+
+def test_jump_brute_force(oracle):
+    st = oracle.seed_state(7)
+    for n in (1, 2, 63, 1000):
+        a = oracle.apply_poly(st, oracle.x_pow_mod(n))
+        b = st.copy()
+        _ = oracle.next(b, 0)
+        ref = oracle.next(st, n + 1)  # state after n steps produces ref[n] next
+        assert oracle.next(a, 1)[0] == ref[n]
+    # J_1 = J_0 applied twice; stream 3 = J_0 J_1 (seed state)
+    s0 = oracle.seed_state(42)
+    twice = oracle.apply_poly(oracle.apply_poly(s0, oracle.jump_poly(0)), oracle.jump_poly(0))
+    assert np.array_equal(twice, oracle.apply_poly(s0, oracle.jump_poly(1)))
+    s3 = oracle.apply_poly(oracle.apply_poly(s0, oracle.jump_poly(0)), oracle.jump_poly(1))
+    assert np.array_equal(s3, oracle.stream_state(42, capi.STREAMS_JUMP, 3))
+
+
+def test_seeded_layout(oracle):
+    for s in (0, 1, 5):
+        assert np.array_equal(oracle.stream_state(42, capi.STREAMS_SEEDED, s),
+                              oracle.seed_state(42 + 999983 * s))
+
+
+def test_uniform_redraws_one(oracle):
+    # s0 = 0, s3 = ~0 makes the first output 2^64 - 1, which rounds to 1.0
+    st = np.array([0, 12345, 678910, (1 << 64) - 1], dtype=np.uint64)
+    raw = oracle.next(st, 2)
+    assert int(raw[0]) == (1 << 64) - 1
+    u = oracle.uniforms(st, 1)
+    assert u[0] == float(int(raw[1])) * 2.0 ** -64
+    st2 = oracle.seed_state(3)
+    assert np.array_equal(oracle.uniforms(st2, 5),
+                          np.array([float(int(x)) * 2.0 ** -64 for x in oracle.next(st2, 5)]))
+
+
+# ---------------------------------------------------- special functions ----
+def test_acklam_accuracy(oracle):
+    g = GOLDEN["ndtri"]
+    p = np.array(g["p"])
+    z = np.array(g["z"])
+    a = np.array([oracle.normcdfinv(x) for x in p])
+    # Acklam's published relative accuracy is 1.15e-9
+    assert np.max(np.abs(a - z) / np.maximum(np.abs(z), 1e-3)) < 1.2e-9
+    dense = np.linspace(1e-6, 1 - 1e-6, 20001)
+    a = np.array([oracle.normcdfinv(x) for x in dense])
+    assert np.max(np.abs(a - special.ndtri(dense))) < 1.5e-8
+
+
+def test_poissinvcdf_brackets(oracle):
+    rng = np.random.default_rng(5)
+    lam = np.concatenate([rng.uniform(0.01, 4, 400), rng.uniform(4, 60, 600), 10 ** rng.uniform(2, 4, 200)])
+    u = rng.uniform(0, 1, lam.size)
+    u[:40] = rng.uniform(0, 1e-6, 40)
+    u[40:80] = 1 - rng.uniform(0, 1e-6, 40)
+    for l_, u_ in zip(lam, u):
+        n = oracle.poissinvcdf(l_, u_)
+        assert n == int(n) and n >= 0
+        lo = stats.poisson.cdf(n - 1, l_) if n > 0 else 0.0
+        hi = stats.poisson.cdf(n, l_)
+        assert lo < u_ + 1e-12 and u_ <= hi + 1e-12, (l_, u_, n)
+    assert oracle.poissinvcdf(3.0, 0.0) == 0.0
+    assert math.isnan(oracle.poissinvcdf(3.0, 1.0))
+
+
+def test_poisson_inversion_matches_ppf(oracle):
+    for mean in (0.005, 0.5, 2.0, 7.5):
+        n = 4000
+        x = oracle.poisson_samples(mean, n, 99)
+        u = oracle.uniforms(oracle.seed_state(99), n)
+        ref = stats.poisson.ppf(u, mean)
+        cdf_gap = np.abs(stats.poisson.cdf(ref, mean) - u)
+        ok = cdf_gap > 1e-12
+        assert np.array_equal(x[ok], ref[ok].astype(np.int32))
+
+
+def test_chi2_quantile_golden(oracle):
+    g = GOLDEN["chi2"]
+    got = np.array([oracle.chi2_quantile(d, p) for d, p in zip(g["dof"], g["p"])])
+    ref = np.array(g["q"])
+    assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) < 1e-12
+    assert oracle.chi2_quantile(0.32, 0.0) == 0.0
+
+
+def test_incomplete_gamma_golden(oracle):
+    g = GOLDEN["igamma"]
+    P = np.array([oracle.gamma_p(a, x) for a, x in zip(g["a"], g["x"])])
+    Q = np.array([oracle.gamma_q(a, x) for a, x in zip(g["a"], g["x"])])
+    assert np.max(np.abs(P - np.array(g["P"]))) < 1e-13
+    assert np.max(np.abs(Q - np.array(g["Q"]))) < 1e-13
+
+
+# ---------------------------------------------------------- CIR moments ----
+def cir_raw_moments(h, t):
+    """Raw moments 1..3 of v_t: v_t = c X, X ~ noncentral chi2(d, lam)."""
+    e = math.exp(-h.kappa * t)
+    c = h.sigma ** 2 * (1 - e) / (4 * h.kappa)
+    d = 4 * h.kappa * h.theta / h.sigma ** 2
+    lam = 4 * h.kappa * e * h.v0 / (h.sigma ** 2 * (1 - e))
+    m1 = d + lam
+    m2 = (d + lam) ** 2 + 2 * (d + 2 * lam)
+    m3 = (d + lam) ** 3 + 6 * (d + lam) * (d + 2 * lam) + 8 * (d + 3 * lam)
+    return c * m1, c * c * m2, c ** 3 * m3
+
+
+@pytest.mark.parametrize("scheme,h,t,steps", [
+    (capi.SCHEME_NCI, H_HI, 1.0, 1), (capi.SCHEME_NCI, H_HI, 0.01, 1),
+    (capi.SCHEME_EXACT, H_HI, 1.0, 1), (capi.SCHEME_EXACT, H_HI, 0.01, 1),
+    (capi.SCHEME_EXACT, Heston(1.5, 0.05, 0.2, 0.0, 0.0), 2.0, 1),
+    (capi.SCHEME_NCI, Heston(0.5, 0.09, 0.1, 0.0, 0.01), 0.5, 1),
+    (capi.SCHEME_NV, H1, 1.0, 64), (capi.SCHEME_NV, Heston(3.0, 0.02, 0.4, 0, 0.06), 2.0, 64),
+])
+def test_cir_moments(oracle, scheme, h, t, steps):
+    n = 200_000
+    v = oracle.cir_terminal(scheme, h, t / steps, steps, n, seed=17)
+    ref = cir_raw_moments(h, t)
+    for k, m in enumerate(ref, start=1):
+        x = v ** k
+        z = (x.mean() - m) / (x.std() / math.sqrt(n))
+        if scheme == capi.SCHEME_NV and k == 3:
+            continue  # weak second order: third moment carries O(dt^2) bias
+        assert abs(z) < Z_MAX, (k, x.mean(), m, z)
+
+
+# --------------------------------------------------- pricer vs Fourier ----
+def fourier_model(n_factors, heston=(), vol=0.0, jump=None):
+    jk, jp = jump if jump else (0, (0, 0, 0, 0))
+    return fourier.AffineModel(n_factors, heston, vol, jk, jp)
+
+
+CASES = {
+    "bs": (0, [], 0.25, None, []),
+    "merton": (0, [], 0.2, merton(0.5, -0.1, 0.15), []),
+    "heston_nv": (1, [H1], 0, None, [capi.SCHEME_NV]),
+    "heston_nci": (1, [H_HI], 0, None, [capi.SCHEME_NCI]),
+    "heston_exact": (1, [H_HI], 0, None, [capi.SCHEME_EXACT]),
+    "bates_nci": (1, [H_HI], 0, merton(0.5, -0.1, 0.15), [capi.SCHEME_NCI]),
+    "bateskou_nv": (1, [Heston(2.0, 0.04, 0.5, -0.7, 0.04)], 0, kou(1.0, 0.4, 10.0, 5.0),
+                    [capi.SCHEME_NV]),
+    "dheston": (2, [H1, H2], 0, None, [capi.SCHEME_NV, capi.SCHEME_NCI]),
+    "dbateskou": (2, [H1, H2], 0, kou(1.0, 0.4, 10.0, 5.0), [capi.SCHEME_EXACT, capi.SCHEME_NV]),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mc_vs_fourier(oracle, name):
+    nf, hs, vol, jump, schemes = CASES[name]
+    jk, jp = jump if jump else (capi.JUMP_NONE, (0, 0, 0, 0))
+    m = make_model(nf, hs, vol=vol, jump_kind=jk, jump=jp)
+    fm = fourier_model(nf, hs, vol, jump)
+    strikes = [85.0, 100.0, 115.0]
+    T, r, q = 0.75, 0.03, 0.01
+    res = oracle.mc(m, schemes, InstrumentSpec(strikes=strikes, maturity=T), capi.GREEK_NONE,
+                    mc_config(1 << 17, 60, seed=423), 100.0, r, q)
+    for i, K in enumerate(strikes):
+        ref = fourier.price(fm, 100.0, K, T, r, q)
+        z = (res["price"][i] - ref) / res["price_std_err"][i]
+        assert abs(z) < Z_MAX, (K, res["price"][i], ref, z)
+
+
+@pytest.mark.parametrize("payoff,otype", [(capi.PAYOFF_CASH, capi.CALL), (capi.PAYOFF_ASSET, capi.PUT),
+                                          (capi.PAYOFF_VANILLA, capi.PUT)])
+def test_digitals_vs_fourier(oracle, payoff, otype):
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H1], jump_kind=jk, jump=jp)
+    fm = fourier_model(1, [H1], 0, (jk, jp))
+    strikes = [90.0, 100.0, 110.0]
+    res = oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=strikes, maturity=1.0, payoff=payoff,
+                                                        option_type=otype),
+                    capi.GREEK_NONE, mc_config(1 << 17, 50, seed=5), 100.0, 0.02, 0.0)
+    for i, K in enumerate(strikes):
+        ref = fourier.price(fm, 100.0, K, 1.0, 0.02, 0.0, payoff, otype)
+        z = (res["price"][i] - ref) / res["price_std_err"][i]
+        assert abs(z) < Z_MAX, (K, z)
+
+
+def test_full_greeks_vs_fourier(oracle):
+    """Full CRN Greeks on identical draws against finite differences of the
+    semi-analytic price (reference mc_full_greek_verifier.cpp)."""
+    h = H1
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [h], jump_kind=jk, jump=jp)
+    strikes = [90.0, 100.0, 110.0]
+    S0, T, r, q = 100.0, 1.0, 0.03, 0.0
+    res = oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=strikes, maturity=T), capi.GREEK_FULL,
+                    mc_config(1 << 17, 50, seed=77, t_bump=1e-2), S0, r, q)
+
+    def fp(S=S0, TT=T, rr=r, v0=h.v0, K=100.0):
+        hh = Heston(h.kappa, h.theta, h.sigma, h.rho, v0)
+        return fourier.price(fourier_model(1, [hh], 0, (jk, jp)), S, K, TT, rr, q)
+
+    for i, K in enumerate(strikes):
+        dS, dv, dT, dr = 0.05, 1e-4, 1e-3, 1e-4
+        delta = (fp(S=S0 + dS, K=K) - fp(S=S0 - dS, K=K)) / (2 * dS)
+        gamma = (fp(S=S0 + dS, K=K) - 2 * fp(K=K) + fp(S=S0 - dS, K=K)) / dS ** 2
+        vega = (fp(v0=h.v0 + dv, K=K) - fp(v0=h.v0 - dv, K=K)) / (2 * dv) * 2 * math.sqrt(h.v0)
+        theta = (fp(TT=T + dT, K=K) - fp(TT=T - dT, K=K)) / (2 * dT)
+        rho = (fp(rr=r + dr, K=K) - fp(rr=r - dr, K=K)) / (2 * dr)
+        for name, ref in (("delta", delta), ("gamma", gamma), ("theta", theta), ("rho", rho)):
+            z = (res[name][i] - ref) / res[name + "_std_err"][i]
+            assert abs(z) < Z_MAX, (name, K, res[name][i], ref, z)
+        z = (res["vega"][0][i] - vega) / res["vega_std_err"][0][i]
+        assert abs(z) < Z_MAX, ("vega", K, z)
+        assert res["vega"][1][i] == 0.0 and res["vanna"][1][i] == 0.0
+
+
+# ------------------------------------------------ exact CRN identities ----
+@pytest.mark.parametrize("scheme,h", [(capi.SCHEME_NV, H1), (capi.SCHEME_EXACT, H_HI)])
+def test_barrier_in_plus_out_is_european(oracle, scheme, h):
+    m = make_model(1, [h])
+    strikes = [90.0, 100.0, 110.0]
+    cfg = mc_config(1 << 13, 52, seed=9, num_streams=64)
+    S0, r, q = 100.0, 0.02, 0.0
+    euro = oracle.mc(m, [scheme], InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=strikes, fixed_strike=False,
+                                                 maturity=1.0), capi.GREEK_NONE, cfg, S0, r, q)
+    del euro
+    out = {}
+    for act in (capi.BARRIER_IN, capi.BARRIER_OUT):
+        out[act] = oracle.mc(m, [scheme], InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=strikes, barrier=115.0,
+                                                         barrier_action=act, maturity=1.0),
+                             capi.GREEK_ESSENTIAL, cfg, S0, r, q)
+    # European priced on the same draws: a barrier no path can reach
+    never = oracle.mc(m, [scheme], InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=strikes, barrier=1e9,
+                                                  maturity=1.0), capi.GREEK_ESSENTIAL, cfg, S0, r, q)
+    eu = oracle.mc(m, [scheme], InstrumentSpec(strikes=strikes, maturity=1.0), capi.GREEK_ESSENTIAL, cfg, S0, r, q)
+    for f in ("price", "delta", "gamma"):
+        tot = out[capi.BARRIER_IN][f] + out[capi.BARRIER_OUT][f]
+        assert np.max(np.abs(tot - never[f])) < 1e-10
+        assert np.max(np.abs(tot - eu[f])) < 1e-10
+
+
+def test_asian_one_step_identity_and_amgm(oracle):
+    m = make_model(1, [H1])
+    cfg1 = mc_config(1 << 14, 1, seed=21, num_streams=128)
+    K, S0 = 100.0, 100.0
+    a = oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[K], maturity=1.0),
+                  capi.GREEK_NONE, cfg1, S0, 0.02, 0.0)
+    e = oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=[2 * K - S0], maturity=1.0), capi.GREEK_NONE,
+                  cfg1, S0, 0.02, 0.0)
+    assert abs(a["price"][0] - 0.5 * e["price"][0]) < 1e-10
+    cfg = mc_config(1 << 13, 64, seed=22, num_streams=128)
+    strikes = [80.0, 100.0, 120.0]
+    ar = oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=strikes, maturity=1.0),
+                   capi.GREEK_NONE, cfg, S0, 0.02, 0.0)
+    ge = oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(kind=capi.INSTR_GEO_ASIAN, strikes=strikes, maturity=1.0),
+                   capi.GREEK_NONE, cfg, S0, 0.02, 0.0)
+    assert np.all(ar["price"] >= ge["price"])
+
+
+def test_in_flight_continuity(oracle):
+    m = make_model(0, vol=0.2)
+    cfg = mc_config(1 << 13, 50, seed=4, num_streams=64)
+    fresh = oracle.mc(m, [], InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=1.0),
+                      capi.GREEK_NONE, cfg, 100.0, 0.02, 0.0)
+    tiny = oracle.mc(m, [], InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=1.0,
+                                           past_avg_S=100.0, past_avg_log=math.log(100.0), t_elapsed=1e-9),
+                     capi.GREEK_NONE, cfg, 100.0, 0.02, 0.0)
+    assert abs(fresh["price"][0] - tiny["price"][0]) <= 1e-4
+
+
+def test_lookback_floating_bs_closed_form(oracle):
+    """Floating-strike lookback call under Black-Scholes against the
+    Goldman-Sosin-Gatto closed form (lookback_test.cpp:65-86); discrete
+    monitoring bias shrinks with the step count, so the bound is loose."""
+    S0, r, q, vol, T = 100.0, 0.05, 0.0, 0.3, 0.5
+    res = oracle.mc(make_model(0, vol=vol), [], InstrumentSpec(kind=capi.INSTR_LOOKBACK, fixed_strike=False,
+                                                               strikes=[0.0], maturity=T),
+                    capi.GREEK_NONE, mc_config(1 << 14, 1024, seed=1, num_streams=256), S0, r, q)
+    b = r - q
+    a1 = (math.log(1.0) + (b + 0.5 * vol ** 2) * T) / (vol * math.sqrt(T))
+    a2 = a1 - vol * math.sqrt(T)
+    N = stats.norm.cdf
+    gsg = (S0 * math.exp(-q * T) * N(a1) - S0 * math.exp(-r * T) * N(a2)
+           + S0 * math.exp(-r * T) * vol ** 2 / (2 * b) * (N(-a1 + 2 * b * math.sqrt(T) / vol)
+                                                           - math.exp(b * T) * N(-a1)))
+    # discrete monitoring makes the MC price lower (the minimum is missed)
+    bias = gsg - res["price"][0]
+    assert -3 * res["price_std_err"][0] < bias < 0.03 * gsg
+
+
+def test_thread_count_does_not_change_results(oracle):
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
+    inst = InstrumentSpec(strikes=[90.0, 110.0], maturity=0.5)
+    cfg = mc_config(1 << 12, 50, seed=42, num_streams=96)
+    a = oracle.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg, 100.0, 0.02, 0.0, nthreads=1)
+    b = oracle.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg, 100.0, 0.02, 0.0, nthreads=5)
+    for f in ("price", "delta", "gamma", "theta", "rho", "charm"):
+        assert np.allclose(a[f], b[f], rtol=1e-12, atol=1e-14)
+
+
+def test_nv_feller_error(oracle):
+    from myquantlib_amd.pricing import EngineError
+    m = make_model(1, [H_HI])
+    with pytest.raises(EngineError) as e:
+        oracle.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=[100.0]), capi.GREEK_NONE, mc_config(16, 4), 100.0,
+                  0.0, 0.0)
+    assert e.value.code == capi.E_NV_FELLER
+    assert "SchemeNV violation" in str(e.value)
+
+
+def test_mlmc_vs_fourier(oracle):
+    m = make_model(1, [H1])
+    res = oracle.mlmc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=[100.0], maturity=1.0), capi.GREEK_NONE,
+                      mlmc_config(epsilon=0.01, L_min=2, L_max=6, N0=4000, base_steps=2, num_streams=256),
+                      100.0, 0.02, 0.0)
+    ref = fourier.price(fourier_model(1, [H1]), 100.0, 100.0, 1.0, 0.02, 0.0)
+    z = (res["price"] - ref) / res["price_std_err"]
+    assert abs(z) < Z_MAX
+    assert res["n_levels"] >= 3 and np.all(res["paths_per_level"][:res["n_levels"]] > 0)
