@@ -1,0 +1,185 @@
+"""Headline benchmark: the Bates strip.
+
+Workload (one "step"): one pricing of the whole strip through
+myql_mc_calculate_batch -- 64 strikes uniform in [70, 130] x 8 maturities
+{0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0}, each maturity a separate
+MonteCarloPricer::calculate call (2^24 paths, 100 steps per year, seed 42,
+GreekMode::Full with common random numbers) under Bates with SchemeNCI:
+kappa 2, theta 0.04, sigma 1.0 (Feller violated), rho -0.7, v0 0.04, Merton
+jumps lambda 0.5, mu -0.1, delta 0.15; S0 100, r 0.02, q 0. Paths are FP32,
+sums FP64. Inputs are synthetic (no market data).
+
+Metric: path-steps/s = (sum over maturities of paths x steps x 7 Greek
+variants) / wall time, the 7 variants being the reference algorithm's
+base, v0 up/down, T up/down and r up/down replays. Also reported: ms to a
+1e-4 relative standard error for plain FP32 and plain FP64 GPU MC.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths-log2 24] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Bates strip: path-steps/s; ms to 1e-4 rel std-err, plain vs learned CV"
+MATURITIES = [0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0]
+STRIKES = np.linspace(70.0, 130.0, 64)
+S0, R, Q = 100.0, 0.02, 0.0
+VARIANTS = 7
+# Algorithmic FP operations per path-step of this workload (FMA = 2): see
+# DESIGN.md, "Roofline of the path kernel".
+FLOP_PER_PATH_STEP = 90.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def model():
+    from myquantlib_amd.pricing import Heston, make_model, merton
+    jk, jp = merton(0.5, -0.1, 0.15)
+    return make_model(1, [Heston(2.0, 0.04, 1.0, -0.7, 0.04)], jump_kind=jk, jump=jp)
+
+
+def jobs(paths: int, precision: int, layout: int = 0, streams: int = 0):
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec, mc_config
+    m = model()
+    out = []
+    for T in MATURITIES:
+        inst = InstrumentSpec(strikes=STRIKES, maturity=T)
+        cfg = mc_config(paths, int(round(100 * T)), seed=42, precision=precision,
+                        stream_layout=layout, num_streams=streams)
+        out.append((m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg, S0, R, Q))
+    return out
+
+
+def path_steps(paths: int) -> int:
+    return sum(paths * int(round(100 * T)) * VARIANTS for T in MATURITIES)
+
+
+def ms_to_target(results, t_run_ms: float, target: float = 1e-4):
+    worst = 0.0
+    for res in results:
+        p, se = res["price"], res["price_std_err"]
+        ok = p >= 0.1
+        if np.any(ok):
+            worst = max(worst, float(np.max(se[ok] / p[ok])))
+    return t_run_ms * (worst / target) ** 2, worst
+
+
+def cpu_baseline(target_s: float = 12.0):
+    """The oracle compiled with the reference's flags, OpenMP over host cores
+    in the reference's structure (per-thread stream seeded seed + tid*999983,
+    static partition). Sample: the T = 1 maturity of the strip (100 steps,
+    Full CRN Greeks, 64 strikes), path count sized to ~target_s seconds."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec, mc_config
+    from oracle.oracle import Oracle
+    cores = int(os.environ.get("MYQL_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    o = Oracle(fast=True)
+    m = model()
+    inst = InstrumentSpec(strikes=STRIKES, maturity=1.0)
+
+    def run(paths):
+        cfg = mc_config(paths, 100, seed=42, stream_layout=capi.STREAMS_SEEDED, num_streams=cores)
+        t0 = time.perf_counter()
+        o.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg, S0, R, Q, nthreads=cores)
+        return time.perf_counter() - t0
+
+    run(cores * 16)  # NCI table build and warm-up
+    pilot = cores * 256
+    t = run(pilot)
+    paths = int(max(pilot, min(1 << 22, pilot * target_s / max(t, 1e-3))))
+    t = run(paths)
+    rate = paths * 100 * VARIANTS / t
+    return {"value": rate, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": f"T=1 maturity of the strip: {paths} paths x 100 steps x 7 CRN variants, "
+                      f"64 strikes, Full Greeks, FP64, {t:.1f} s on {platform.processor() or 'host'}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--paths-log2", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described "
+                                   "in DESIGN.md but not launched"}))
+        sys.exit(2)
+
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import Engine
+    eng = Engine(0)
+    paths = 1 << args.paths_log2
+    J32 = jobs(paths, capi.FP32)
+    for _ in range(args.warmup):
+        eng.mc_batch(J32)
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = eng.mc_batch(J32)
+        dev_ms.append(res[0]["device_ms"])
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    total = path_steps(paths)
+    value = total / (wall / args.steps)
+    t32, worst32 = ms_to_target(res, ms_per_step)
+    print(f"# fp32 strip: {ms_per_step:.1f} ms/step, device {np.mean(dev_ms):.1f} ms, "
+          f"worst rel se {worst32:.2e}", file=sys.stderr, flush=True)
+
+    ms64 = None
+    if not args.no_fp64:
+        J64 = jobs(paths, capi.FP64)
+        eng.mc_batch(J64)
+        t0 = time.perf_counter()
+        r64 = eng.mc_batch(J64)
+        w64 = 1e3 * (time.perf_counter() - t0)
+        ms64, worst64 = ms_to_target(r64, w64)
+        print(f"# fp64 strip: {w64:.1f} ms, worst rel se {worst64:.2e}", file=sys.stderr, flush=True)
+
+    kern_s = np.mean(dev_ms) / 1e3
+    achieved = total * FLOP_PER_PATH_STEP / kern_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "bates_strip: Bates SchemeNCI (Feller violated) + Merton, 64 strikes x 8 "
+                        f"maturities, 2^{args.paths_log2} paths per maturity, 100 steps/yr, "
+                        "GreekMode::Full CRN, FP32 paths, FP64 sums, seed 42",
+            "paths_per_maturity": paths,
+            "path_steps_per_step": total,
+        },
+        "ms_to_1e-4_rel_se": {"plain_fp32": t32, "plain_fp64": ms64, "learned_cv_fp32": None},
+        "roofline": {"bound": "valu_fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": float(np.mean(dev_ms)),
+                     "flop_per_path_step": FLOP_PER_PATH_STEP},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
