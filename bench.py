@@ -124,13 +124,17 @@ def main():
     eng = Engine(0)
     paths = 1 << args.paths_log2
     J32 = jobs(paths, capi.FP32)
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
         eng.mc_batch(J32)
+        print(f"# warmup {i}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
     dev_ms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         res = eng.mc_batch(J32)
         dev_ms.append(res[0]["device_ms"])
+        print(f"# step {i}: {time.perf_counter() - t0:.3f} s elapsed, device {dev_ms[-1]:.1f} ms",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     total = path_steps(paths)

@@ -151,12 +151,25 @@ __device__ __forceinline__ double normcdfinv<double>(double p) { return normcdfi
 template <>
 __device__ __forceinline__ float normcdfinv<float>(double p) { return normcdfinv_f(p); }
 
+// Coefficients of the Temme correction in Giles' inverse Poisson CDF, read
+// through scalar loads from constant memory.
+__constant__ static const double kTemmeB1[13] = {
+    8.0995211567045583e-16,  -5.1391118342426808e-16, -1.3923887224181616e-13, 1.1004392031956284e-13,
+    2.4361948020667402e-11,  -2.5514193994946487e-11, -4.3820360184533521e-09, 6.7078535434015332e-09,
+    8.2967113409530833e-07,  -2.1854485106799979e-06, -0.00017875514403292177, 0.0011574074074074078,
+    0.083333333333333329};
+__constant__ static const double kTemmeB0[13] = {
+    -1.9752288294349411e-15, 2.8534893807047458e-14,  3.3717632624009806e-13,  -5.0276692801141763e-12,
+    -5.8307721325504166e-11, 9.1476995822367933e-10,  1.0261809784240299e-08,  -1.7665952736826086e-07,
+    -1.8540622107151585e-06, 3.9192631785224383e-05,  0.00035273368606701921,  -0.014814814814814815,
+    -0.33333333333333331};
+
 // ---------------------------------------------------------------------------
 // Giles' inverse Poisson CDF (reference Numerics.hpp:49-272; ACM TOMS Alg.
 // 955) in FP64. Decides the integer N that fixes the CIR mixture of the NCI
 // and Exact schemes, so it runs in double on both path precisions.
 // ---------------------------------------------------------------------------
-__device__ static double poissinv(double lam, double U) {
+__device__ static __noinline__ double poissinv(double lam, double U) {
   if (U <= 0.0) return 0.0;
   if (U >= 1.0) return __builtin_nan("");
   const double V = 1.0 - U;
@@ -200,17 +213,9 @@ __device__ static double poissinv(double lam, double U) {
         double Eta = X * ilam;
         Eta = sqrt(2.0 * (1.0 - Eta + Eta * log(Eta)) / Eta);
         if (X > lam) Eta = -Eta;
-        // Temme expansion coefficients: odd / even Horner chains in 1/X
-        const double b1[13] = {8.0995211567045583e-16,  -5.1391118342426808e-16, -1.3923887224181616e-13,
-                               1.1004392031956284e-13,  2.4361948020667402e-11,  -2.5514193994946487e-11,
-                               -4.3820360184533521e-09, 6.7078535434015332e-09,  8.2967113409530833e-07,
-                               -2.1854485106799979e-06, -0.00017875514403292177, 0.0011574074074074078,
-                               0.083333333333333329};
-        const double b0[13] = {-1.9752288294349411e-15, 2.8534893807047458e-14,  3.3717632624009806e-13,
-                               -5.0276692801141763e-12, -5.8307721325504166e-11, 9.1476995822367933e-10,
-                               1.0261809784240299e-08,  -1.7665952736826086e-07, -1.8540622107151585e-06,
-                               3.9192631785224383e-05,  0.00035273368606701921,  -0.014814814814814815,
-                               -0.33333333333333331};
+        // Temme expansion coefficients (kTemmeB1 / kTemmeB0): odd / even
+        // Horner chains in 1/X
+        const double *b1 = kTemmeB1, *b0 = kTemmeB0;
         double B1 = b1[0], Sm = B1, B0 = b0[0];
         Sm = B0 + Sm * Eta;
         double mult = 25.0;
@@ -324,7 +329,7 @@ __device__ __forceinline__ R exponential(R rate, Rng &g) {
 // 368-369). Power series below a+1, Legendre continued fraction (Lentz)
 // above; Halley steps inside a shrinking bracket.
 // ---------------------------------------------------------------------------
-__device__ static double igam_series(double a, double x) {
+__device__ static __noinline__ double igam_series(double a, double x) {
   double sum = 1.0, term = 1.0, ap = a;
   for (int n = 1; n < 2000; ++n) {
     ap += 1.0;
@@ -335,7 +340,7 @@ __device__ static double igam_series(double a, double x) {
   return exp(a * log(x) - x - lgamma(a + 1.0)) * sum;
 }
 
-__device__ static double igamc_cf(double a, double x) {
+__device__ static __noinline__ double igamc_cf(double a, double x) {
   const double tiny = 1e-300;
   double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
   for (int i = 1; i < 2000; ++i) {
@@ -353,7 +358,7 @@ __device__ static double igamc_cf(double a, double x) {
   return exp(a * log(x) - x - lgamma(a)) * h;
 }
 
-__device__ static double chi2_quantile(double dof, double p) {
+__device__ static __noinline__ double chi2_quantile(double dof, double p) {
   if (p <= 0.0) return 0.0;
   if (p >= 1.0) return __builtin_inf();
   const double a = 0.5 * dof;

@@ -271,7 +271,7 @@ struct myql_ctx {
   DevBuf d_polys;
   std::map<std::tuple<uint64_t, int, uint64_t>, std::unique_ptr<DevBuf>> streams;
   std::map<std::tuple<uint64_t, int, int>, std::unique_ptr<DevBuf>> nci;  // (d bits, rows, prec)
-  DevBuf jobs, units, partial, sums, meta, counter, tmp;
+  DevBuf jobs, units, partial, sums, meta, counter, tmp, accg;
   DevBuf mlmc_streams, mlmc_sums, mlmc_job;
 };
 
@@ -470,6 +470,7 @@ struct JobSpec {
 
 struct BuiltJob {
   int nf, nv;        // kernel instantiation
+  bool pd;
   int n_slots;       // work units
   uint64_t cost;     // per-unit cost estimate
   int nk;
@@ -656,7 +657,7 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
       J.group_size[i] = 1;
     }
   }
-  if (!mc_supported(nf, nv)) throw Fail(MYQL_E_UNSUPPORTED, "no kernel for this factor count / variant set");
+  if (!mc_supported(nf, nv, pd)) throw Fail(MYQL_E_UNSUPPORTED, "no kernel for this factor count / variant set");
   uint64_t S = cfg.num_streams ? cfg.num_streams : std::min<uint64_t>(65536, M);
   if (S > M) S = M;
   J.M = M;
@@ -674,6 +675,7 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
   BuiltJob b;
   b.nf = nf;
   b.nv = nv;
+  b.pd = pd;
   b.n_slots = static_cast<int>((S + BLOCK - 1) / BLOCK);
   const uint64_t per_stream = (M + S - 1) / S;
   b.cost = per_stream * steps * static_cast<uint64_t>(nsim) * (nf == 0 ? 1 : nf) + per_stream * js.nk;
@@ -732,9 +734,10 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   const Job<R> *d_jobs = c->jobs.as<Job<R>>();
 
   // units grouped by kernel instantiation, most expensive first
-  std::map<std::pair<int, int>, std::vector<Unit>> groups;
+  std::map<std::tuple<int, int, bool>, std::vector<Unit>> groups;
   for (size_t i = 0; i < jobs.size(); ++i)
-    for (int b = 0; b < n_slots[i]; ++b) groups[{built[i].nf, built[i].nv}].push_back({static_cast<int>(i), b});
+    for (int b = 0; b < n_slots[i]; ++b)
+      groups[{built[i].nf, built[i].nv, built[i].pd}].push_back({static_cast<int>(i), b});
   size_t max_units = 0;
   for (auto &kv : groups) max_units = std::max(max_units, kv.second.size());
   c->units.reserve(groups.size() * max_units * sizeof(Unit) + 64);
@@ -750,6 +753,7 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   }
   HIPCHECK(hipMemcpyAsync(c->units.p, all_units.data(), all_units.size() * sizeof(Unit), hipMemcpyHostToDevice,
                           c->stream));
+  c->accg.reserve(all_units.size() * 2 * NSTAT * BLOCK * sizeof(double));
 
   // meta for the reduction: per job the element offset of its partials,
   // its unit count, strike count and output offset
@@ -768,17 +772,19 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   size_t gi = 0;
   for (auto &kv : groups) {
-    const int nf = kv.first.first, nv = kv.first.second;
+    const int nf = std::get<0>(kv.first), nv = std::get<1>(kv.first);
+    const bool pdk = std::get<2>(kv.first);
     int nfields = 1;
     for (size_t i = 0; i < jobs.size(); ++i)
-      if (built[i].nf == nf && built[i].nv == nv) nfields = std::max(nfields, jobs[i].n_fields);
+      if (built[i].nf == nf && built[i].nv == nv && built[i].pd == pdk) nfields = std::max(nfields, jobs[i].n_fields);
     const size_t smem = static_cast<size_t>(nfields) * MAXV * BLOCK * sizeof(R);
     if (smem > 64 * 1024) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds 64 KiB of LDS");
-    const int bpc = mc_blocks_per_cu<R>(nf, nv, smem);
+    const int bpc = mc_blocks_per_cu<R>(nf, nv, pdk, smem);
     const int n_units = static_cast<int>(ranges[gi].second);
     const int grid = std::max(1, std::min(n_units, bpc * c->n_cu));
-    HIPCHECK(launch_mc<R>(nf, nv, d_jobs, c->units.as<Unit>() + ranges[gi].first, n_units,
-                          c->counter.as<unsigned>() + 16 * gi, smem, grid, c->stream));
+    HIPCHECK(launch_mc<R>(nf, nv, pdk, d_jobs, c->units.as<Unit>() + ranges[gi].first, n_units,
+                          c->counter.as<unsigned>() + 16 * gi,
+                          c->accg.as<double>() + ranges[gi].first * 2 * NSTAT * BLOCK, smem, grid, c->stream));
     ++gi;
   }
   {
@@ -1053,11 +1059,13 @@ void run_specs(myql_ctx *c, const std::vector<JobSpec> &in_specs) {
 }  // namespace
 
 namespace myql {
-bool mc_supported(int nf, int nv) {
+bool mc_supported(int nf, int nv, bool pd) {
+  // lockstep widths: Europeans simulate base, vol bumps and maturity bumps
+  // (5, or 7 with two factors) and derive the rate bumps; path-dependent
+  // lockstep adds the two rate-bumped paths (7 or 9)
   if (nv == 1) return nf >= 0 && nf <= 2;
-  if (nf == 0 || nf == 1) return nv == 5 || nv == 7;
-  if (nf == 2) return nv == 7 || nv == 9;
-  return false;
+  if (!pd) return nv == (nf == 2 ? 7 : 5);
+  return nv == (nf == 2 ? 9 : 7);
 }
 }  // namespace myql
 

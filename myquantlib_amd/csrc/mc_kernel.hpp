@@ -311,7 +311,9 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
 }
 
 // Simulate one variant group for the lane's current sample and stage it.
-template <typename R, int NF, int NV>
+// PD selects path-dependent trackers; European kernels carry no tracker
+// state at all.
+template <typename R, int NF, int NV, bool PD>
 __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfirst, Rng &g,
                                           R *__restrict__ stage, int tid) {
   constexpr int NFX = NF > 0 ? NF : 1;
@@ -323,18 +325,17 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
     P[k] = &J->var[vid[k]];
   }
   R x[NV], v[NV][NFX];
-  Trk<R> tr[NV];
+  Trk<R> tr[PD ? NV : 1];
   const R x0 = static_cast<R>(J->logS0);
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     x[k] = x0;
 #pragma unroll
     for (int f = 0; f < NFX; ++f) v[k][f] = P[k]->v0[f];
-    trk_init(J, tr[k], x0, P[k]->dt);
+    if constexpr (PD) trk_init(J, tr[k], x0, P[k]->dt);
   }
-  const bool pd = J->pathdep;
   const unsigned steps = static_cast<unsigned>(J->steps);
-  if (NF == 0 && !pd) {
+  if (NF == 0 && !PD) {
     // zero-factor European: one step to maturity (ASVJstepper.hpp:145-161)
     const R G = normcdfinv<R>(uniform(g));
 #pragma unroll
@@ -363,7 +364,7 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
           for (int f = 0; f < NF; ++f) v[k][f] = vn[k][f];
         }
       }
-      if (pd) {
+      if constexpr (PD) {
         if (J->jump_kind != 0) {
           if constexpr (NV == 1) x[0] += log_jump(J, P[0], g);
         }
@@ -371,37 +372,44 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
         for (int k = 0; k < NV; ++k) trk_update(J, tr[k], x[k], P[k]->dt);
       }
     }
-    if (!pd && NF > 0) euro_jump<R, NV>(J, P, x, g);
+    if constexpr (!PD && NF > 0) euro_jump<R, NV>(J, P, x, g);
   }
 #pragma unroll
-  for (int k = 0; k < NV; ++k)
-    finalize_stage(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
+  for (int k = 0; k < NV; ++k) {
+    if constexpr (PD) {
+      finalize_stage(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
+    } else {
+      stage[(F_SPOT * MAXV + vid[k]) * BLOCK + tid] = dexp<R>(x[k]);
+    }
+  }
 }
 
 // ---- payoffs (Payoffs.hpp:14-69) and per-variant payoff triples -----------
+// Digital payoffs, erfc-smoothed with bandwidth eps in Greek modes; kept out
+// of line so the strike loop of vanilla strips stays small.
+__device__ static __noinline__ double digital_payoff(int kind, bool smooth, double spot, double diff,
+                                                     double eps_scale) {
+  const double w = smooth ? 0.5 * erfc(-diff * eps_scale) : (diff > 0.0 ? 1.0 : 0.0);
+  return kind == 1 ? w : spot * w;
+}
+
 template <typename R>
 __device__ __forceinline__ double payoff(const Job<R> *__restrict__ J, bool smooth, double spot, double K) {
   const double diff = J->call ? spot - K : K - spot;
   if (J->payoff == 0) return diff > 0.0 ? diff : 0.0;
-  const double w = smooth ? 0.5 * erfc(-diff * J->eps_scale) : (diff > 0.0 ? 1.0 : 0.0);
-  return J->payoff == 1 ? w : spot * w;
+  return digital_payoff(J->payoff, smooth, spot, diff, J->eps_scale);
 }
 
-// base / up / dn payoffs of staged variant v at sample s (calculate_to_buffer)
+// base / up / dn payoffs of staged variant v at sample s for path-dependent
+// instruments (Asian.hpp:168-206, Lookback.hpp:229-268, Barrier.hpp:76-110)
 template <typename R>
-__device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
-                                      int s, double K, bool bumps, double &p, double &pu, double &pd) {
+__device__ static __noinline__ void eval3_pd(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
+                                             int s, double K, bool bumps, double &p, double &pu, double &pd) {
   const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLOCK + s]);
   const double mu = J->mult_up, md = J->mult_dn;
   const bool smooth = J->mode != 0;
   const int trk = J->tracker;
-  if (trk == 0) {
-    p = payoff(J, smooth, S, K);
-    if (bumps) {
-      pu = payoff(J, smooth, S * mu, K);
-      pd = payoff(J, smooth, S * md, K);
-    }
-  } else if (trk != 4) {
+  if (trk != 4) {
     const double A = static_cast<double>(stage[(F_AUX * MAXV + v) * BLOCK + s]);
     if (J->fixed_strike) {
       p = payoff(J, smooth, A, K);
@@ -440,73 +448,163 @@ __device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__r
   }
 }
 
-// Strike loop over one staged batch: lane (j, grp) adds strike j of samples
-// [grp*nkp, grp*nkp + nkp) into its FP64 accumulators (MonteCarloPricer.hpp
-// :264-276 and :312-409 per-path estimators).
+// base / up / dn payoffs of staged variant v at sample s (calculate_to_buffer;
+// European.hpp:40-74 scales the terminal spot by (S0 +- h)/S0)
 template <typename R>
-__device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
-                                           const int *__restrict__ valid, int tid, double *acc) {
-  const int nkp = J->nkp;
-  const int j = tid & (nkp - 1);
-  const int grp = tid / nkp;
-  if (j >= J->nk) return;
-  const double K = J->strikes[j];
-  const int mode = J->mode;
-  const bool ess = mode != 0, full = mode == 2;
-  for (int s = grp * nkp; s < grp * nkp + nkp; ++s) {
-    if (!valid[s]) continue;
-    double p, pu = 0, pd = 0;
-    eval3(J, stage, V_BASE, s, K, ess, p, pu, pd);
-    acc[2 * ST_PRICE] += p;
-    acc[2 * ST_PRICE + 1] += p * p;
-    if (!ess) continue;
-    const double d = (pu - pd) * J->inv2h;
-    const double gm = (pu - p * 2.0 + pd) * J->invh2;
-    acc[2 * ST_DELTA] += d;
-    acc[2 * ST_DELTA + 1] += d * d;
-    acc[2 * ST_GAMMA] += gm;
-    acc[2 * ST_GAMMA + 1] += gm * gm;
-    if (!full) continue;
-    double ap, au = 0, ad = 0, bp, bu = 0, bd = 0;
-    for (int f = 0; f < J->nvega; ++f) {
-      eval3(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
-      eval3(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
-      const double vg = (ap - bp) * J->inv2dv * J->cv[f];
-      const double vn = ((au - ad) - (bu - bd)) * (J->inv2dv * J->inv2h) * J->cv[f];
-      acc[2 * (ST_VEGA0 + f)] += vg;
-      acc[2 * (ST_VEGA0 + f) + 1] += vg * vg;
-      acc[2 * (ST_VANNA0 + f)] += vn;
-      acc[2 * (ST_VANNA0 + f) + 1] += vn * vn;
-    }
-    eval3(J, stage, V_TUP, s, K, true, ap, au, ad);
-    eval3(J, stage, V_TDN, s, K, true, bp, bu, bd);
-    const double th = (ap * J->dfTu - bp * J->dfTd) * J->inv2dT;
-    const double ch = ((au - ad) * J->dfTu - (bu - bd) * J->dfTd) * (J->inv2dT * J->inv2h);
-    acc[2 * ST_THETA] += th;
-    acc[2 * ST_THETA + 1] += th * th;
-    acc[2 * ST_CHARM] += ch;
-    acc[2 * ST_CHARM + 1] += ch * ch;
-    if (J->derive_r) {
-      // European: the rate-bumped paths are the base path shifted by
-      // +-r_bump*T in log space (only C0 of the log step depends on r)
-      const double S = static_cast<double>(stage[(F_SPOT * MAXV + V_BASE) * BLOCK + s]);
-      ap = payoff(J, mode != 0, S * J->erb_up, K);
-      bp = payoff(J, mode != 0, S * J->erb_dn, K);
-    } else {
-      eval3(J, stage, V_RUP, s, K, false, ap, au, ad);
-      eval3(J, stage, V_RDN, s, K, false, bp, bu, bd);
-    }
-    const double rh = (ap * J->dfru - bp * J->dfrd) * J->inv2dr;
-    acc[2 * ST_RHO] += rh;
-    acc[2 * ST_RHO + 1] += rh * rh;
+__device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
+                                      int s, double K, bool bumps, double &p, double &pu, double &pd) {
+  if (J->tracker != 0) {
+    eval3_pd(J, stage, v, s, K, bumps, p, pu, pd);
+    return;
+  }
+  const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLOCK + s]);
+  const bool smooth = J->mode != 0;
+  p = payoff(J, smooth, S, K);
+  if (bumps) {
+    pu = payoff(J, smooth, S * J->mult_up, K);
+    pd = payoff(J, smooth, S * J->mult_dn, K);
   }
 }
 
-// Persistent work-queue kernel. `counter` must be zero at launch.
-template <typename R, int NF, int NV>
+// Strike loop over one staged batch: lane (j, grp) adds strike j of samples
+// [grp*nkp, grp*nkp + nkp) (MonteCarloPricer.hpp:264-276 and :312-409
+// per-path estimators). The statistics are gathered in passes over the
+// staged samples -- price/delta/gamma, then vega+vanna per factor, then
+// theta+charm, then rho -- so at most six FP64 sums are live at once; each
+// pass adds its batch sums into the lane's own slots in global memory
+// (accg[a*BLOCK + tid]), so the path simulation holds no accumulators.
+__device__ __forceinline__ void flush(double *__restrict__ accg, int a, int tid, bool first, double x) {
+  double *slot = accg + a * BLOCK + tid;
+  *slot = first ? x : *slot + x;
+}
+
+template <typename R>
+__device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
+                                           const int *__restrict__ valid, int tid, double *__restrict__ accg,
+                                           bool first) {
+  const int nkp = J->nkp;
+  const int j = tid & (nkp - 1);
+  const int grp = tid / nkp;
+  const int mode = J->mode;
+  const bool ess = mode != 0, full = mode == 2;
+  const bool live = j < J->nk;
+  const double K = live ? J->strikes[j] : 0.0;
+  const int s0 = grp * nkp, s1 = grp * nkp + nkp;
+  {
+    double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg = 0, sg2 = 0;
+    if (live) {
+#pragma unroll 1
+      for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
+        double p, pu = 0, pd = 0;
+        eval3(J, stage, V_BASE, s, K, ess, p, pu, pd);
+        sp += p;
+        sp2 += p * p;
+        if (ess) {
+          const double d = (pu - pd) * J->inv2h;
+          const double gm = (pu - p * 2.0 + pd) * J->invh2;
+          sd += d;
+          sd2 += d * d;
+          sg += gm;
+          sg2 += gm * gm;
+        }
+      }
+    }
+    flush(accg, 2 * ST_PRICE, tid, first, sp);
+    flush(accg, 2 * ST_PRICE + 1, tid, first, sp2);
+    if (ess) {
+      flush(accg, 2 * ST_DELTA, tid, first, sd);
+      flush(accg, 2 * ST_DELTA + 1, tid, first, sd2);
+      flush(accg, 2 * ST_GAMMA, tid, first, sg);
+      flush(accg, 2 * ST_GAMMA + 1, tid, first, sg2);
+    }
+  }
+  if (!full) return;
+#pragma unroll 1
+  for (int f = 0; f < J->nvega; ++f) {
+    double sv = 0, sv2 = 0, sn = 0, sn2 = 0;
+    if (live) {
+#pragma unroll 1
+      for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
+        double ap, au, ad, bp, bu, bd;
+        eval3(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
+        eval3(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
+        const double vg = (ap - bp) * J->inv2dv * J->cv[f];
+        const double vn = ((au - ad) - (bu - bd)) * (J->inv2dv * J->inv2h) * J->cv[f];
+        sv += vg;
+        sv2 += vg * vg;
+        sn += vn;
+        sn2 += vn * vn;
+      }
+    }
+    flush(accg, 2 * (ST_VEGA0 + f), tid, first, sv);
+    flush(accg, 2 * (ST_VEGA0 + f) + 1, tid, first, sv2);
+    flush(accg, 2 * (ST_VANNA0 + f), tid, first, sn);
+    flush(accg, 2 * (ST_VANNA0 + f) + 1, tid, first, sn2);
+  }
+  if (J->nvega < 2) {
+    flush(accg, 2 * ST_VEGA1, tid, first, 0.0);
+    flush(accg, 2 * ST_VEGA1 + 1, tid, first, 0.0);
+    flush(accg, 2 * ST_VANNA1, tid, first, 0.0);
+    flush(accg, 2 * ST_VANNA1 + 1, tid, first, 0.0);
+  }
+  {
+    double st = 0, st2 = 0, sc = 0, sc2 = 0;
+    if (live) {
+#pragma unroll 1
+      for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
+        double ap, au, ad, bp, bu, bd;
+        eval3(J, stage, V_TUP, s, K, true, ap, au, ad);
+        eval3(J, stage, V_TDN, s, K, true, bp, bu, bd);
+        const double th = (ap * J->dfTu - bp * J->dfTd) * J->inv2dT;
+        const double ch = ((au - ad) * J->dfTu - (bu - bd) * J->dfTd) * (J->inv2dT * J->inv2h);
+        st += th;
+        st2 += th * th;
+        sc += ch;
+        sc2 += ch * ch;
+      }
+    }
+    flush(accg, 2 * ST_THETA, tid, first, st);
+    flush(accg, 2 * ST_THETA + 1, tid, first, st2);
+    flush(accg, 2 * ST_CHARM, tid, first, sc);
+    flush(accg, 2 * ST_CHARM + 1, tid, first, sc2);
+  }
+  {
+    double sr = 0, sr2 = 0;
+    if (live) {
+#pragma unroll 1
+      for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
+        double ap, bp, dum0, dum1;
+        if (J->derive_r) {
+          // European: the rate-bumped paths are the base path shifted by
+          // +-r_bump*T in log space (only C0 of the log step depends on r)
+          const double S = static_cast<double>(stage[(F_SPOT * MAXV + V_BASE) * BLOCK + s]);
+          ap = payoff(J, mode != 0, S * J->erb_up, K);
+          bp = payoff(J, mode != 0, S * J->erb_dn, K);
+        } else {
+          eval3(J, stage, V_RUP, s, K, false, ap, dum0, dum1);
+          eval3(J, stage, V_RDN, s, K, false, bp, dum0, dum1);
+        }
+        const double rh = (ap * J->dfru - bp * J->dfrd) * J->inv2dr;
+        sr += rh;
+        sr2 += rh * rh;
+      }
+    }
+    flush(accg, 2 * ST_RHO, tid, first, sr);
+    flush(accg, 2 * ST_RHO + 1, tid, first, sr2);
+  }
+}
+
+// Persistent work-queue kernel. `counter` must be zero at launch; `accg`
+// holds 2*NSTAT*BLOCK doubles per work unit.
+template <typename R, int NF, int NV, bool PD>
 __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restrict__ jobs,
                                                          const Unit *__restrict__ units, int n_units,
-                                                         unsigned *__restrict__ counter) {
+                                                         unsigned *__restrict__ counter,
+                                                         double *__restrict__ accg_all) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_unit;
   __shared__ int s_valid[BLOCK];
@@ -523,6 +621,7 @@ __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restric
     const int job = __builtin_amdgcn_readfirstlane(units[u].job);
     const int blk = __builtin_amdgcn_readfirstlane(units[u].blk);
     const Job<R> *__restrict__ J = jobs + job;
+    double *__restrict__ accg = accg_all + static_cast<size_t>(u) * (2 * NSTAT * BLOCK);
 
     const uint64_t S = J->S, M = J->M;
     const uint64_t s = static_cast<uint64_t>(blk) * BLOCK + tid;
@@ -540,25 +639,25 @@ __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restric
     const uint64_t count = active ? q + (s < rem ? 1 : 0) : 0;
     const uint64_t n_it = q + (static_cast<uint64_t>(blk) * BLOCK < rem ? 1 : 0);
 
-    double acc[2 * NSTAT];
-#pragma unroll
-    for (int a = 0; a < 2 * NSTAT; ++a) acc[a] = 0.0;
-
     for (uint64_t it = 0; it < n_it; ++it) {
       const bool valid = it < count;
       if (valid) {
-        const Rng snap = g;
-        Rng next = g;
-        for (int gi = 0; gi < J->n_groups; ++gi) {
-          Rng gg = snap;
-          sim_group<R, NF, NV>(J, J->group_first[gi], gg, stage, tid);
-          if (gi == 0) next = gg;
+        if constexpr (NV > 1) {
+          // lockstep: one group, the stream advances as the base path does
+          sim_group<R, NF, NV, PD>(J, 0, g, stage, tid);
+        } else {
+          // replay: every variant restarts from the sample's stream state
+          const Rng snap = g;
+          sim_group<R, NF, 1, PD>(J, 0, g, stage, tid);
+          for (int gi = 1; gi < J->n_groups; ++gi) {
+            Rng gg = snap;
+            sim_group<R, NF, 1, PD>(J, J->group_first[gi], gg, stage, tid);
+          }
         }
-        g = next;
       }
       s_valid[tid] = valid ? 1 : 0;
       __syncthreads();
-      accumulate(J, stage, s_valid, tid, acc);
+      accumulate(J, stage, s_valid, tid, accg, it == 0);
       __syncthreads();
     }
 
@@ -566,9 +665,11 @@ __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restric
     // waves in order; one FP64 partial per unit, accumulator and strike
     const int nkp = J->nkp;
     const int lane = tid & 63, wave = tid >> 6;
+    const int mode = J->mode;
+    const int na = mode == 2 ? 2 * NSTAT : (mode == 1 ? 2 * (ST_GAMMA + 1) : 2);
     double *out = J->partial + static_cast<size_t>(J->first_slot + blk) * (2 * NSTAT) * J->nk;
     for (int a = 0; a < 2 * NSTAT; ++a) {
-      double x = acc[a];
+      double x = (a < na && n_it > 0) ? accg[a * BLOCK + tid] : 0.0;
       for (int off = nkp; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
       s_red[tid] = x;
       __syncthreads();

@@ -7,15 +7,16 @@
 
 namespace myql {
 
-// Lockstep widths compiled for each factor count (see host grouping):
-//   NF 0: 1, 5, 7   NF 1: 1, 5, 7   NF 2: 1, 7, 9
-bool mc_supported(int nf, int nv);
+// Lockstep widths compiled for each (factor count, path dependence):
+//   European NF 0/1: 1, 5; NF 2: 1, 7. Path-dependent NF 0/1: 1, 7; NF 2: 1, 9.
+bool mc_supported(int nf, int nv, bool pd);
+
+// pd: path-dependent tracker kernels; accg: 2*NSTAT*BLOCK doubles per unit.
+template <typename R>
+hipError_t launch_mc(int nf, int nv, bool pd, const Job<R> *jobs, const Unit *units, int n_units,
+                     unsigned *counter, double *accg, size_t smem, int grid, hipStream_t stream);
 
 template <typename R>
-hipError_t launch_mc(int nf, int nv, const Job<R> *jobs, const Unit *units, int n_units,
-                     unsigned *counter, size_t smem, int grid, hipStream_t stream);
-
-template <typename R>
-int mc_blocks_per_cu(int nf, int nv, size_t smem);
+int mc_blocks_per_cu(int nf, int nv, bool pd, size_t smem);
 
 }  // namespace myql

@@ -179,7 +179,8 @@ __global__ void __launch_bounds__(BLOCK) mlmc_level_kernel(const Job<R> *__restr
     for (uint64_t i = 0; i < cnt; ++i) {
       double y, p1, d1 = 0, d2 = 0;
       if (!coupled) {
-        sim_group<R, NF, 1>(J, 0, g, stage, tid);
+        if (J->pathdep) sim_group<R, NF, 1, true>(J, 0, g, stage, tid);
+        else sim_group<R, NF, 1, false>(J, 0, g, stage, tid);
         eval3(J, stage, V_BASE, tid, K, false, y, d1, d2);
       } else {
         sim_coupled<R, NF>(J, g, stage, tid);
