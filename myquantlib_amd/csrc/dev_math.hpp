@@ -331,11 +331,11 @@ __device__ __forceinline__ R exponential(R rate, Rng &g) {
 // ---------------------------------------------------------------------------
 __device__ static __noinline__ double igam_series(double a, double x) {
   double sum = 1.0, term = 1.0, ap = a;
-  for (int n = 1; n < 2000; ++n) {
+  for (int n = 1; n < 1000; ++n) {
     ap += 1.0;
     term *= x / ap;
     sum += term;
-    if (term < sum * 1e-17) break;
+    if (term <= sum * 1e-17) break;
   }
   return exp(a * log(x) - x - lgamma(a + 1.0)) * sum;
 }
@@ -343,7 +343,7 @@ __device__ static __noinline__ double igam_series(double a, double x) {
 __device__ static __noinline__ double igamc_cf(double a, double x) {
   const double tiny = 1e-300;
   double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
-  for (int i = 1; i < 2000; ++i) {
+  for (int i = 1; i < 1000; ++i) {
     const double an = -i * (i - a);
     b += 2.0;
     d = an * d + b;
@@ -353,7 +353,7 @@ __device__ static __noinline__ double igamc_cf(double a, double x) {
     d = 1.0 / d;
     const double del = d * c;
     h *= del;
-    if (fabs(del - 1.0) < 1e-16) break;
+    if (fabs(del - 1.0) <= 4e-16) break;  // within 2 ulps of 1
   }
   return exp(a * log(x) - x - lgamma(a)) * h;
 }
@@ -384,7 +384,7 @@ __device__ static __noinline__ double chi2_quantile(double dof, double p) {
   }
   if (!(x > 0.0)) x = 1e-300;
   double lo = 0.0, hi = __builtin_inf();
-  for (int it = 0; it < 200; ++it) {
+  for (int it = 0; it < 64; ++it) {
     double f;
     if (upper) f = q - ((x < a + 1.0) ? 1.0 - igam_series(a, x) : igamc_cf(a, x));
     else f = ((x < a + 1.0) ? igam_series(a, x) : 1.0 - igamc_cf(a, x)) - p;
@@ -405,7 +405,8 @@ __device__ static __noinline__ double chi2_quantile(double dof, double p) {
     }
     const double dx = fabs(xn - x);
     x = xn;
-    if (dx <= 2e-16 * x) break;
+    // stop once the step or the bracket is within a few ulps
+    if (dx <= 4e-16 * x || (hi - lo) <= 8e-16 * x) break;
   }
   return 2.0 * x;
 }

@@ -264,6 +264,7 @@ __global__ void debug_chi2_kernel(const double *__restrict__ dof, const double *
 struct myql_ctx {
   int device = 0;
   int n_cu = 256;
+  size_t lds_max = 64 * 1024;  // dynamic LDS available to one block
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -778,7 +779,7 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     for (size_t i = 0; i < jobs.size(); ++i)
       if (built[i].nf == nf && built[i].nv == nv && built[i].pd == pdk) nfields = std::max(nfields, jobs[i].n_fields);
     const size_t smem = static_cast<size_t>(nfields) * MAXV * BLOCK * sizeof(R);
-    if (smem > 64 * 1024) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds 64 KiB of LDS");
+    if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
     const int bpc = mc_blocks_per_cu<R>(nf, nv, pdk, smem);
     const int n_units = static_cast<int>(ranges[gi].second);
     const int grid = std::max(1, std::min(n_units, bpc * c->n_cu));
@@ -896,7 +897,7 @@ struct MlmcRun {
     HIPCHECK(hipMemcpyAsync(d_k, in.strikes, sizeof(double), hipMemcpyHostToDevice, c->stream));
     const unsigned nb = static_cast<unsigned>((S + BLOCK - 1) / BLOCK);
     const size_t smem = static_cast<size_t>(J.n_fields) * MAXV * BLOCK * sizeof(R);
-    if (smem > 64 * 1024) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds 64 KiB of LDS");
+    if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
     const Job<R> *dj = c->mlmc_job.as<Job<R>>();
     uint64_t *st = c->mlmc_streams.as<uint64_t>();
@@ -1086,6 +1087,8 @@ myql_ctx *myql_create(int device) {
     hipDeviceProp_t prop;
     HIPCHECK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
+    // static LDS of the kernels (queue slot, valid flags, reduction) is < 4 KiB
+    c->lds_max = prop.sharedMemPerBlock > 4096 ? prop.sharedMemPerBlock - 4096 : prop.sharedMemPerBlock;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));
     HIPCHECK(hipEventCreate(&c->ev1));

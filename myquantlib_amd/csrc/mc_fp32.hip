@@ -13,6 +13,9 @@ hipError_t launch_mc<float>(int nf, int nv, bool pd, const Job<float> *jobs, con
                            unsigned *counter, double *accg, size_t smem, int grid, hipStream_t stream) {
 #define MYQL_LAUNCH(NF_, NV_, PD_)                                                                \
   if (nf == NF_ && nv == NV_ && pd == PD_) {                                                       \
+    if (smem > 64 * 1024)                                                                          \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(mc_paths_kernel<float, NF_, NV_, PD_>), \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem)); \
     hipLaunchKernelGGL((mc_paths_kernel<float, NF_, NV_, PD_>), dim3(grid), dim3(BLOCK), smem, stream, \
                        jobs, units, n_units, counter, accg);                                      \
     return hipGetLastError();                                                                     \

@@ -238,11 +238,11 @@ static inline double lgam(double a) { return std::lgamma(a); }
 // P by its power series: x^a e^-x / Gamma(a+1) * sum x^n / ((a+1)...(a+n))
 static inline double gamma_p_series(double a, double x) {
   double sum = 1.0, term = 1.0, ap = a;
-  for (int n = 1; n < 2000; ++n) {
+  for (int n = 1; n < 1000; ++n) {
     ap += 1.0;
     term *= x / ap;
     sum += term;
-    if (term < sum * 1e-17) break;
+    if (term <= sum * 1e-17) break;
   }
   return std::exp(a * std::log(x) - x - lgam(a + 1.0)) * sum;
 }
@@ -252,7 +252,7 @@ static inline double gamma_q_cf(double a, double x) {
   const double tiny = 1e-300;
   double b = x + 1.0 - a;
   double c = 1.0 / tiny, d = 1.0 / b, h = d;
-  for (int i = 1; i < 2000; ++i) {
+  for (int i = 1; i < 1000; ++i) {
     const double an = -i * (i - a);
     b += 2.0;
     d = an * d + b;
@@ -262,7 +262,7 @@ static inline double gamma_q_cf(double a, double x) {
     d = 1.0 / d;
     const double del = d * c;
     h *= del;
-    if (std::fabs(del - 1.0) < 1e-16) break;
+    if (std::fabs(del - 1.0) <= 4e-16) break;  // within 2 ulps of 1
   }
   return std::exp(a * std::log(x) - x - lgam(a)) * h;
 }
@@ -305,7 +305,7 @@ static inline double gamma_p_inv(double a, double p) {
   }
   if (!(x > 0.0)) x = 1e-300;
   double lo = 0.0, hi = std::numeric_limits<double>::infinity();
-  for (int it = 0; it < 200; ++it) {
+  for (int it = 0; it < 64; ++it) {
     // residual f (increasing in x) and pdf
     double f;
     if (upper) f = q - gamma_q(a, x);  // = P - p computed through Q
@@ -332,7 +332,8 @@ static inline double gamma_p_inv(double a, double p) {
     }
     const double dx = std::fabs(xn - x);
     x = xn;
-    if (dx <= 2e-16 * x) break;
+    // stop once the step or the bracket is within a few ulps
+    if (dx <= 4e-16 * x || (hi - lo) <= 8e-16 * x) break;
   }
   return x;
 }
