@@ -329,7 +329,8 @@ __device__ __forceinline__ R exponential(R rate, Rng &g) {
 // 368-369). Power series below a+1, Legendre continued fraction (Lentz)
 // above; Halley steps inside a shrinking bracket.
 // ---------------------------------------------------------------------------
-__device__ static __noinline__ double igam_series(double a, double x) {
+// lga1 = lgamma(a + 1)
+__device__ static __noinline__ double igam_series(double a, double x, double lga1) {
   double sum = 1.0, term = 1.0, ap = a;
   for (int n = 1; n < 1000; ++n) {
     ap += 1.0;
@@ -337,10 +338,11 @@ __device__ static __noinline__ double igam_series(double a, double x) {
     sum += term;
     if (term <= sum * 1e-16) break;
   }
-  return exp(a * log(x) - x - lgamma(a + 1.0)) * sum;
+  return exp(a * log(x) - x - lga1) * sum;
 }
 
-__device__ static __noinline__ double igamc_cf(double a, double x) {
+// lga = lgamma(a)
+__device__ static __noinline__ double igamc_cf(double a, double x, double lga) {
   const double tiny = 1e-300;
   double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
   for (int i = 1; i < 1000; ++i) {
@@ -355,16 +357,19 @@ __device__ static __noinline__ double igamc_cf(double a, double x) {
     h *= del;
     if (fabs(del - 1.0) <= 1e-15) break;  // a few ulps of 1, above FMA noise
   }
-  return exp(a * log(x) - x - lgamma(a)) * h;
+  return exp(a * log(x) - x - lga) * h;
 }
 
-__device__ static __noinline__ double chi2_quantile(double dof, double p) {
+// Chi-squared quantile given lga = lgamma(dof / 2). The device lgamma costs
+// thousands of cycles, so callers pass it (the path kernels read it from a
+// per-table list of lgamma(d/2 + N)).
+__device__ static __noinline__ double chi2_quantile_lg(double dof, double p, double lga) {
   if (p <= 0.0) return 0.0;
   if (p >= 1.0) return __builtin_inf();
   const double a = 0.5 * dof;
+  const double lga1 = lga + log(a);  // lgamma(a + 1)
   const bool upper = p > 0.5;
   const double q = 1.0 - p;
-  const double lga = lgamma(a);
   double x;
   if (a > 1.0) {
     const double z = normcdfinv_d(p);
@@ -372,7 +377,7 @@ __device__ static __noinline__ double chi2_quantile(double dof, double p) {
     const double w = 1.0 - t + z * sqrt(t);
     x = (w > 0.0) ? a * w * w * w : 0.5 * a;
   } else {
-    const double xl = exp((log(p) + lgamma(a + 1.0)) / a);
+    const double xl = exp((log(p) + lga1) / a);
     if (!upper && xl < 1.0) {
       x = xl;
     } else {
@@ -386,8 +391,8 @@ __device__ static __noinline__ double chi2_quantile(double dof, double p) {
   double lo = 0.0, hi = __builtin_inf();
   for (int it = 0; it < 24; ++it) {
     double f;
-    if (upper) f = q - ((x < a + 1.0) ? 1.0 - igam_series(a, x) : igamc_cf(a, x));
-    else f = ((x < a + 1.0) ? igam_series(a, x) : 1.0 - igamc_cf(a, x)) - p;
+    if (upper) f = q - ((x < a + 1.0) ? 1.0 - igam_series(a, x, lga1) : igamc_cf(a, x, lga));
+    else f = ((x < a + 1.0) ? igam_series(a, x, lga1) : 1.0 - igamc_cf(a, x, lga)) - p;
     if (f > 0.0) hi = x; else lo = x;
     if (f == 0.0) break;
     const double pdf = exp((a - 1.0) * log(x) - x - lga);
@@ -411,5 +416,7 @@ __device__ static __noinline__ double chi2_quantile(double dof, double p) {
   }
   return 2.0 * x;
 }
+
+__device__ static double chi2_quantile(double dof, double p) { return chi2_quantile_lg(dof, p, lgamma(0.5 * dof)); }
 
 }  // namespace myql

@@ -182,11 +182,14 @@ __global__ void stream_states_kernel(uint64_t seed, int layout, uint64_t S, cons
   out[4 * s + 3] = g.s3;
 }
 
-__global__ void nci_quantiles_kernel(double d, int rows, int grid, double dx, double *__restrict__ y) {
+// Row k holds chi2 quantiles for dof d + 2k at u = i*dx (VolSchemes.hpp:267-290);
+// lg[k] = lgamma(d/2 + k).
+__global__ void nci_quantiles_kernel(double d, int rows, int grid, double dx, const double *__restrict__ lg,
+                                     double *__restrict__ y) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= static_cast<int64_t>(rows) * grid) return;
   const int k = static_cast<int>(e / grid), i = static_cast<int>(e % grid);
-  y[e] = chi2_quantile(d + 2.0 * k, static_cast<double>(i) * dx);
+  y[e] = chi2_quantile_lg(d + 2.0 * k, static_cast<double>(i) * dx, lg[k]);
 }
 
 // Fritsch-Butland slopes (VolSchemes.hpp:292-310) and the interleaved
@@ -300,11 +303,13 @@ const uint64_t *stream_states(myql_ctx *c, uint64_t seed, int layout, uint64_t S
 
 struct NciInfo {
   const void *tab;
+  const double *lg;  // lgamma(d/2 + N), N < NCI_LG
   int rows, grid;
   double dx, inv_dx;
 };
 
 constexpr int NCI_GRID = 10000;
+constexpr int NCI_LG = 2048;
 
 // NCI row count for one workspace (VolSchemes.hpp:245-255)
 int nci_nmax(const myql_heston_params &p, double dt) {
@@ -326,17 +331,26 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec) {
   const double dx = 0.999 / static_cast<double>(NCI_GRID - 1);
   const double inv_dx = 1.0 / dx;
   // a cached table with at least as many rows serves as well
+  const size_t esz = prec == MYQL_FP32 ? 4 : 8;
+  auto lg_of = [&](const DevBuf &b, int r) {
+    return reinterpret_cast<const double *>(static_cast<const char *>(b.p) + static_cast<size_t>(r) * NCI_GRID * 2 * esz);
+  };
   for (auto &kv : c->nci)
     if (std::get<0>(kv.first) == dbits && std::get<2>(kv.first) == prec && std::get<1>(kv.first) >= rows)
-      return {kv.second->p, std::get<1>(kv.first), NCI_GRID, dx, inv_dx};
+      return {kv.second->p, lg_of(*kv.second, std::get<1>(kv.first)), std::get<1>(kv.first), NCI_GRID, dx, inv_dx};
   if (c->nci.size() > 32) c->nci.clear();
   const int64_t n = static_cast<int64_t>(rows) * NCI_GRID;
   c->tmp.reserve(n * sizeof(double));
   auto buf = std::make_unique<DevBuf>();
-  buf->reserve(n * 2 * (prec == MYQL_FP32 ? 4 : 8));
+  buf->reserve(n * 2 * esz + NCI_LG * sizeof(double));
+  // lgamma(d/2 + N) for the quantile's incomplete-gamma prefactors
+  std::vector<double> lgh(NCI_LG);
+  for (int k = 0; k < NCI_LG; ++k) lgh[k] = std::lgamma(0.5 * d + k);
+  double *d_lg = const_cast<double *>(lg_of(*buf, rows));
+  HIPCHECK(hipMemcpyAsync(d_lg, lgh.data(), NCI_LG * sizeof(double), hipMemcpyHostToDevice, c->stream));
   const int tb = 256;
   const unsigned nb = static_cast<unsigned>((n + tb - 1) / tb);
-  hipLaunchKernelGGL(nci_quantiles_kernel, dim3(nb), dim3(tb), 0, c->stream, d, rows, NCI_GRID, dx,
+  hipLaunchKernelGGL(nci_quantiles_kernel, dim3(nb), dim3(tb), 0, c->stream, d, rows, NCI_GRID, dx, d_lg,
                      c->tmp.as<double>());
   HIPCHECK(hipGetLastError());
   if (prec == MYQL_FP32)
@@ -349,7 +363,7 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec) {
   HIPCHECK(hipStreamSynchronize(c->stream));  // the scratch rows are reused by the next table
   const void *p = buf->p;
   c->nci[std::make_tuple(dbits, rows, prec)] = std::move(buf);
-  return {p, rows, NCI_GRID, dx, inv_dx};
+  return {p, d_lg, rows, NCI_GRID, dx, inv_dx};
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +640,8 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
       const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
       NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision);
       J.nci[f] = ni.tab;
+      J.nci_lgam[f] = ni.lg;
+      J.nci_lg_n = NCI_LG;
       J.nci_grid = ni.grid;
       J.nci_dx = ni.dx;
       J.nci_inv_dx = ni.inv_dx;
@@ -874,6 +890,8 @@ struct MlmcRun {
         nmax[f] = nci_nmax(p, dtf);
         NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nmax[f], cfg.precision);
         J.nci[f] = ni.tab;
+        J.nci_lgam[f] = ni.lg;
+        J.nci_lg_n = NCI_LG;
         J.nci_grid = ni.grid;
         J.nci_dx = ni.dx;
         J.nci_inv_dx = ni.inv_dx;

@@ -63,6 +63,8 @@ struct Job {
   uint64_t M, S, steps;
   const uint64_t *streams;         // S x 4 stream states
   const void *nci[2];              // interleaved {y, dx*m} rows, R precision
+  const double *nci_lgam[2];       // lgamma(d/2 + N) for N < nci_lg_n
+  int nci_lg_n;
   int nci_grid;
   double nci_dx, nci_inv_dx;
   const double *strikes;

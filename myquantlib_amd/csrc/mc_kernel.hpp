@@ -298,7 +298,9 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
       } else {
-        vn[k][f] = static_cast<R>(chi2_quantile(P[k]->Kd[f][1] + 2.0 * N, u2) * P[k]->Kd[f][2]);
+        const double dof = P[k]->Kd[f][1] + 2.0 * N;
+        const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
+        vn[k][f] = static_cast<R>(chi2_quantile_lg(dof, u2, lg) * P[k]->Kd[f][2]);
       }
     }
   } else {  // Exact (Poisson-gamma mixture); variable draws, so never lockstep

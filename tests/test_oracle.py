@@ -131,7 +131,8 @@ def test_chi2_quantile_golden(oracle):
     g = GOLDEN["chi2"]
     got = np.array([oracle.chi2_quantile(d, p) for d, p in zip(g["dof"], g["p"])])
     ref = np.array(g["q"])
-    assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) < 1e-12
+    # quantiles below ~1e-290 underflow in the reference (scipy returns 0)
+    assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-290)) < 1e-12
     assert oracle.chi2_quantile(0.32, 0.0) == 0.0
 
 
