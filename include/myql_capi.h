@@ -86,6 +86,15 @@ enum { MYQL_FP64 = 0, MYQL_FP32 = 1 };
  * In both layouts stream s prices the contiguous block of paths that an
  * OpenMP static schedule over S threads would give thread s. */
 enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
+/* Config flags.
+ *  REFERENCE_NCI_ROWS  size the SchemeNCI quantile table exactly as the
+ *      reference does (n_max from v_max = max(v0, 3 theta), VolSchemes.hpp
+ *      :245-255) and invert the chi-squared law exactly above it. Without the
+ *      flag the table gets rows up to n_ext = ceil(l + 6 sqrt(l)), l the
+ *      Poisson intensity at v = max(v0, 25 theta) (at most 1023 rows), so the
+ *      exact inversion -- thousands of cycles, run by the whole wavefront --
+ *      is left for u >= 0.999 and essentially never for the row count. */
+enum { MYQL_FLAG_REFERENCE_NCI_ROWS = 1 };
 
 /* ---- model (ModelParams.hpp:7-47, ASVJmodel.hpp:14-87) ----------------- */
 typedef struct {
@@ -129,6 +138,7 @@ typedef struct {
   int32_t precision;     /* MYQL_FP64 / MYQL_FP32 */
   int32_t stream_layout; /* MYQL_STREAMS_* */
   uint64_t num_streams;  /* S; 0 = library default (65536, at most num_paths) */
+  uint32_t flags;        /* MYQL_FLAG_* */
 } myql_mc_config;
 
 /* ---- MonteCarloResult<Mode, T> (PricingTypes.hpp:46-97) ----------------
@@ -169,6 +179,7 @@ typedef struct {
   int32_t precision;     /* path arithmetic; level sums are FP64 */
   int32_t stream_layout;
   uint64_t num_streams;
+  uint32_t flags;        /* MYQL_FLAG_* */
 } myql_mlmc_config;
 
 /* MlmcResult<None, double> (PricingTypes.hpp:102-115). The three arrays are

@@ -24,6 +24,7 @@ BARRIER_IN, BARRIER_OUT = range(2)
 REBATE_NONE, REBATE_AT_HIT, REBATE_AT_MATURITY = range(3)
 FP64, FP32 = range(2)
 STREAMS_JUMP, STREAMS_SEEDED = range(2)
+FLAG_REFERENCE_NCI_ROWS = 1
 
 
 class HestonParams(C.Structure):
@@ -51,7 +52,7 @@ class MCConfig(C.Structure):
     _fields_ = [("num_paths", C.c_uint64), ("time_steps", C.c_uint64), ("seed", C.c_uint64),
                 ("fd_bump", C.c_double), ("vol_bump", C.c_double), ("t_bump", C.c_double),
                 ("r_bump", C.c_double), ("precision", C.c_int32), ("stream_layout", C.c_int32),
-                ("num_streams", C.c_uint64)]
+                ("num_streams", C.c_uint64), ("flags", C.c_uint32)]
 
 
 _PD = C.POINTER(C.c_double)
@@ -78,7 +79,7 @@ class MLMCConfig(C.Structure):
                 ("N0", C.c_uint64), ("base_steps", C.c_uint64), ("alpha", C.c_double),
                 ("beta", C.c_double), ("adaptive_rates", C.c_int32), ("seed", C.c_uint64),
                 ("fd_bump", C.c_double), ("precision", C.c_int32),
-                ("stream_layout", C.c_int32), ("num_streams", C.c_uint64)]
+                ("stream_layout", C.c_int32), ("num_streams", C.c_uint64), ("flags", C.c_uint32)]
 
 
 class MLMCResult(C.Structure):

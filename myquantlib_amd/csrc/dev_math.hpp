@@ -389,7 +389,7 @@ __device__ static __noinline__ double chi2_quantile_lg(double dof, double p, dou
   }
   if (!(x > 0.0)) x = 1e-300;
   double lo = 0.0, hi = __builtin_inf();
-  for (int it = 0; it < 24; ++it) {
+  for (int it = 0; it < 64; ++it) {
     double f;
     if (upper) f = q - ((x < a + 1.0) ? 1.0 - igam_series(a, x, lga1) : igamc_cf(a, x, lga));
     else f = ((x < a + 1.0) ? igam_series(a, x, lga1) : 1.0 - igamc_cf(a, x, lga)) - p;

@@ -311,13 +311,17 @@ struct NciInfo {
 constexpr int NCI_GRID = 10000;
 constexpr int NCI_LG = 2048;
 
-// NCI row count for one workspace (VolSchemes.hpp:245-255)
-int nci_nmax(const myql_heston_params &p, double dt) {
-  const double v_max = std::max(p.v0, 3.0 * p.theta);
+// NCI rows for one workspace. Reference rule (VolSchemes.hpp:245-255):
+// intensity at v_max = max(v0, 3 theta), n_max = ceil(l + 4 sqrt(l)). The
+// default sizes the table at v = max(v0, 25 theta) with a 6 sqrt(l) margin so
+// that the exact chi2 inversion above the last row is practically never run;
+// both are clamped to [23, 1023].
+int nci_nmax(const myql_heston_params &p, double dt, bool reference_rows) {
+  const double v_max = std::max(p.v0, (reference_rows ? 3.0 : 25.0) * p.theta);
   const double ex = std::exp(-p.kappa * dt);
   const double psi = (std::fabs(p.kappa) < 1e-10) ? dt : (1.0 - ex) / p.kappa;
   const double lam_max = 2.0 * v_max * ex / (p.sigma * p.sigma * psi);
-  double nm = std::ceil(lam_max + 4.0 * std::sqrt(lam_max));
+  double nm = std::ceil(lam_max + (reference_rows ? 4.0 : 6.0) * std::sqrt(lam_max));
   size_t n = static_cast<size_t>(nm);
   n = std::max<size_t>(n, 23);
   n = std::min<size_t>(n, 1023);
@@ -631,7 +635,7 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
   for (size_t i = 0; i < vd.size(); ++i)
     for (int f = 0; f < nf; ++f)
       if (js.schemes[f] == MYQL_SCHEME_NCI) {
-        nmaxv[i][f] = nci_nmax(vd[i].m.heston[f], vd[i].dt);
+        nmaxv[i][f] = nci_nmax(vd[i].m.heston[f], vd[i].dt, cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS);
         nmax_max[f] = std::max(nmax_max[f], nmaxv[i][f]);
       }
   for (int f = 0; f < nf; ++f)
@@ -887,7 +891,7 @@ struct MlmcRun {
     for (int f = 0; f < nf; ++f)
       if (schemes[f] == MYQL_SCHEME_NCI) {
         const myql_heston_params &p = m.heston[f];
-        nmax[f] = nci_nmax(p, dtf);
+        nmax[f] = nci_nmax(p, dtf, cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS);
         NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nmax[f], cfg.precision);
         J.nci[f] = ni.tab;
         J.nci_lgam[f] = ni.lg;

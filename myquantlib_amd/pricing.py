@@ -101,18 +101,18 @@ class InstrumentSpec:
 def mc_config(num_paths: int, time_steps: int, seed: int = 42, fd_bump: float = 1e-4,
               vol_bump: float = 1e-4, t_bump: float = 1e-4, r_bump: float = 1e-4,
               precision: int = capi.FP64, stream_layout: int = capi.STREAMS_JUMP,
-              num_streams: int = 0) -> MCConfig:
+              num_streams: int = 0, flags: int = 0) -> MCConfig:
     return MCConfig(num_paths, time_steps, seed, fd_bump, vol_bump, t_bump, r_bump, precision,
-                    stream_layout, num_streams)
+                    stream_layout, num_streams, flags)
 
 
 def mlmc_config(epsilon: float = 0.005, L_min: int = 2, L_max: int = 15, N0: int = 10000,
                 base_steps: int = 1, alpha: float = 2.0, beta: float = 1.0,
                 adaptive_rates: bool = True, seed: int = 42, fd_bump: float = 1e-4,
                 precision: int = capi.FP64, stream_layout: int = capi.STREAMS_JUMP,
-                num_streams: int = 0) -> MLMCConfig:
+                num_streams: int = 0, flags: int = 0) -> MLMCConfig:
     return MLMCConfig(epsilon, L_min, L_max, N0, base_steps, alpha, beta, int(adaptive_rates),
-                      seed, fd_bump, precision, stream_layout, num_streams)
+                      seed, fd_bump, precision, stream_layout, num_streams, flags)
 
 
 class ResultBuffers:

@@ -51,22 +51,17 @@ inline void partition(uint64_t n, uint64_t S, uint64_t s, uint64_t &begin, uint6
 
 struct TableCache {
   std::map<std::pair<uint64_t, size_t>, std::shared_ptr<NciTable>> m;
+  bool reference_rows = false;
   std::shared_ptr<NciTable> get(const myql_heston_params &p, double dt) {
-    // the table depends on (d, n_max) only; build the probe's n_max first
-    auto t = std::make_shared<NciTable>();
+    // the table depends on (d, n_max) only
     const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
-    const double v_max = std::max(p.v0, 3.0 * p.theta);
-    const double ex = std::exp(-p.kappa * dt);
-    const double psi = (std::fabs(p.kappa) < 1e-10) ? dt : (1.0 - ex) / p.kappa;
-    const double lam_max = 2.0 * v_max * ex / (p.sigma * p.sigma * psi);
-    size_t n_max = static_cast<size_t>(std::ceil(lam_max + 4.0 * std::sqrt(lam_max)));
-    n_max = std::min<size_t>(std::max<size_t>(n_max, 23), 1023);
+    const size_t n_max = nci_rows_nmax(p, dt, reference_rows);
     uint64_t dbits;
     std::memcpy(&dbits, &d, 8);
     auto key = std::make_pair(dbits, n_max);
     auto it = m.find(key);
     if (it != m.end()) return it->second;
-    auto tab = build_nci_table(p, dt);
+    auto tab = build_nci_table(p, dt, reference_rows);
     m[key] = tab;
     return tab;
   }
@@ -100,6 +95,13 @@ enum { ST_PRICE, ST_DELTA, ST_GAMMA, ST_VEGA0, ST_VEGA1, ST_THETA, ST_RHO, ST_VA
 extern "C" {
 
 const char *oracle_last_error() { return g_err.c_str(); }
+
+// number of NCI steps that used the exact chi2 quantile instead of the table
+uint64_t oracle_nci_fallbacks(int reset) {
+  const uint64_t n = nci_fallbacks().load();
+  if (reset) nci_fallbacks().store(0);
+  return n;
+}
 
 void oracle_seed_state(uint64_t seed, uint64_t *out) {
   Xoshiro g = seed_state(seed);
@@ -174,7 +176,7 @@ int oracle_cir_terminal(int32_t scheme, const myql_heston_params *p, double dt, 
                         uint64_t seed, double *out, int nthreads) {
   try {
     std::shared_ptr<NciTable> tab;
-    if (scheme == MYQL_SCHEME_NCI) tab = build_nci_table(*p, dt);
+    if (scheme == MYQL_SCHEME_NCI) tab = build_nci_table(*p, dt, false);
     Factor f;
     prepare_factor(f, scheme, *p, 0.0, dt, tab);
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -215,6 +217,7 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
     const TrackerCfg tc = tracker_config(I, mode, S0, h);
 
     TableCache cache;
+    cache.reference_rows = (cfg->flags & MYQL_FLAG_REFERENCE_NCI_ROWS) != 0;
     auto mk = [&](const myql_model &m, double dtv, double rv, double Tv) {
       Tables tb = tables_for(cache, m, schemes, dtv);
       return make_stepper(m, schemes, dtv, rv, q, Tv, pd, tb.t[0], tb.t[1]);
@@ -372,6 +375,7 @@ int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const
     for (uint64_t s = 0; s < S; ++s) pool[s] = stream_state(cfg->seed, cfg->stream_layout, s);
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
     TableCache cache;
+    cache.reference_rows = (cfg->flags & MYQL_FLAG_REFERENCE_NCI_ROWS) != 0;
 
     auto evaluate_level = [&](size_t l, uint64_t dN, double &sum, double &sq) {
       std::vector<double> ps(S, 0.0), pq(S, 0.0);

@@ -14,6 +14,7 @@
 //   Payoffs      Payoffs.hpp:14-69
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
 #include <stdexcept>
@@ -41,14 +42,25 @@ struct NciTable {
   std::vector<double> y, m;
 };
 
-static inline std::shared_ptr<NciTable> build_nci_table(const myql_heston_params &p, double dt) {
-  auto gw = std::make_shared<NciTable>();
-  gw->d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
-  const double v_max = std::max(p.v0, 3.0 * p.theta);
+// Row count: the reference's rule (VolSchemes.hpp:245-255, v_max = max(v0,
+// 3 theta), margin 4 sqrt(l)) or the engine's default extended rule (v =
+// max(v0, 25 theta), margin 6 sqrt(l)); both clamped to [23, 1023].
+static inline size_t nci_rows_nmax(const myql_heston_params &p, double dt, bool reference_rows) {
+  const double v_max = std::max(p.v0, (reference_rows ? 3.0 : 25.0) * p.theta);
   const double ex = std::exp(-p.kappa * dt);
   const double psi = (std::fabs(p.kappa) < 1e-10) ? dt : (1.0 - ex) / p.kappa;
   const double lam_max = 2.0 * v_max * ex / (p.sigma * p.sigma * psi);
-  double nm = std::ceil(lam_max + 4.0 * std::sqrt(lam_max));
+  size_t n_max = static_cast<size_t>(std::ceil(lam_max + (reference_rows ? 4.0 : 6.0) * std::sqrt(lam_max)));
+  if (n_max < 23) n_max = 23;
+  if (n_max > 1023) n_max = 1023;
+  return n_max;
+}
+
+static inline std::shared_ptr<NciTable> build_nci_table(const myql_heston_params &p, double dt,
+                                                        bool reference_rows) {
+  auto gw = std::make_shared<NciTable>();
+  gw->d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
+  double nm = static_cast<double>(nci_rows_nmax(p, dt, reference_rows));
   size_t n_max = static_cast<size_t>(nm);
   if (n_max < 23) n_max = 23;
   if (n_max > 1023) n_max = 1023;
@@ -177,6 +189,12 @@ static inline double exact_gamma(const Factor &f, int N, Xoshiro &g) {
   }
 }
 
+// diagnostics: how often the exact quantile replaces the table
+inline std::atomic<uint64_t> &nci_fallbacks() {
+  static std::atomic<uint64_t> n{0};
+  return n;
+}
+
 static inline double nci_lookup(const Factor &f, int N, double U) {
   const NciTable &t = *f.nci;
   if (static_cast<size_t>(N) <= t.n_max && U < 0.999) {
@@ -191,6 +209,7 @@ static inline double nci_lookup(const Factor &f, int N, double U) {
             (t3 - t2) * t.dx * m1) *
            f.K[2];
   }
+  nci_fallbacks().fetch_add(1, std::memory_order_relaxed);
   return chi2_quantile(f.K[1] + 2.0 * N, U) * f.K[2];
 }
 

@@ -305,7 +305,7 @@ static inline double gamma_p_inv(double a, double p) {
   }
   if (!(x > 0.0)) x = 1e-300;
   double lo = 0.0, hi = std::numeric_limits<double>::infinity();
-  for (int it = 0; it < 24; ++it) {
+  for (int it = 0; it < 64; ++it) {
     // residual f (increasing in x) and pdf
     double f;
     if (upper) f = q - gamma_q(a, x);  // = P - p computed through Q

@@ -117,14 +117,16 @@ def test_heston_nv_essential_fp64(engine, oracle):
     assert g["total_steps"] == (1 << 16) * 50
 
 
-@pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-9), (capi.FP32, 2e-3)])
-def test_bates_nci_full_strip(engine, oracle, prec, tol):
+@pytest.mark.parametrize("prec,tol,flags", [(capi.FP64, 1e-9, 0), (capi.FP32, 2e-3, 0),
+                                            (capi.FP64, 1e-9, capi.FLAG_REFERENCE_NCI_ROWS)])
+def test_bates_nci_full_strip(engine, oracle, prec, tol, flags):
     """Configuration 2 shape (Bates, NCI with Feller violated, Merton, Full
-    CRN Greeks on a strike strip) at reduced size."""
+    CRN Greeks on a strike strip) at reduced size; with the reference's NCI
+    row rule the exact chi2 inversion runs on ~4% of steps."""
     jk, jp = merton(0.5, -0.1, 0.15)
     m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
     inst = InstrumentSpec(strikes=np.linspace(70, 130, 16), maturity=0.5)
-    cfg = mc_config(1 << 14, 50, seed=42, precision=prec, num_streams=1024)
+    cfg = mc_config(1 << 14, 50, seed=42, precision=prec, num_streams=1024, flags=flags)
     g, c = _run_both(engine, oracle, m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg)
     if prec == capi.FP64:
         compare(g, c, FULL, tol)
