@@ -360,19 +360,19 @@ __device__ static __noinline__ double igamc_cf(double a, double x, double lga) {
   return exp(a * log(x) - x - lga) * h;
 }
 
-// Chi-squared quantile given lga = lgamma(dof / 2). The device lgamma costs
-// thousands of cycles, so callers pass it (the path kernels read it from a
-// per-table list of lgamma(d/2 + N)).
-__device__ static __noinline__ double chi2_quantile_lg(double dof, double p, double lga) {
+// x with P(a, x) = p, given lga = lgamma(a) and q = 1 - p. The upper
+// branch (p > 1/2) is solved on Q(a, x) = q with q passed separately, so a
+// tail probability far below 2^-53 keeps its precision. The device lgamma
+// costs thousands of cycles, so callers pass it (the path kernels read it
+// from a per-table list of lgamma(d/2 + N)).
+__device__ static __noinline__ double gamma_inv(double a, double p, double q, double lga) {
   if (p <= 0.0) return 0.0;
-  if (p >= 1.0) return __builtin_inf();
-  const double a = 0.5 * dof;
+  if (q <= 0.0) return __builtin_inf();
   const double lga1 = lga + log(a);  // lgamma(a + 1)
   const bool upper = p > 0.5;
-  const double q = 1.0 - p;
   double x;
   if (a > 1.0) {
-    const double z = normcdfinv_d(p);
+    const double z = upper ? -normcdfinv_d(q) : normcdfinv_d(p);
     const double t = 1.0 / (9.0 * a);
     const double w = 1.0 - t + z * sqrt(t);
     x = (w > 0.0) ? a * w * w * w : 0.5 * a;
@@ -414,7 +414,17 @@ __device__ static __noinline__ double chi2_quantile_lg(double dof, double p, dou
     // below one ulp; the bracket test ends oscillation at the rounding floor
     if (dx <= 1e-14 * x || (hi - lo) <= 1e-14 * x) break;
   }
-  return 2.0 * x;
+  return x;
+}
+
+// chi-squared quantile at probability p, lga = lgamma(dof / 2)
+__device__ static double chi2_quantile_lg(double dof, double p, double lga) {
+  return 2.0 * gamma_inv(0.5 * dof, p, 1.0 - p, lga);
+}
+
+// chi-squared quantile at upper-tail probability qt (P(X > x) = qt)
+__device__ static double chi2_quantile_upper_lg(double dof, double qt, double lga) {
+  return 2.0 * gamma_inv(0.5 * dof, 1.0 - qt, qt, lga);
 }
 
 __device__ static double chi2_quantile(double dof, double p) { return chi2_quantile_lg(dof, p, lgamma(0.5 * dof)); }

@@ -192,17 +192,22 @@ __global__ void nci_quantiles_kernel(double d, int rows, int grid, double dx, co
   y[e] = chi2_quantile_lg(d + 2.0 * k, static_cast<double>(i) * dx, lg[k]);
 }
 
-// Fritsch-Butland slopes (VolSchemes.hpp:292-310) and the interleaved
-// {y, dx*m} layout the path kernels gather from.
+// Fritsch-Butland slopes (VolSchemes.hpp:292-310) for rows k <= n_fb, exact
+// slopes dq/du = 1/pdf(q) above, in the interleaved {y, dx*m} layout the path
+// kernels gather from.
 template <typename R>
 __global__ void nci_pack_kernel(const double *__restrict__ y, int rows, int grid, double dx, double inv_dx,
-                                R *__restrict__ tab) {
+                                double d, int n_fb, const double *__restrict__ lg, R *__restrict__ tab) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= static_cast<int64_t>(rows) * grid) return;
   const int i = static_cast<int>(e % grid);
+  const int k = static_cast<int>(e / grid);
   const double *r = y + (e - i);
   double m;
-  if (i == 0) m = (r[1] - r[0]) * inv_dx;
+  if (k > n_fb && i > 0) {
+    const double dof = d + 2.0 * k;
+    m = exp(-((0.5 * dof - 1.0) * log(r[i]) - 0.5 * r[i] - 0.5 * dof * 0.69314718055994531 - lg[k]));
+  } else if (i == 0) m = (r[1] - r[0]) * inv_dx;
   else if (i == grid - 1) m = (r[i] - r[i - 1]) * inv_dx;
   else {
     const double d1 = (r[i] - r[i - 1]) * inv_dx, d2 = (r[i + 1] - r[i]) * inv_dx;
@@ -228,6 +233,23 @@ __global__ void reduce_partials_kernel(const double *__restrict__ partial, doubl
   sums[out_off[job] + e] = t;
 }
 
+
+// Upper-tail rows: quantiles at tail probability e^-w, w = w0 + j hw, and
+// hw * dq/dw = hw e^-w / pdf(q).
+template <typename R>
+__global__ void nci_tail_kernel(double d, int rows, int pts, double w0, double hw, const double *__restrict__ lg,
+                                R *__restrict__ tail) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<int64_t>(rows) * pts) return;
+  const int k = static_cast<int>(e / pts), j = static_cast<int>(e % pts);
+  const double dof = d + 2.0 * k;
+  const double w = w0 + j * hw;
+  const double qt = exp(-w);
+  const double x = chi2_quantile_upper_lg(dof, qt, lg[k]);
+  const double lpdf = (0.5 * dof - 1.0) * log(x) - 0.5 * x - 0.5 * dof * 0.69314718055994531 - lg[k];
+  tail[2 * e] = static_cast<R>(x);
+  tail[2 * e + 1] = static_cast<R>(hw * qt * exp(-lpdf));
+}
 
 __global__ void debug_draws_kernel(const uint64_t *__restrict__ states, uint64_t S, uint64_t n,
                                    uint64_t *__restrict__ out) {
@@ -274,7 +296,7 @@ struct myql_ctx {
   std::vector<uint64_t> jump_polys;
   DevBuf d_polys;
   std::map<std::tuple<uint64_t, int, uint64_t>, std::unique_ptr<DevBuf>> streams;
-  std::map<std::tuple<uint64_t, int, int>, std::unique_ptr<DevBuf>> nci;  // (d bits, rows, prec)
+  std::map<std::tuple<uint64_t, int, int, int>, std::unique_ptr<DevBuf>> nci;  // (d bits, rows, prec, n_fb)
   DevBuf jobs, units, partial, sums, meta, counter, tmp, accg;
   DevBuf mlmc_streams, mlmc_sums, mlmc_job;
 };
@@ -303,6 +325,7 @@ const uint64_t *stream_states(myql_ctx *c, uint64_t seed, int layout, uint64_t S
 
 struct NciInfo {
   const void *tab;
+  const void *tail;  // upper-tail rows
   const double *lg;  // lgamma(d/2 + N), N < NCI_LG
   int rows, grid;
   double dx, inv_dx;
@@ -310,6 +333,9 @@ struct NciInfo {
 
 constexpr int NCI_GRID = 10000;
 constexpr int NCI_LG = 2048;
+constexpr int NCI_TAIL = 512;
+constexpr double NCI_TAIL_W1 = 38.0;
+const double NCI_TAIL_W0 = std::log(1000.0);
 
 // NCI rows for one workspace. Reference rule (VolSchemes.hpp:245-255):
 // intensity at v_max = max(v0, 3 theta), n_max = ceil(l + 4 sqrt(l)). The
@@ -328,46 +354,75 @@ int nci_nmax(const myql_heston_params &p, double dt, bool reference_rows) {
   return static_cast<int>(n);
 }
 
-NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec) {
+// One buffer per (d, rows, precision, n_fb): main rows [rows x grid x 2],
+// tail rows [rows x NCI_TAIL x 2] (R precision), then NCI_LG doubles of
+// lgamma(d/2 + N).
+NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   const int rows = nmax + 1;
   uint64_t dbits;
   std::memcpy(&dbits, &d, 8);
   const double dx = 0.999 / static_cast<double>(NCI_GRID - 1);
   const double inv_dx = 1.0 / dx;
-  // a cached table with at least as many rows serves as well
   const size_t esz = prec == MYQL_FP32 ? 4 : 8;
-  auto lg_of = [&](const DevBuf &b, int r) {
-    return reinterpret_cast<const double *>(static_cast<const char *>(b.p) + static_cast<size_t>(r) * NCI_GRID * 2 * esz);
+  const size_t main_b = static_cast<size_t>(rows) * NCI_GRID * 2 * esz;
+  const size_t tail_b = static_cast<size_t>(rows) * NCI_TAIL * 2 * esz;
+  auto info = [&](const DevBuf &b) {
+    const char *p = static_cast<const char *>(b.p);
+    return NciInfo{p, p + main_b, reinterpret_cast<const double *>(p + main_b + tail_b), rows, NCI_GRID, dx, inv_dx};
   };
-  for (auto &kv : c->nci)
-    if (std::get<0>(kv.first) == dbits && std::get<2>(kv.first) == prec && std::get<1>(kv.first) >= rows)
-      return {kv.second->p, lg_of(*kv.second, std::get<1>(kv.first)), std::get<1>(kv.first), NCI_GRID, dx, inv_dx};
+  const auto key = std::make_tuple(dbits, rows, prec, n_fb);
+  auto it = c->nci.find(key);
+  if (it != c->nci.end()) return info(*it->second);
   if (c->nci.size() > 32) c->nci.clear();
   const int64_t n = static_cast<int64_t>(rows) * NCI_GRID;
   c->tmp.reserve(n * sizeof(double));
   auto buf = std::make_unique<DevBuf>();
-  buf->reserve(n * 2 * esz + NCI_LG * sizeof(double));
+  buf->reserve(main_b + tail_b + NCI_LG * sizeof(double));
+  NciInfo ni = info(*buf);
   // lgamma(d/2 + N) for the quantile's incomplete-gamma prefactors
   std::vector<double> lgh(NCI_LG);
   for (int k = 0; k < NCI_LG; ++k) lgh[k] = std::lgamma(0.5 * d + k);
-  double *d_lg = const_cast<double *>(lg_of(*buf, rows));
+  double *d_lg = const_cast<double *>(ni.lg);
   HIPCHECK(hipMemcpyAsync(d_lg, lgh.data(), NCI_LG * sizeof(double), hipMemcpyHostToDevice, c->stream));
   const int tb = 256;
   const unsigned nb = static_cast<unsigned>((n + tb - 1) / tb);
   hipLaunchKernelGGL(nci_quantiles_kernel, dim3(nb), dim3(tb), 0, c->stream, d, rows, NCI_GRID, dx, d_lg,
                      c->tmp.as<double>());
   HIPCHECK(hipGetLastError());
-  if (prec == MYQL_FP32)
+  const int64_t nt = static_cast<int64_t>(rows) * NCI_TAIL;
+  const unsigned nbt = static_cast<unsigned>((nt + tb - 1) / tb);
+  const double hw = (NCI_TAIL_W1 - NCI_TAIL_W0) / (NCI_TAIL - 1);
+  if (prec == MYQL_FP32) {
     hipLaunchKernelGGL(nci_pack_kernel<float>, dim3(nb), dim3(tb), 0, c->stream, c->tmp.as<double>(), rows,
-                       NCI_GRID, dx, inv_dx, buf->as<float>());
-  else
+                       NCI_GRID, dx, inv_dx, d, n_fb, d_lg, static_cast<float *>(const_cast<void *>(ni.tab)));
+    hipLaunchKernelGGL(nci_tail_kernel<float>, dim3(nbt), dim3(tb), 0, c->stream, d, rows, NCI_TAIL, NCI_TAIL_W0,
+                       hw, d_lg, static_cast<float *>(const_cast<void *>(ni.tail)));
+  } else {
     hipLaunchKernelGGL(nci_pack_kernel<double>, dim3(nb), dim3(tb), 0, c->stream, c->tmp.as<double>(), rows,
-                       NCI_GRID, dx, inv_dx, buf->as<double>());
+                       NCI_GRID, dx, inv_dx, d, n_fb, d_lg, static_cast<double *>(const_cast<void *>(ni.tab)));
+    hipLaunchKernelGGL(nci_tail_kernel<double>, dim3(nbt), dim3(tb), 0, c->stream, d, rows, NCI_TAIL, NCI_TAIL_W0,
+                       hw, d_lg, static_cast<double *>(const_cast<void *>(ni.tail)));
+  }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));  // the scratch rows are reused by the next table
-  const void *p = buf->p;
-  c->nci[std::make_tuple(dbits, rows, prec)] = std::move(buf);
-  return {p, d_lg, rows, NCI_GRID, dx, inv_dx};
+  c->nci[key] = std::move(buf);
+  return ni;
+}
+
+// Table fields of a job factor; the tail rows are used unless the config
+// asks for the reference's table semantics.
+template <typename R>
+void set_nci(Job<R> &J, int f, const NciInfo &ni, bool reference) {
+  J.nci[f] = ni.tab;
+  J.nci_lgam[f] = ni.lg;
+  J.nci_lg_n = NCI_LG;
+  J.nci_grid = ni.grid;
+  J.nci_dx = ni.dx;
+  J.nci_inv_dx = ni.inv_dx;
+  J.nci_tail[f] = reference ? nullptr : ni.tail;
+  J.tail_pts = NCI_TAIL;
+  J.tail_w0 = NCI_TAIL_W0;
+  J.tail_inv_hw = (NCI_TAIL - 1) / (NCI_TAIL_W1 - NCI_TAIL_W0);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,13 +697,9 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
     if (js.schemes[f] == MYQL_SCHEME_NCI) {
       const myql_heston_params &p = M0.heston[f];
       const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
-      NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision);
-      J.nci[f] = ni.tab;
-      J.nci_lgam[f] = ni.lg;
-      J.nci_lg_n = NCI_LG;
-      J.nci_grid = ni.grid;
-      J.nci_dx = ni.dx;
-      J.nci_inv_dx = ni.inv_dx;
+      const bool ref = cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS;
+      NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision, nci_nmax(p, dt, true));
+      set_nci(J, f, ni, ref);
     }
   for (size_t i = 0; i < vd.size(); ++i)
     fill_variant(J.var[vd[i].id], vd[i].m, js.schemes, vd[i].dt, vd[i].r, q, vd[i].T, steps, pd,
@@ -800,9 +851,12 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
       if (built[i].nf == nf && built[i].nv == nv && built[i].pd == pdk) nfields = std::max(nfields, jobs[i].n_fields);
     const size_t smem = static_cast<size_t>(nfields) * MAXV * BLOCK * sizeof(R);
     if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
-    const int bpc = mc_blocks_per_cu<R>(nf, nv, pdk, smem);
+    // Resident blocks per CU: the occupancy query, but never fewer than the
+    // VGPR budget allows (512 per SIMD lane, 4 waves per block, 4 SIMDs);
+    // surplus blocks of the persistent grid find the queue empty and exit.
+    const int bpc = std::max(mc_blocks_per_cu<R>(nf, nv, pdk, smem), 2);
     const int n_units = static_cast<int>(ranges[gi].second);
-    const int grid = std::max(1, std::min(n_units, bpc * c->n_cu));
+    const int grid = std::max(1, std::min(n_units, 4 * bpc * c->n_cu));
     HIPCHECK(launch_mc<R>(nf, nv, pdk, d_jobs, c->units.as<Unit>() + ranges[gi].first, n_units,
                           c->counter.as<unsigned>() + 16 * gi,
                           c->accg.as<double>() + ranges[gi].first * 2 * NSTAT * BLOCK, smem, grid, c->stream));
@@ -892,13 +946,10 @@ struct MlmcRun {
       if (schemes[f] == MYQL_SCHEME_NCI) {
         const myql_heston_params &p = m.heston[f];
         nmax[f] = nci_nmax(p, dtf, cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS);
-        NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nmax[f], cfg.precision);
-        J.nci[f] = ni.tab;
-        J.nci_lgam[f] = ni.lg;
-        J.nci_lg_n = NCI_LG;
-        J.nci_grid = ni.grid;
-        J.nci_dx = ni.dx;
-        J.nci_inv_dx = ni.inv_dx;
+        const bool ref = cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS;
+        NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nmax[f], cfg.precision,
+                               nci_nmax(p, dtf, true));
+        set_nci(J, f, ni, ref);
       }
     fill_variant(J.var[0], m, schemes, dtf, r, q, T, coupled ? fs : steps, pd, nmax);
     if (coupled) fill_variant(J.var[1], m, schemes, 2.0 * dtf, r, q, T, steps, pd, nmax);

@@ -65,6 +65,9 @@ struct Job {
   const void *nci[2];              // interleaved {y, dx*m} rows, R precision
   const double *nci_lgam[2];       // lgamma(d/2 + N) for N < nci_lg_n
   int nci_lg_n;
+  const void *nci_tail[2];         // upper-tail rows {q, hw*dq/dw} on w = -log(1-u); null = exact
+  double tail_w0, tail_inv_hw;
+  int tail_pts;
   int nci_grid;
   double nci_dx, nci_inv_dx;
   const double *strikes;

@@ -297,6 +297,17 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       if (N <= P[k]->nmax[f] && u2 < 0.999) {
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
+      } else if (N <= P[k]->nmax[f] && J->nci_tail[f] != nullptr) {
+        // u >= 0.999: cubic Hermite in w = -log(1 - u) with exact slopes
+        double tp = (-log1p(-u2) - J->tail_w0) * J->tail_inv_hw;
+        if (tp < 0.0) tp = 0.0;
+        int j = static_cast<int>(tp);
+        if (j >= J->tail_pts - 1) j = J->tail_pts - 2;
+        const R tt = static_cast<R>(tp - static_cast<double>(j)), tt2 = tt * tt, tt3 = tt2 * tt;
+        const R *row = static_cast<const R *>(J->nci_tail[f]) + 2 * (static_cast<size_t>(N) * J->tail_pts + j);
+        vn[k][f] = ((R(2) * tt3 - R(3) * tt2 + R(1)) * row[0] + (tt3 - R(2) * tt2 + tt) * row[1] +
+                    (R(-2) * tt3 + R(3) * tt2) * row[2] + (tt3 - tt2) * row[3]) *
+                   P[k]->Kr[f][2];
       } else {
         const double dof = P[k]->Kd[f][1] + 2.0 * N;
         const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);

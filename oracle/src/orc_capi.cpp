@@ -52,16 +52,25 @@ inline void partition(uint64_t n, uint64_t S, uint64_t s, uint64_t &begin, uint6
 struct TableCache {
   std::map<std::pair<uint64_t, size_t>, std::shared_ptr<NciTable>> m;
   bool reference_rows = false;
+  // reference row count of the base workspace per d (where the slopes switch
+  // from Fritsch-Butland to exact); defaults to the probe's own
+  std::map<uint64_t, long> n_fb;
+  static uint64_t bits(double d) {
+    uint64_t b;
+    std::memcpy(&b, &d, 8);
+    return b;
+  }
   std::shared_ptr<NciTable> get(const myql_heston_params &p, double dt) {
     // the table depends on (d, n_max) only
     const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
     const size_t n_max = nci_rows_nmax(p, dt, reference_rows);
-    uint64_t dbits;
-    std::memcpy(&dbits, &d, 8);
+    const uint64_t dbits = bits(d);
+    auto fb = n_fb.find(dbits);
+    const long nfb = fb != n_fb.end() ? fb->second : static_cast<long>(nci_rows_nmax(p, dt, true));
     auto key = std::make_pair(dbits, n_max);
     auto it = m.find(key);
     if (it != m.end()) return it->second;
-    auto tab = build_nci_table(p, dt, reference_rows);
+    auto tab = build_nci_table(p, dt, reference_rows, nfb);
     m[key] = tab;
     return tab;
   }
@@ -218,6 +227,12 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
 
     TableCache cache;
     cache.reference_rows = (cfg->flags & MYQL_FLAG_REFERENCE_NCI_ROWS) != 0;
+    for (int i = 0; i < M0.n_factors; ++i)
+      if (schemes[i] == MYQL_SCHEME_NCI) {
+        const myql_heston_params &h = M0.heston[i];
+        cache.n_fb[TableCache::bits(4.0 * h.kappa * h.theta / (h.sigma * h.sigma))] =
+            static_cast<long>(nci_rows_nmax(h, dt, true));
+      }
     auto mk = [&](const myql_model &m, double dtv, double rv, double Tv) {
       Tables tb = tables_for(cache, m, schemes, dtv);
       return make_stepper(m, schemes, dtv, rv, q, Tv, pd, tb.t[0], tb.t[1]);

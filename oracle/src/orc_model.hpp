@@ -40,6 +40,13 @@ struct NciTable {
   size_t n_max = 0, grid = 10000;
   double dx = 0.0, inv_dx = 0.0;
   std::vector<double> y, m;
+  // Upper-tail rows (u >= 0.999) on w = -log(1 - u): quantiles at
+  // w_j = w0 + j hw and exact slopes dq/dw = (1 - u) / pdf(q). Empty under
+  // the reference's table semantics (exact inversion there).
+  static constexpr size_t kTail = 512;
+  static constexpr double kTailW1 = 38.0;
+  std::vector<double> ty, tm;
+  double tw0 = 0.0, thw = 0.0, inv_thw = 0.0;
 };
 
 // Row count: the reference's rule (VolSchemes.hpp:245-255, v_max = max(v0,
@@ -56,8 +63,11 @@ static inline size_t nci_rows_nmax(const myql_heston_params &p, double dt, bool 
   return n_max;
 }
 
+// Rows up to n_fb get the reference's Fritsch-Butland slopes; rows above it
+// (present only outside the reference's table semantics) get the exact
+// slopes dq/du = 1/pdf(q).
 static inline std::shared_ptr<NciTable> build_nci_table(const myql_heston_params &p, double dt,
-                                                        bool reference_rows) {
+                                                        bool reference_rows, long n_fb = -1) {
   auto gw = std::make_shared<NciTable>();
   gw->d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
   double nm = static_cast<double>(nci_rows_nmax(p, dt, reference_rows));
@@ -77,8 +87,13 @@ static inline std::shared_ptr<NciTable> build_nci_table(const myql_heston_params
     double *y = &gw->y[k * G];
     double *m = &gw->m[k * G];
     for (size_t i = 0; i < G; ++i) y[i] = chi2_quantile(dof, static_cast<double>(i) * gw->dx);
+    const bool exact_slopes = !reference_rows && n_fb >= 0 && k > n_fb;
+    const double lgh = std::lgamma(0.5 * dof);
     for (size_t i = 0; i < G; ++i) {
-      if (i == 0) {
+      if (exact_slopes && i > 0) {
+        const double lpdf = (0.5 * dof - 1.0) * std::log(y[i]) - 0.5 * y[i] - 0.5 * dof * std::log(2.0) - lgh;
+        m[i] = 1.0 / std::exp(lpdf);
+      } else if (i == 0) {
         m[i] = (y[1] - y[0]) * gw->inv_dx;
       } else if (i == G - 1) {
         m[i] = (y[i] - y[i - 1]) * gw->inv_dx;
@@ -86,6 +101,27 @@ static inline std::shared_ptr<NciTable> build_nci_table(const myql_heston_params
         const double d1 = (y[i] - y[i - 1]) * gw->inv_dx;
         const double d2 = (y[i + 1] - y[i]) * gw->inv_dx;
         m[i] = (d1 * d2 > 0.0) ? 2.0 * d1 * d2 / (d1 + d2) : 0.0;
+      }
+    }
+  }
+  if (!reference_rows) {
+    const size_t P = NciTable::kTail;
+    gw->tw0 = std::log(1000.0);
+    gw->thw = (NciTable::kTailW1 - gw->tw0) / static_cast<double>(P - 1);
+    gw->inv_thw = 1.0 / gw->thw;
+    gw->ty.assign((n_max + 1) * P, 0.0);
+    gw->tm.assign((n_max + 1) * P, 0.0);
+#pragma omp parallel for schedule(dynamic)
+    for (long k = 0; k <= static_cast<long>(n_max); ++k) {
+      const double dof = gw->d + 2.0 * static_cast<double>(k);
+      const double lgh = std::lgamma(0.5 * dof);
+      for (size_t j = 0; j < P; ++j) {
+        const double w = gw->tw0 + static_cast<double>(j) * gw->thw;
+        const double qt = std::exp(-w);
+        const double x = chi2_quantile_upper(dof, qt);
+        const double lpdf = (0.5 * dof - 1.0) * std::log(x) - 0.5 * x - 0.5 * dof * std::log(2.0) - lgh;
+        gw->ty[k * P + j] = x;
+        gw->tm[k * P + j] = qt / std::exp(lpdf);
       }
     }
   }
@@ -207,6 +243,20 @@ static inline double nci_lookup(const Factor &f, int N, double U) {
     const double t2 = tt * tt, t3 = t2 * tt;
     return ((2 * t3 - 3 * t2 + 1) * y0 + (t3 - 2 * t2 + tt) * t.dx * m0 + (-2 * t3 + 3 * t2) * y1 +
             (t3 - t2) * t.dx * m1) *
+           f.K[2];
+  }
+  if (static_cast<size_t>(N) <= t.n_max && !t.ty.empty()) {
+    // upper tail: cubic Hermite in w = -log(1 - U) with exact slopes
+    const size_t P = NciTable::kTail;
+    double pos = (-std::log1p(-U) - t.tw0) * t.inv_thw;
+    if (pos < 0.0) pos = 0.0;
+    size_t j = static_cast<size_t>(pos);
+    if (j >= P - 1) j = P - 2;
+    const double tt = pos - static_cast<double>(j);
+    const size_t off = static_cast<size_t>(N) * P + j;
+    const double t2 = tt * tt, t3 = t2 * tt;
+    return ((2 * t3 - 3 * t2 + 1) * t.ty[off] + (t3 - 2 * t2 + tt) * t.thw * t.tm[off] +
+            (-2 * t3 + 3 * t2) * t.ty[off + 1] + (t3 - t2) * t.thw * t.tm[off + 1]) *
            f.K[2];
   }
   nci_fallbacks().fetch_add(1, std::memory_order_relaxed);

@@ -276,18 +276,17 @@ static inline double gamma_q(double a, double x) {
   return (x < a + 1.0) ? 1.0 - gamma_p_series(a, x) : gamma_q_cf(a, x);
 }
 
-// x with P(a, x) = p. The lower tail is solved on P, the upper tail
-// (p > 0.5) on Q = 1 - p so that no precision is lost to cancellation.
-static inline double gamma_p_inv(double a, double p) {
+// x with P(a, x) = p, q = 1 - p passed separately. The lower tail is solved
+// on P, the upper tail (p > 0.5) on Q = q so that a tail probability far
+// below 2^-53 keeps its precision.
+static inline double gamma_inv(double a, double p, double q) {
   if (p <= 0.0) return 0.0;
-  if (p >= 1.0) return std::numeric_limits<double>::infinity();
+  if (q <= 0.0) return std::numeric_limits<double>::infinity();
   const bool upper = p > 0.5;
-  const double q = 1.0 - p;
   const double lg_a = lgam(a);
-  // starting point
   double x;
-  const double z = normcdfinv(p);
   if (a > 1.0) {
+    const double z = upper ? -normcdfinv(q) : normcdfinv(p);
     const double t = 1.0 / (9.0 * a);
     const double w = 1.0 - t + z * std::sqrt(t);
     x = (w > 0.0) ? a * w * w * w : 0.5 * a;
@@ -338,6 +337,11 @@ static inline double gamma_p_inv(double a, double p) {
   }
   return x;
 }
+
+static inline double gamma_p_inv(double a, double p) { return gamma_inv(a, p, 1.0 - p); }
+
+// chi-squared quantile at upper-tail probability qt
+static inline double chi2_quantile_upper(double dof, double qt) { return 2.0 * gamma_inv(0.5 * dof, 1.0 - qt, qt); }
 
 static inline double chi2_quantile(double dof, double p) {
   return 2.0 * gamma_p_inv(0.5 * dof, p);
