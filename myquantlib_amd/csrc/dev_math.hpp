@@ -303,6 +303,24 @@ __device__ static __noinline__ double poissinv(double lam, double U) {
   return X;
 }
 
+// Is n the Poisson(lam) quantile of U, i.e. F(n-1) < U <= F(n)? Checked as
+// S(n-1) < U e^lam <= S(n), S(k) = sum_{j<=k} lam^j / j!, which costs n FMAs
+// and one exp. Used for lockstep Greek variants whose intensities differ
+// from the base path's by a bump: they almost always share the base's
+// count, and a failed check falls back to the full inversion. Only called
+// with n <= 40 and lam < 700.
+__device__ __forceinline__ bool poisson_quantile_is(double lam, double U, int n) {
+  const double E = U * exp(lam);
+  double term = 1.0, S_prev = 0.0, S = 1.0;
+  for (int j = 1; j <= n; ++j) {
+    term *= lam / j;
+    S_prev = S;
+    S += term;
+  }
+  if (n == 0) S_prev = 0.0;
+  return S_prev < E && E <= S;
+}
+
 // Poisson(mean) by sequential inversion on one uniform (Boost's method for
 // means below 10; JumpPolicies.hpp:60,116).
 __device__ __forceinline__ int poisson_count(double mean, double exp_mean, double u) {

@@ -290,10 +290,17 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     const R h00 = R(2) * t3 - R(3) * t2 + R(1), h10 = t3 - R(2) * t2 + t;
     const R h01 = R(-2) * t3 + R(3) * t2, h11 = t3 - t2;
     const R *__restrict__ tab = static_cast<const R *>(J->nci[f]);
+    // the base path (slot 0) inverts the Poisson CDF; bumped variants
+    // verify that they share its count and only invert on a mismatch
+    const double lam0 = P[0]->Kd[f][0] * static_cast<double>(v[0][f]);
+    const int N0 = static_cast<int>(poissinv(lam0, u1));
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const double lam = P[k]->Kd[f][0] * static_cast<double>(v[k][f]);
-      const int N = static_cast<int>(poissinv(lam, u1));
+      int N = N0;
+      if (k > 0 && lam != lam0) {
+        if (!(N0 <= 40 && lam < 700.0 && poisson_quantile_is(lam, u1, N0))) N = static_cast<int>(poissinv(lam, u1));
+      }
       if (N <= P[k]->nmax[f] && u2 < 0.999) {
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
