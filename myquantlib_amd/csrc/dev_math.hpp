@@ -168,8 +168,19 @@ __constant__ static const double kTemmeB0[13] = {
 // Giles' inverse Poisson CDF (reference Numerics.hpp:49-272; ACM TOMS Alg.
 // 955) in FP64. Decides the integer N that fixes the CIR mixture of the NCI
 // and Exact schemes, so it runs in double on both path precisions.
+//
+// poissinv_w takes W = Phi^-1(U) computed as +-Phi^-1(min(U, 1 - U)) by
+// giles_w (only read when lam > 4): the Greek variants of one sample share
+// their Poisson uniform, so they share W -- the costliest part of the
+// method -- and each runs only its own asymptotic and summation steps.
 // ---------------------------------------------------------------------------
-__device__ static __noinline__ double poissinv(double lam, double U) {
+__device__ __forceinline__ double giles_w(double U) {
+  const double V = 1.0 - U;
+  const double W = normcdfinv_d(fmin(U, V));
+  return U > V ? -W : W;
+}
+
+__device__ __forceinline__ double poissinv_w(double lam, double U, double W) {
   if (U <= 0.0) return 0.0;
   if (U >= 1.0) return __builtin_nan("");
   const double V = 1.0 - U;
@@ -177,8 +188,6 @@ __device__ static __noinline__ double poissinv(double lam, double U) {
   double X = 0.0, Del = 0.0;
   if (lam > 4.0) {
     double S;
-    double W = normcdfinv_d(fmin(U, V));
-    if (U > V) W = -W;
     if (fabs(W) < 3.0) {
       const double sl = sqrt(lam);
       S = sl * W + (1.0 / 3.0 + (1.0 / 6.0) * W * W) * (1.0 - W / (12.0 * sl));
@@ -303,22 +312,8 @@ __device__ static __noinline__ double poissinv(double lam, double U) {
   return X;
 }
 
-// Is n the Poisson(lam) quantile of U, i.e. F(n-1) < U <= F(n)? Checked as
-// S(n-1) < U e^lam <= S(n), S(k) = sum_{j<=k} lam^j / j!, which costs n FMAs
-// and one exp. Used for lockstep Greek variants whose intensities differ
-// from the base path's by a bump: they almost always share the base's
-// count, and a failed check falls back to the full inversion. Only called
-// with n <= 40 and lam < 700.
-__device__ __forceinline__ bool poisson_quantile_is(double lam, double U, int n) {
-  const double E = U * exp(lam);
-  double term = 1.0, S_prev = 0.0, S = 1.0;
-  for (int j = 1; j <= n; ++j) {
-    term *= lam / j;
-    S_prev = S;
-    S += term;
-  }
-  if (n == 0) S_prev = 0.0;
-  return S_prev < E && E <= S;
+__device__ static __noinline__ double poissinv(double lam, double U) {
+  return poissinv_w(lam, U, (lam > 4.0 && U > 0.0 && U < 1.0) ? giles_w(U) : 0.0);
 }
 
 // Poisson(mean) by sequential inversion on one uniform (Boost's method for

@@ -16,7 +16,7 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/myql_capi.h"
+#include "myql_capi.h"
 #include "dev_math.hpp"
 #include "engine_types.hpp"
 #include "mc_launch.hpp"

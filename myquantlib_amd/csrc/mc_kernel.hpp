@@ -290,17 +290,36 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     const R h00 = R(2) * t3 - R(3) * t2 + R(1), h10 = t3 - R(2) * t2 + t;
     const R h01 = R(-2) * t3 + R(3) * t2, h11 = t3 - t2;
     const R *__restrict__ tab = static_cast<const R *>(J->nci[f]);
-    // the base path (slot 0) inverts the Poisson CDF; bumped variants
-    // verify that they share its count and only invert on a mismatch
-    const double lam0 = P[0]->Kd[f][0] * static_cast<double>(v[0][f]);
-    const int N0 = static_cast<int>(poissinv(lam0, u1));
+    // Poisson counts of all slots from the shared uniform u1: Giles' method
+    // per slot, with the normal quantile W of u1 computed once. The slot
+    // loop stays rolled (one inlined copy of the method); the slot's
+    // intensity and count move through select chains, not memory.
+    int Ns[NV];
+    if constexpr (NV == 1) {
+      Ns[0] = static_cast<int>(poissinv(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1));
+    } else {
+      double lams[NV];
+      bool big = false;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        lams[k] = P[k]->Kd[f][0] * static_cast<double>(v[k][f]);
+        big = big || lams[k] > 4.0;
+        Ns[k] = 0;
+      }
+      const double W = (big && u1 > 0.0) ? giles_w(u1) : 0.0;
+#pragma unroll 1
+      for (int k = 0; k < NV; ++k) {
+        double lam = lams[0];
+#pragma unroll
+        for (int j = 1; j < NV; ++j) lam = (k == j) ? lams[j] : lam;
+        const int n = static_cast<int>(poissinv_w(lam, u1, W));
+#pragma unroll
+        for (int j = 0; j < NV; ++j) Ns[j] = (k == j) ? n : Ns[j];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      const double lam = P[k]->Kd[f][0] * static_cast<double>(v[k][f]);
-      int N = N0;
-      if (k > 0 && lam != lam0) {
-        if (!(N0 <= 40 && lam < 700.0 && poisson_quantile_is(lam, u1, N0))) N = static_cast<int>(poissinv(lam, u1));
-      }
+      const int N = Ns[k];
       if (N <= P[k]->nmax[f] && u2 < 0.999) {
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
