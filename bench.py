@@ -36,10 +36,24 @@ MATURITIES = [0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0]
 STRIKES = np.linspace(70.0, 130.0, 64)
 S0, R, Q = 100.0, 0.02, 0.0
 VARIANTS = 7
-# Algorithmic FP operations per path-step of this workload (FMA = 2): see
-# DESIGN.md, "Roofline of the path kernel".
-FLOP_PER_PATH_STEP = 90.0
+# Algorithmic floating-point operations per path-step of this workload: the
+# reference algorithm as written (each of the 7 variants replays its path; an
+# FMA counts 2; add, mul, div, sqrt, exp, log, erfc 1 each; stepping, Poisson
+# and chi-squared inversion, jumps, payoffs, Greek estimators and sums; NCI
+# table set-up excluded), counted by the oracle's op-counting build:
+#   python oracle/count_flops.py --paths-log2 16   -> 103.67
+FLOP_PER_PATH_STEP = 103.67
+# MI355X published FP32 vector peak (packed dual-issue FP32; scalar FP32 and
+# FP64 vector issue peak at 78.6 TFLOP/s)
 FP32_PEAK_TFLOPS = 157.3
+# Path variants the engine simulates per sample for this European strip: the
+# r +- variants are the base path shifted by +-r_bump*T in log space (only the
+# drift depends on r), so 5 of the 7 are stepped
+EXECUTED_VARIANTS = 5
+# HBM bytes per path-kernel launch (FETCH_SIZE + WRITE_SIZE from rocprofv3,
+# scaled by the same counters on a plain copy of known size), by
+# --paths-log2 of the measured run; None where not measured
+TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 1.575e11}
 
 
 def model():
@@ -94,15 +108,35 @@ def cpu_baseline(target_s: float = 12.0):
         o.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg, S0, R, Q, nthreads=cores)
         return time.perf_counter() - t0
 
-    run(cores * 16)  # NCI table build and warm-up
-    pilot = cores * 256
-    t = run(pilot)
-    paths = int(max(pilot, min(1 << 22, pilot * target_s / max(t, 1e-3))))
+    run(cores * 16)  # warm-up
+    # two pilots separate the per-call set-up (the NCI table build, which the
+    # reference also repeats in every calculate call) from the per-path cost
+    p1, p2 = cores * 256, cores * 2048
+    t1, t2 = run(p1), run(p2)
+    per_path = max((t2 - t1) / (p2 - p1), 1e-9)
+    fixed = max(t1 - p1 * per_path, 0.0)
+    paths = int(max(p2, min(1 << 23, (target_s - fixed) / per_path)))
     t = run(paths)
     rate = paths * 100 * VARIANTS / t
     return {"value": rate, "unit": "path-steps/s", "cores": cores, "kind": "port",
             "sample": f"T=1 maturity of the strip: {paths} paths x 100 steps x 7 CRN variants, "
                       f"64 strikes, Full Greeks, FP64, {t:.1f} s on {platform.processor() or 'host'}"}
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain device-to-device copy of known size: each launch reads and
+    writes nbytes, so FETCH_SIZE and WRITE_SIZE of these dispatches give the
+    counters' scale (expected nbytes / 1024 KiB each)."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for i in range(reps):
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy {i}: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr,
+              flush=True)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def main():
@@ -113,7 +147,13 @@ def main():
     ap.add_argument("--paths-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described "
                                    "in DESIGN.md but not launched"}))
@@ -176,8 +216,11 @@ def main():
             "path_steps_per_step": total,
         },
         "ms_to_1e-4_rel_se": {"plain_fp32": t32, "plain_fp64": ms64, "learned_cv_fp32": None},
-        "roofline": {"bound": "valu_fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+        "sample_steps_per_s": value / VARIANTS,
+        "executed_variant_fraction": EXECUTED_VARIANTS / VARIANTS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(args.paths_log2),
                      "kernel_ms": float(np.mean(dev_ms)),
                      "flop_per_path_step": FLOP_PER_PATH_STEP},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),

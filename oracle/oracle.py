@@ -21,6 +21,7 @@ from myquantlib_amd.pricing import (ResultBuffers, MLMCBuffers, schemes_array, I
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "lib", "libmyql_oracle.so")
 LIB_FAST = os.path.join(HERE, "lib", "libmyql_oracle_fast.so")
+LIB_OPS = os.path.join(HERE, "lib", "libmyql_oracle_ops.so")
 
 
 def build(quiet: bool = True) -> None:
@@ -31,14 +32,18 @@ def build(quiet: bool = True) -> None:
 
 
 class Oracle:
-    def __init__(self, fast: bool = False):
-        path = LIB_FAST if fast else LIB
+    def __init__(self, fast: bool = False, ops: bool = False):
+        """fast: the reference-flags build (CPU baseline); ops: the op-counting
+        build (flops per path-step; run it with nthreads=1)."""
+        path = LIB_OPS if ops else (LIB_FAST if fast else LIB)
         if not os.path.exists(path):
             build()
         lib = C.CDLL(path)
         u64p = C.POINTER(C.c_uint64)
         dp = C.POINTER(C.c_double)
         lib.oracle_last_error.restype = C.c_char_p
+        lib.oracle_op_count.restype = C.c_uint64
+        lib.oracle_op_count.argtypes = [C.c_int]
         lib.oracle_seed_state.argtypes = [C.c_uint64, u64p]
         lib.oracle_next.argtypes = [u64p, C.c_int64, u64p]
         lib.oracle_char_poly.argtypes = [u64p]
@@ -70,6 +75,13 @@ class Oracle:
                                               C.POINTER(capi.MLMCConfig), C.c_double, C.c_double,
                                               C.c_double, C.POINTER(capi.MLMCResult), C.c_int]
         self.lib = lib
+
+    def op_count(self, reset: bool = False) -> int:
+        """Floating-point ops counted since the last reset (ops build only)."""
+        n = int(self.lib.oracle_op_count(int(reset)))
+        if n == 2 ** 64 - 1:
+            raise RuntimeError("op counting needs Oracle(ops=True)")
+        return n
 
     # ---- RNG -------------------------------------------------------------
     @staticmethod

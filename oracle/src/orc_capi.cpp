@@ -112,6 +112,19 @@ uint64_t oracle_nci_fallbacks(int reset) {
   return n;
 }
 
+// floating-point operations counted since the last reset (builds with
+// ORC_COUNT_OPS only; UINT64_MAX otherwise)
+uint64_t oracle_op_count(int reset) {
+#ifdef ORC_COUNT_OPS
+  const uint64_t n = g_ops;
+  if (reset) g_ops = 0;
+  return n;
+#else
+  (void)reset;
+  return UINT64_MAX;
+#endif
+}
+
 void oracle_seed_state(uint64_t seed, uint64_t *out) {
   Xoshiro g = seed_state(seed);
   std::memcpy(out, g.s, 32);
@@ -279,6 +292,7 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
       auto add = [&](int st, size_t i, double x) {
         acc[(2 * st) * nk + i] += x;
         acc[(2 * st + 1) * nk + i] += x * x;
+        ORC_OPS(3);
       };
       auto run = [&](const Stepper &stp, Xoshiro &g, double *p, double *u, double *d) {
         PathState ps;
@@ -299,6 +313,7 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
             if (ess) {
               add(ST_DELTA, i, (pu[i] - pdn[i]) * (1.0 / (2.0 * h)));
               add(ST_GAMMA, i, (pu[i] - pb[i] * 2.0 + pdn[i]) * (1.0 / (h * h)));
+              ORC_OPS(6);
             }
           }
           if (!full) continue;
@@ -309,6 +324,7 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
             for (size_t i = 0; i < nk; ++i) {
               add(ST_VEGA0 + f, i, (a_p[i] - b_p[i]) * inv2dv * cv[f]);
               add(ST_VANNA0 + f, i, ((a_u[i] - a_d[i]) - (b_u[i] - b_d[i])) * (inv2dv / (2.0 * h)) * cv[f]);
+              ORC_OPS(8);
             }
           }
           {
@@ -318,13 +334,17 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
             for (size_t i = 0; i < nk; ++i) {
               add(ST_THETA, i, (a_p[i] * dfTu - b_p[i] * dfTd) * inv2dT);
               add(ST_CHARM, i, ((a_u[i] - a_d[i]) * dfTu - (b_u[i] - b_d[i]) * dfTd) * (inv2dT / (2.0 * h)));
+              ORC_OPS(11);
             }
           }
           {
             Xoshiro g1 = snap, g2 = snap;
             run(rup, g1, a_p.data(), a_u.data(), a_d.data());
             run(rdn, g2, b_p.data(), b_u.data(), b_d.data());
-            for (size_t i = 0; i < nk; ++i) add(ST_RHO, i, (a_p[i] * dfru - b_p[i] * dfrd) * inv2dr);
+            for (size_t i = 0; i < nk; ++i) {
+              add(ST_RHO, i, (a_p[i] * dfru - b_p[i] * dfrd) * inv2dr);
+              ORC_OPS(4);
+            }
           }
         }
       }

@@ -68,6 +68,7 @@ static inline size_t nci_rows_nmax(const myql_heston_params &p, double dt, bool 
 // slopes dq/du = 1/pdf(q).
 static inline std::shared_ptr<NciTable> build_nci_table(const myql_heston_params &p, double dt,
                                                         bool reference_rows, long n_fb = -1) {
+  const OpsPause setup_work;
   auto gw = std::make_shared<NciTable>();
   gw->d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
   double nm = static_cast<double>(nci_rows_nmax(p, dt, reference_rows));
@@ -189,6 +190,7 @@ static inline void prepare_factor(Factor &f, int scheme, const myql_heston_param
 
 static inline double nv_coupled(const Factor &f, double v, double Z) {
   const double root = std::sqrt(f.K[0] * v + f.K[1]) + f.K[2] * Z;
+  ORC_OPS(8);  // FMA, sqrt, FMA, square, FMA
   return f.K[0] * (root * root) + f.K[1];
 }
 
@@ -202,16 +204,20 @@ static inline double exact_gamma(const Factor &f, int N, Xoshiro &g) {
       do {
         Z = normcdfinv(uniform(g));
         v = 1.0 + c * Z;
+        ORC_OPS(2);
       } while (v <= 0.0);
       v = v * v * v;
       const double U = uniform(g);
+      ORC_OPS(7);  // cube, squeeze test
       if (U < 1.0 - 0.0331 * Z * Z * Z * Z) return d * v;
+      ORC_OPS(8);  // log test
       if (std::log(U) < 0.5 * Z * Z + d * (1.0 - v + std::log(v))) return d * v;
     }
   }
   for (;;) {
     const double u = uniform(g);
     const double y = exponential(1.0, g);
+    ORC_OPS(6);
     if (u < f.sm_p) {
       const double x = std::exp(-y * f.sm_inv_a);
       if (u * f.sm_inv_p < 1.0 - x) return x;
@@ -241,6 +247,7 @@ static inline double nci_lookup(const Factor &f, int N, double U) {
     const size_t off = static_cast<size_t>(N) * t.grid + i;
     const double y0 = t.y[off], y1 = t.y[off + 1], m0 = t.m[off], m1 = t.m[off + 1];
     const double t2 = tt * tt, t3 = t2 * tt;
+    ORC_OPS(28);  // position, Hermite basis, 4-term combination, scale
     return ((2 * t3 - 3 * t2 + 1) * y0 + (t3 - 2 * t2 + tt) * t.dx * m0 + (-2 * t3 + 3 * t2) * y1 +
             (t3 - t2) * t.dx * m1) *
            f.K[2];
@@ -255,11 +262,13 @@ static inline double nci_lookup(const Factor &f, int N, double U) {
     const double tt = pos - static_cast<double>(j);
     const size_t off = static_cast<size_t>(N) * P + j;
     const double t2 = tt * tt, t3 = t2 * tt;
+    ORC_OPS(31);  // as above plus the log1p of the tail coordinate
     return ((2 * t3 - 3 * t2 + 1) * t.ty[off] + (t3 - 2 * t2 + tt) * t.thw * t.tm[off] +
             (-2 * t3 + 3 * t2) * t.ty[off + 1] + (t3 - t2) * t.thw * t.tm[off + 1]) *
            f.K[2];
   }
   nci_fallbacks().fetch_add(1, std::memory_order_relaxed);
+  ORC_OPS(3);
   return chi2_quantile(f.K[1] + 2.0 * N, U) * f.K[2];
 }
 
@@ -271,11 +280,13 @@ static inline double evolve(const Factor &f, double v, Xoshiro &g) {
       return nv_coupled(f, v, Z);
     }
     case MYQL_SCHEME_NCI: {
+      ORC_OPS(1);  // Poisson intensity
       const int N = static_cast<int>(poissinvcdf(f.K[0] * v, uniform(g)));
       const double U = uniform(g);
       return nci_lookup(f, N, U);
     }
     case MYQL_SCHEME_EXACT: {
+      ORC_OPS(2);  // intensity, scale
       const int N = static_cast<int>(poissinvcdf(f.K[0] * v, uniform(g)));
       return exact_gamma(f, N, g) * f.K[2];
     }
@@ -323,6 +334,7 @@ static inline double log_jump(const JumpWs &w, Xoshiro &g) {
   if (n == 0) return w.drift;
   if (w.kind == MYQL_JUMP_MERTON) {
     const double Z = normcdfinv(uniform(g));
+    ORC_OPS(6);
     return w.drift + n * w.a + std::sqrt(static_cast<double>(n)) * w.b * Z;
   }
   double total = 0.0;
@@ -331,6 +343,7 @@ static inline double log_jump(const JumpWs &w, Xoshiro &g) {
     if (u < w.a) total += exponential(w.b, g);
     else total -= exponential(w.c, g);
   }
+  ORC_OPS(n + 1);
   return w.drift + total;
 }
 
@@ -394,10 +407,12 @@ static inline void trk_update(PathState &s, const TrackerCfg &c, double dt) {
   switch (c.kind) {
     case MYQL_INSTR_GEO_ASIAN:
       s.sumLogS += s.logS * dt;
+      ORC_OPS(2);
       break;
     case MYQL_INSTR_ARITH_ASIAN:
       s.sumLogS += s.logS * dt;
       s.sumS += std::exp(s.logS) * dt;
+      ORC_OPS(5);
       break;
     case MYQL_INSTR_LOOKBACK:
       s.maxLogS = (s.logS > s.maxLogS) ? s.logS : s.maxLogS;
@@ -405,6 +420,7 @@ static inline void trk_update(PathState &s, const TrackerCfg &c, double dt) {
       break;
     case MYQL_INSTR_BARRIER:
       s.t += dt;
+      ORC_OPS(1);
       // nested tests, easiest barrier first (PathTrackers.hpp:250-294)
       if (c.up) {
         if (s.logS >= c.b_up) {
@@ -452,6 +468,7 @@ static inline void trk_finalize(PathState &s, const TrackerCfg &c, double dt, do
       break;
   }
   s.logS = std::exp(s.logS);
+  ORC_OPS(c.kind == MYQL_INSTR_EUROPEAN ? 1 : 7);
 }
 
 // ---------------------------------------------------------------------------
@@ -506,6 +523,7 @@ static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg 
   if (st.nf == 0 && !st.pathdep) {
     const double G = normcdfinv(uniform(g));
     x += st.gbm_drift * n + st.gbm_vol * std::sqrt(static_cast<double>(n)) * G;
+    ORC_OPS(6);
     x += log_jump(st.jump, g);
     s.logS = x;
     trk_finalize(s, c, st.dt, total_time);
@@ -515,12 +533,14 @@ static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg 
     if (st.nf == 0) {
       const double G = normcdfinv(uniform(g));
       x += st.gbm_drift + st.gbm_vol * G;
+      ORC_OPS(3);
     } else if (st.nf == 1) {
       const double *C = st.f[0].C;
       const double G = normcdfinv(uniform(g));
       const double vn = evolve(st.f[0], v0, g);
       const double root = std::sqrt(C[3] * (v0 + vn));
       x += C[0] + C[1] * v0 + C[2] * vn + root * G;
+      ORC_OPS(10);  // log-S step: C3 (v + v'), sqrt, three FMAs, adds
       v0 = vn;
     } else {
       const double *C0 = st.f[0].C, *C1 = st.f[1].C;
@@ -529,6 +549,7 @@ static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg 
       const double vn1 = evolve(st.f[1], v1, g);
       const double root = std::sqrt(C0[3] * (v0 + vn0) + C1[3] * (v1 + vn1));
       x += C0[0] + C0[1] * v0 + C0[2] * vn0 + C1[0] + C1[1] * v1 + C1[2] * vn1 + root * G;
+      ORC_OPS(19);
       v0 = vn0;
       v1 = vn1;
     }
@@ -693,14 +714,17 @@ struct Instr {
 
 static inline double payoff(const Instr &I, int mode, double spot, double strike) {
   const double diff = (I.in.option_type == MYQL_CALL) ? (spot - strike) : (strike - spot);
+  ORC_OPS(1);
   switch (I.in.payoff) {
     case MYQL_PAYOFF_VANILLA:
       return diff > 0.0 ? diff : 0.0;
     case MYQL_PAYOFF_CASH:
       if (mode == MYQL_GREEK_NONE) return diff > 0.0 ? 1.0 : 0.0;
+      ORC_OPS(4);
       return 0.5 * std::erfc(-diff * 0.70710678118654752440 / I.eps);
     case MYQL_PAYOFF_ASSET:
       if (mode == MYQL_GREEK_NONE) return diff > 0.0 ? spot : 0.0;
+      ORC_OPS(5);
       return spot * 0.5 * std::erfc(-diff * 0.70710678118654752440 / I.eps);
   }
   return 0.0;
@@ -720,6 +744,7 @@ static inline void to_buffer(const Instr &I, int mode, const PathState &s, doubl
     if (kind == MYQL_INSTR_EUROPEAN) {
       base[i] = payoff(I, mode, ST, K);
       if (bumps) {
+        ORC_OPS(2);
         up[i] = payoff(I, mode, ST * mu, K);
         dn[i] = payoff(I, mode, ST * md, K);
       }

@@ -44,12 +44,14 @@ static inline double normcdfinv(double p) {
     const double q = p - 0.5, r = q * q;
     const double num = (((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q;
     const double den = ((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1.0;
+    ORC_OPS(24);  // q, r, 5+5 FMA, *q, divide
     return num / den;
   }
   const double q = (p <= lo) ? p : 1.0 - p;
   const double r = std::sqrt(-2.0 * std::log(q));
   const double x = (((((c[0] * r + c[1]) * r + c[2]) * r + c[3]) * r + c[4]) * r + c[5]) /
                    ((((d[0] * r + d[1]) * r + d[2]) * r + d[3]) * r + 1.0);
+  ORC_OPS(23);  // 1-p, log, *-2, sqrt, 5+4 FMA, divide
   return (p <= lo) ? x : -x;
 }
 
@@ -61,6 +63,7 @@ static inline double poissinvcdf(double lam, double U) {
   const double V = 1.0 - U;
   const double ilam = 1.0 / lam;
   double X = 0.0, S = 0.0, Del = 0.0;
+  ORC_OPS(2);
 
   if (lam > 4.0) {
     double W = normcdfinv(U < V ? U : V);
@@ -74,6 +77,7 @@ static inline double poissinvcdf(double lam, double U) {
       Del = 1.0 / 40.0 + Del * (W * W);
       Del = Del * ilam;
       S = lam + (S + Del);
+      ORC_OPS(19);
     } else {
       // tails: Newton iteration for r in sqrt(2(1 - r + r log r)) = S
       S = W / std::sqrt(lam);
@@ -87,6 +91,7 @@ static inline double poissinvcdf(double lam, double U) {
         if (R < 1.0) S2 = -S2;
         R = Rprev - (S2 - S) * S2 / Tl;
         if (R < 0.1 * Rprev) R = 0.1 * Rprev;
+        ORC_OPS(12);
       } while (std::fabs(R - Rprev) > 1e-8);
       const double Tl = std::log(R);
       S = lam * R + std::log(std::sqrt(2.0 * R * ((1.0 - R) + R * Tl)) /
@@ -94,8 +99,10 @@ static inline double poissinvcdf(double lam, double U) {
       S = S - 0.0218 / (S + 0.065 * lam);
       Del = 0.01 / S;
       S = S + Del;
+      ORC_OPS(20);
     }
     X = std::floor(S);
+    ORC_OPS(1);
 
     if (S > 10.0 && S < X + 2.0 * Del) {
       if (X > 0.5 * lam && X < 2.0 * lam) {
@@ -104,6 +111,7 @@ static inline double poissinvcdf(double lam, double U) {
         double Eta = X * ilam;
         Eta = std::sqrt(2.0 * (1.0 - Eta + Eta * std::log(Eta)) / Eta);
         if (X > lam) Eta = -Eta;
+        ORC_OPS(10);
         // Coefficient table of the expansion (Giles, Alg. 955), applied as
         // the two interleaved Horner recurrences B0/B1 in 1/X.
         static const double cb1[13] = {
@@ -131,9 +139,11 @@ static inline double poissinvcdf(double lam, double U) {
           B0 = cb0[j] + mult * B0 * Xi;
           Sm = B0 + Sm * Eta;
           --mult;
+          ORC_OPS(10);
         }
         Sm = Sm / (1.0 + B1 * Xi);
         Sm = Sm * std::exp(-0.5 * X * Eta * Eta) / std::sqrt(2.0 * 3.141592653589793 * X);
+        ORC_OPS(19);  // Horner start, normalisation, exp/sqrt factor, erfc term
         if (X < lam) {
           Sm += 0.5 * std::erfc(Eta * std::sqrt(0.5 * X));
           if (Sm > U) X -= 1.0;
@@ -152,6 +162,7 @@ static inline double poissinvcdf(double lam, double U) {
         Sm = (1.0 / 12.0) + Sm * Xi * Xi;
         Sm = Sm * Xi;
         Sm = (X - lam) - X * std::log(X * ilam) - Sm;
+        ORC_OPS(24);
         if (X < lam) {
           double Tt = std::exp(-0.5 * Sm);
           Sm = 1.0 - Tt * (U * Tt) * std::sqrt(2.0 * 3.141592653589793 * Xi) * lam;
@@ -162,6 +173,7 @@ static inline double poissinvcdf(double lam, double U) {
             Tt *= Xi * ilam;
             Sm += Tt;
           }
+          ORC_OPS(6 + 49 * 4);
           if (Sm > 0.0) X -= 1.0;
         } else {
           const double Tt = std::exp(-0.5 * Sm);
@@ -171,6 +183,7 @@ static inline double poissinvcdf(double lam, double U) {
             Xi += 1.0;
             Sm = Sm * Xi * ilam + 1.0;
           }
+          ORC_OPS(6 + 50 * 4);
           if (Sm < 0.0) X -= 1.0;
         }
       }
@@ -184,11 +197,13 @@ static inline double poissinvcdf(double lam, double U) {
     Del = 0.0;
     if (U > 0.5) Del = Tt * (1e-13 * Tt);
     double Sm = 1.0 - Tt * (U * Tt) + Del;
+    ORC_OPS(7);
     while (Sm < 0.0) {
       X += 1.0;
       Tt = X * ilam;
       Del = Tt * Del;
       Sm = Tt * Sm + 1.0;
+      ORC_OPS(5);
     }
     if (Sm < 2.0 * Del) {
       // top-down summation when rounding leaves the answer ambiguous
@@ -198,6 +213,7 @@ static inline double poissinvcdf(double lam, double U) {
       while (Del < Tt) {
         X += 1.0;
         Del *= X * ilam;
+        ORC_OPS(3);
       }
       Sm = Del;
       Tt = 1.0;
@@ -205,6 +221,7 @@ static inline double poissinvcdf(double lam, double U) {
         Tt *= X * ilam;
         Sm -= Tt;
         X -= 1.0;
+        ORC_OPS(4);
       }
     }
   }
@@ -221,12 +238,14 @@ static inline int poisson_inversion(double mean, double exp_mean, Xoshiro &g) {
     u = u - p;
     ++x;
     p = mean * p / x;
+    ORC_OPS(3);
     if (p == 0.0) break;  // u left above a rounded-down total: stop
   }
   return x;
 }
 
 static inline double exponential(double rate, Xoshiro &g) {
+  ORC_OPS(3);
   return -std::log1p(-uniform(g)) / rate;
 }
 
@@ -242,8 +261,10 @@ static inline double gamma_p_series(double a, double x) {
     ap += 1.0;
     term *= x / ap;
     sum += term;
+    ORC_OPS(4);
     if (term <= sum * 1e-16) break;
   }
+  ORC_OPS(7);
   return std::exp(a * std::log(x) - x - lgam(a + 1.0)) * sum;
 }
 
@@ -262,8 +283,10 @@ static inline double gamma_q_cf(double a, double x) {
     d = 1.0 / d;
     const double del = d * c;
     h *= del;
+    ORC_OPS(12);
     if (std::fabs(del - 1.0) <= 1e-15) break;  // a few ulps of 1, above FMA noise
   }
+  ORC_OPS(7);
   return std::exp(a * std::log(x) - x - lgam(a)) * h;
 }
 
@@ -329,6 +352,7 @@ static inline double gamma_inv(double a, double p, double q) {
       else if (lo > 0.0 && hi / lo > 4.0) xn = std::sqrt(lo * hi);
       else xn = 0.5 * (lo + hi);
     }
+    ORC_OPS(16);  // residual, log-pdf, exp, Halley step
     const double dx = std::fabs(xn - x);
     x = xn;
     // Halley converges cubically: a step below 1e-14 x leaves an error far

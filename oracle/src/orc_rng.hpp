@@ -22,6 +22,27 @@
 
 namespace orc {
 
+// Op counting (build with -DORC_COUNT_OPS: lib/libmyql_oracle_ops.so). Each
+// executed block adds its floating-point operation count -- an FMA counts 2;
+// an add, mul, div, sqrt, exp, log or erfc counts 1 -- to g_ops. The counts
+// are for the reference's algorithm as written (every Greek variant replays
+// its own path); integer RNG work is not counted. Single-threaded use only.
+// Set-up work (the NCI quantile tables) runs under OpsPause and is not
+// counted.
+#ifdef ORC_COUNT_OPS
+inline uint64_t g_ops = 0;
+inline bool g_ops_on = true;
+#define ORC_OPS(n) (::orc::g_ops += ::orc::g_ops_on ? static_cast<uint64_t>(n) : 0u)
+struct OpsPause {
+  bool was = g_ops_on;
+  OpsPause() { g_ops_on = false; }
+  ~OpsPause() { g_ops_on = was; }
+};
+#else
+#define ORC_OPS(n) ((void)0)
+struct OpsPause {};
+#endif
+
 struct Xoshiro {
   uint64_t s[4];
 };
@@ -64,6 +85,7 @@ static inline Xoshiro seed_state(uint64_t seed) {
 static inline double uniform(Xoshiro &g) {
   for (;;) {
     const double u = static_cast<double>(xnext(g)) * 0x1p-64;
+    ORC_OPS(1);
     if (u < 1.0) return u;
   }
 }
