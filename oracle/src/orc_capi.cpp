@@ -12,6 +12,29 @@
 // per-thread seeding seed + s * 999983 (MonteCarloPricer.hpp:191).
 // Full-mode CRN: every bumped variant replays the sample's draws from a copy
 // of the stream state taken before the base path (MonteCarloPricer.hpp:304).
+//
+// How each part is pinned (tests/test_oracle.py unless noted):
+//   xoshiro256++ next          golden outputs from an independent big-integer
+//                              restatement (test_xoshiro_golden)
+//   jump-ahead polynomials     equal the published JUMP / LONG_JUMP constants;
+//                              brute-force stepping (test_jump_*)
+//   splitmix64 seeding         golden vectors of the published splitmix64.
+//                              Equality with Boost's xoshiro256pp(u64) seeding:
+//                              parity unpinned (Boost is not in the image)
+//   uniform                    closed form; forced redraw of 1.0
+//   normcdfinv (Acklam)        scipy ndtri, max rel. error 1.2e-9
+//   poissinvcdf (Giles)        F(N-1) < U <= F(N) brackets from scipy
+//   poisson_inversion          scipy poisson.ppf away from CDF boundaries;
+//                              bit equality with Boost: parity unpinned
+//   exponential                inversion replaces Boost's ziggurat: parity
+//                              unpinned (statistical only: Kou prices vs
+//                              Fourier)
+//   chi2 quantile, P, Q        scipy golden values to 1e-13 relative
+//   NV / NCI / Exact CIR       conditional moments M1-M3 (test_cir_moments)
+//   stepper + pricer           Fourier (Gil-Pelaez) prices and Greeks within
+//                              4 SE (fixed seeds); barrier in + out = European; Asian
+//                              one-step identity and AM-GM; lookback closed
+//                              form; in-flight continuity; MLMC vs Fourier
 #include <omp.h>
 
 #include <array>
