@@ -180,7 +180,12 @@ __device__ __forceinline__ double giles_w(double U) {
   return U > V ? -W : W;
 }
 
-__device__ __forceinline__ double poissinv_w(double lam, double U, double W) {
+// DEFER: instead of running the rare branches (the tail Newton iteration for
+// |W| >= 3 and the Temme / Stirling correction near an integer boundary), set
+// *rare and return early; the caller then runs the full inversion for those
+// cases (poissinv), so the result is the same.
+template <bool DEFER = false>
+__device__ __forceinline__ double poissinv_w(double lam, double U, double W, bool *rare = nullptr) {
   if (U <= 0.0) return 0.0;
   if (U >= 1.0) return __builtin_nan("");
   const double V = 1.0 - U;
@@ -197,6 +202,10 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W) {
       Del = Del * ilam;
       S = lam + (S + Del);
     } else {
+      if constexpr (DEFER) {
+        *rare = true;
+        return 0.0;
+      }
       S = W / sqrt(lam);
       double R = 1.0 + S;
       if (R < 0.1) R = 0.1;
@@ -217,6 +226,10 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W) {
     }
     X = floor(S);
     if (S > 10.0 && S < X + 2.0 * Del) {
+      if constexpr (DEFER) {
+        *rare = true;
+        return X;
+      }
       if (X > 0.5 * lam && X < 2.0 * lam) {
         const double Xi = 1.0 / X;
         double Eta = X * ilam;

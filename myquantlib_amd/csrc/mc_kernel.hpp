@@ -266,6 +266,63 @@ __device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
   for (int k = 0; k < NV; ++k) x[k] += P[k]->jdrift + tot[k];
 }
 
+// ---- rare Giles branches, packed across the wavefront ----------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u)));
+}
+
+// Slots whose Poisson inversion hit one of the rare branches (bit k of
+// `rare`) are numbered slot-major across the active lanes of the wave
+// (ballot + mbcnt), written to this wave's LDS list, inverted by the active
+// lanes one item each, and read back by their owners. The wave sits through
+// the rare code once per 64 items instead of once per slot. Must be entered
+// by every active lane of the wave (a wave-uniform condition).
+template <int NV>
+__device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, unsigned rare, int (&Ns)[NV]) {
+  constexpr int W64 = 64;
+  __shared__ double s_lam[BLOCK / W64][W64];
+  __shared__ double s_u[BLOCK / W64][W64];
+  __shared__ int s_n[BLOCK / W64][W64];
+  const int wave = threadIdx.x / W64;
+  const unsigned long long active = __ballot(1);
+  const int n_active = __popcll(active);
+  const int rank = lanes_below(active);
+  int idx[NV];
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const unsigned long long b = __ballot((rare >> k) & 1u);
+    idx[k] = total + lanes_below(b);
+    total += __popcll(b);
+  }
+  for (int base = 0; base < total; base += n_active) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int g = idx[k] - base;
+      if (((rare >> k) & 1u) && g >= 0 && g < n_active) {
+        s_lam[wave][g] = lams[k];
+        s_u[wave][g] = U;
+      }
+    }
+    wave_sync();
+    if (rank < total - base) s_n[wave][rank] = static_cast<int>(poissinv(s_lam[wave][rank], s_u[wave][rank]));
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int g = idx[k] - base;
+      if (((rare >> k) & 1u) && g >= 0 && g < n_active) Ns[k] = s_n[wave][g];
+    }
+    wave_sync();
+  }
+}
+
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
 template <typename R, int NV, int NFX>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
@@ -307,15 +364,19 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         Ns[k] = 0;
       }
       const double W = (big && u1 > 0.0) ? giles_w(u1) : 0.0;
+      unsigned rare = 0;
 #pragma unroll 1
       for (int k = 0; k < NV; ++k) {
         double lam = lams[0];
 #pragma unroll
         for (int j = 1; j < NV; ++j) lam = (k == j) ? lams[j] : lam;
-        const int n = static_cast<int>(poissinv_w(lam, u1, W));
+        bool r = false;
+        const int n = static_cast<int>(poissinv_w<true>(lam, u1, W, &r));
+        rare |= r ? (1u << k) : 0u;
 #pragma unroll
         for (int j = 0; j < NV; ++j) Ns[j] = (k == j) ? n : Ns[j];
       }
+      if (__any(rare != 0u)) giles_rare<NV>(lams, u1, rare, Ns);
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -388,8 +449,10 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
         for (int k = 0; k < NV; ++k) x[k] += P[k]->gdrift + P[k]->gvol * G;
       } else {
         R vn[NV][NFX];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) evolve_factor<R, NV, NFX>(J, P, f, v, vn, g);
+        // factors in draw order; written out so the factor index is a
+        // compile-time constant and the slot arrays stay in registers
+        evolve_factor<R, NV, NFX>(J, P, 0, v, vn, g);
+        if constexpr (NF > 1) evolve_factor<R, NV, NFX>(J, P, 1, v, vn, g);
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
           R a = 0, d = 0;
