@@ -215,6 +215,58 @@ int myql_mlmc_calculate(myql_ctx *ctx, const myql_model *model,
                         int32_t greek_mode, const myql_mlmc_config *cfg,
                         double S0, double r, double q, myql_mlmc_result *res);
 
+/* ---- learned martingale control variate ---------------------------------
+ * No counterpart in the reference (its README.md:332 lists control variates
+ * as future work). u(t, x) ~ E[e^{-r(T-t)} P | X_t = x] is an MLP
+ * 6 -> 64 -> 64 -> 64 -> 1 (SiLU, 8,833 parameters) on the features
+ * (t, log(S/K), v1, v2, T, log(K/S0)), trained by regression of discounted
+ * payoffs on engine paths. Pricing then subtracts, per path and strike,
+ *   CV = sum_j e^{r(T - t_j)} grad_x u(t_j, X_j) . sigma(X_j) dW_j
+ * over `points` control points (integrand frozen on each block, dW the
+ * stepper's own increments), which has zero mean for any weights. Europeans
+ * with price only (GreekMode None); all jobs of one call share the model,
+ * schemes, payoff, option type and (S0, r, q); at most 64 strikes per job. */
+enum { MYQL_CV_NPARAM = 8833, MYQL_CV_MAX_POINTS = 16 };
+
+typedef struct {
+  uint64_t train_paths;  /* paths per job in the training run */
+  uint64_t train_seed;   /* seed of the training paths and minibatches */
+  uint32_t steps;        /* Adam steps */
+  uint32_t batch;        /* rows per Adam step (multiple of 32) */
+  double lr;             /* Adam learning rate */
+  uint32_t points;       /* control points per path (1..16) */
+  uint32_t init_seed;    /* weight initialisation */
+} myql_cv_config;
+
+typedef struct {
+  /* W1[64][6], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4 */
+  float params[MYQL_CV_NPARAM];
+  float norm[6];       /* feature scaling fixed by the training call */
+  double final_loss;   /* mean squared error over the last steps (scaled by S0^2) */
+  double train_ms;     /* GPU time of recording and training */
+} myql_cv_net;
+
+/* Per job of myql_mc_calculate_batch_cv; every pointer is an n_strikes
+ * array or NULL. */
+typedef struct {
+  double *plain_price, *plain_std_err; /* plain estimate on the same paths */
+  double *var_ratio;                   /* plain variance / CV variance */
+  int32_t *used_cv;                    /* 1: result holds the CV estimate; 0:
+                                          CV variance not lower, plain kept */
+} myql_cv_result;
+
+/* Train a network for the option sets of `jobs` (their model, strikes and
+ * maturities; their num_paths and seed are not used). */
+int myql_cv_train(myql_ctx *ctx, const myql_mc_job *jobs, int64_t n_jobs, const myql_cv_config *cfg,
+                  myql_cv_net *net);
+
+/* Price each job (its own paths, seed, precision) with the control variate
+ * of `net`: result->price / price_std_err hold the CV estimate per strike,
+ * or the plain one where the CV does not lower the variance. */
+int myql_mc_calculate_batch_cv(myql_ctx *ctx, const myql_mc_job *jobs, int64_t n_jobs,
+                               const myql_cv_config *cfg, const myql_cv_net *net,
+                               myql_cv_result *cv_results);
+
 /* ---- diagnostics used by the parity tests ------------------------------
  * First n_per_stream raw xoshiro256++ outputs of streams [0, n_streams) for
  * (seed, layout), written stream-major into out (host memory). The draws
@@ -231,6 +283,14 @@ int myql_debug_poissinv(myql_ctx *ctx, const double *lambda, const double *u,
 /* Device chi-squared quantile (inverse regularised incomplete gamma). */
 int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
                              int64_t n, double *out);
+/* Control-variate network on n caller feature rows (n x 6, scaled features):
+ * the BF16 MFMA input gradient d net/d f_{1,2,3} used by the pricing kernel
+ * (out: n x 3), and the FP32 training gradient of mean (net - target)^2 over
+ * the rows (n a multiple of 32; grad: MYQL_CV_NPARAM; loss: the mean). */
+int myql_debug_cv_input_grad(myql_ctx *ctx, const float *params, const float *feats,
+                             int64_t n, float *out);
+int myql_debug_cv_param_grad(myql_ctx *ctx, const float *params, const float *feats,
+                             const float *targets, int64_t n, float *grad, double *loss);
 
 #ifdef __cplusplus
 }

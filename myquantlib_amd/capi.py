@@ -90,11 +90,32 @@ class MLMCResult(C.Structure):
                 ("beta_estimated", C.c_double), ("device_ms", C.c_double)]
 
 
+CV_NPARAM, CV_MAX_POINTS = 8833, 16
+
+
+class CVConfig(C.Structure):
+    _fields_ = [("train_paths", C.c_uint64), ("train_seed", C.c_uint64), ("steps", C.c_uint32),
+                ("batch", C.c_uint32), ("lr", C.c_double), ("points", C.c_uint32),
+                ("init_seed", C.c_uint32)]
+
+
+class CVNet(C.Structure):
+    _fields_ = [("params", C.c_float * CV_NPARAM), ("norm", C.c_float * 6),
+                ("final_loss", C.c_double), ("train_ms", C.c_double)]
+
+
+class CVResult(C.Structure):
+    _fields_ = [("plain_price", _PD), ("plain_std_err", _PD), ("var_ratio", _PD),
+                ("used_cv", C.POINTER(C.c_int32))]
+
+
 # Every extern "C" symbol declared in include/myql_capi.h.
 EXPORTED = [
     "myql_create", "myql_destroy", "myql_last_error", "myql_mc_calculate",
-    "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_debug_stream_draws",
-    "myql_debug_normcdfinv", "myql_debug_poissinv", "myql_debug_chi2_quantile",
+    "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_cv_train",
+    "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
+    "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_cv_input_grad",
+    "myql_debug_cv_param_grad",
 ]
 
 
@@ -134,4 +155,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_poissinv.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
     lib.myql_debug_chi2_quantile.restype = C.c_int
     lib.myql_debug_chi2_quantile.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
+    pf = C.POINTER(C.c_float)
+    lib.myql_cv_train.restype = C.c_int
+    lib.myql_cv_train.argtypes = [vp, C.POINTER(MCJob), C.c_int64, C.POINTER(CVConfig),
+                                  C.POINTER(CVNet)]
+    lib.myql_mc_calculate_batch_cv.restype = C.c_int
+    lib.myql_mc_calculate_batch_cv.argtypes = [vp, C.POINTER(MCJob), C.c_int64,
+                                               C.POINTER(CVConfig), C.POINTER(CVNet),
+                                               C.POINTER(CVResult)]
+    lib.myql_debug_cv_input_grad.restype = C.c_int
+    lib.myql_debug_cv_input_grad.argtypes = [vp, pf, pf, C.c_int64, pf]
+    lib.myql_debug_cv_param_grad.restype = C.c_int
+    lib.myql_debug_cv_param_grad.argtypes = [vp, pf, pf, pf, C.c_int64, pf, _PD]
     return lib

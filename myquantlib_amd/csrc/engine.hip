@@ -21,6 +21,8 @@
 #include "engine_types.hpp"
 #include "mc_launch.hpp"
 #include "mlmc_kernel.hpp"
+#include "cv_launch.hpp"
+#include "cv_types.hpp"
 
 using namespace myql;
 
@@ -1104,6 +1106,263 @@ void mlmc_validate(const myql_model *m, const int32_t *schemes, const myql_instr
   if (!(cfg->epsilon > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "epsilon must be positive");
 }
 
+// ---------------------------------------------------------------------------
+// learned control variate (cv_kernel.hpp)
+// ---------------------------------------------------------------------------
+void cv_validate(const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg) {
+  if (!jobs || n < 1 || !cfg) throw Fail(MYQL_E_INVALID_ARG, "control variate: no jobs or config");
+  if (cfg->points < 1 || cfg->points > MYQL_CV_MAX_POINTS) throw Fail(MYQL_E_INVALID_ARG, "points must be in 1..16");
+  const myql_mc_job &a = jobs[0];
+  for (int64_t i = 0; i < n; ++i) {
+    const myql_mc_job &b = jobs[i];
+    if (!b.model || !b.instrument || !b.config || !b.result) throw Fail(MYQL_E_INVALID_ARG, "null job field");
+    validate(b.model, b.schemes, b.instrument, b.config);
+    if (std::memcmp(b.model, a.model, sizeof(myql_model)) != 0 || b.schemes[0] != a.schemes[0] ||
+        b.schemes[1] != a.schemes[1])
+      throw Fail(MYQL_E_INVALID_ARG, "control variate: all jobs must share model and schemes");
+    const myql_instrument &ib = *b.instrument, &ia = *a.instrument;
+    if (ib.kind != MYQL_INSTR_EUROPEAN) throw Fail(MYQL_E_UNSUPPORTED, "control variate: European options only");
+    if (ib.payoff != ia.payoff || ib.option_type != ia.option_type)
+      throw Fail(MYQL_E_INVALID_ARG, "control variate: all jobs must share payoff and option type");
+    if (ib.n_strikes > 64) throw Fail(MYQL_E_UNSUPPORTED, "control variate: at most 64 strikes per job");
+    if (b.S0 != a.S0 || b.r != a.r || b.q != a.q) throw Fail(MYQL_E_INVALID_ARG, "control variate: jobs must share S0, r, q");
+  }
+}
+
+CvNorm cv_norm(const myql_mc_job *jobs, int64_t n) {
+  double Tmax = 0.0;
+  for (int64_t i = 0; i < n; ++i) Tmax = std::max(Tmax, jobs[i].instrument->maturity);
+  const myql_model &m = *jobs[0].model;
+  const double vbar = std::max(expected_avg_variance(m, Tmax), 1e-6);
+  CvNorm nm;
+  nm.ts = static_cast<float>(1.0 / Tmax);
+  nm.xs = static_cast<float>(1.0 / std::sqrt(vbar * Tmax));
+  nm.vs0 = m.n_factors > 0 ? static_cast<float>(1.0 / m.heston[0].theta) : 0.0f;
+  nm.vs1 = m.n_factors > 1 ? static_cast<float>(1.0 / m.heston[1].theta) : 0.0f;
+  nm.logS0 = static_cast<float>(std::log(jobs[0].S0));
+  nm.S0 = static_cast<float>(jobs[0].S0);
+  return nm;
+}
+
+// Owner of one recorded option set.
+struct CvRecord {
+  CvSet set{};
+  DevBuf rec, st, strikes, job;
+};
+
+// Record `paths` base paths of job `jb` (seed, stream layout and precision
+// as given) at `points` control points.
+template <typename R>
+void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, int points, CvRecord &out) {
+  const myql_model &m = *jb.model;
+  const myql_instrument &in = *jb.instrument;
+  const int nk = static_cast<int>(in.n_strikes);
+  JobSpec js{jb.model, jb.schemes, jb.instrument, MYQL_GREEK_NONE, &cfg, jb.S0, jb.r, jb.q, nullptr, 0, nk};
+  Job<R> J{};
+  std::vector<double> ks;
+  build_job<R>(c, js, J, ks);
+  J.strikes = nullptr;
+  J.partial = nullptr;
+  out.job.reserve(sizeof(Job<R>));
+  HIPCHECK(hipMemcpyAsync(out.job.p, &J, sizeof(Job<R>), hipMemcpyHostToDevice, c->stream));
+  const uint64_t M = cfg.num_paths, steps = cfg.time_steps;
+  const double T = in.maturity, dt = T / static_cast<double>(steps);
+  CvSet &S = out.set;
+  S = CvSet{};
+  S.M = M;
+  const int nf = m.n_factors;
+  S.nj = nf == 0 ? 1 : static_cast<int>(std::min<uint64_t>(points, steps));
+  for (int j = 0; j <= S.nj; ++j) S.step_of[j] = static_cast<int>((static_cast<uint64_t>(j) * steps) / S.nj);
+  for (int j = 0; j < S.nj; ++j) {
+    const double tj = S.step_of[j] * dt;
+    S.t_of[j] = static_cast<float>(tj);
+    S.disc_of[j] = std::exp(jb.r * (T - tj));
+  }
+  S.sqdt = static_cast<float>(nf == 0 ? std::sqrt(T) : std::sqrt(dt));
+  S.nf = nf;
+  S.vol = static_cast<float>(m.vol);
+  for (int f = 0; f < 2; ++f) {
+    const bool on = f < nf;
+    S.rho[f] = on ? static_cast<float>(m.heston[f].rho) : 0.0f;
+    S.sig[f] = on && jb.schemes[f] == MYQL_SCHEME_NV ? static_cast<float>(m.heston[f].sigma) : 0.0f;
+    S.rperp[f] = on ? static_cast<float>(1.0 - m.heston[f].rho * m.heston[f].rho) : 0.0f;
+  }
+  // schemes without a recorded variance increment keep only the stock's
+  // independent part: the correlated stock term needs the variance noise too
+  for (int f = 0; f < nf; ++f)
+    if (jb.schemes[f] != MYQL_SCHEME_NV) S.rho[f] = 0.0f;
+  out.strikes.reserve(nk * sizeof(double));
+  HIPCHECK(hipMemcpyAsync(out.strikes.p, in.strikes, nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  S.strikes = out.strikes.as<double>();
+  S.nk = nk;
+  S.payoff = in.payoff;
+  S.call = in.option_type == MYQL_CALL ? 1 : 0;
+  S.T = T;
+  S.df = std::exp(-jb.r * T);
+  S.inv_S0 = 1.0 / jb.S0;
+  out.rec.reserve(static_cast<size_t>(S.nj) * CV_REC * M * sizeof(float));
+  out.st.reserve(M * sizeof(double));
+  S.rec = out.rec.as<float>();
+  S.ST = out.st.as<double>();
+  HIPCHECK(launch_cv_paths<R>(nf, out.job.as<Job<R>>(), S, J.S, c->stream));
+}
+
+void cv_init_params(uint32_t seed, std::vector<float> &p) {
+  p.assign(CV_NPARAM, 0.0f);
+  uint64_t x = 0x5DEECE66Dull ^ seed;
+  auto u = [&]() {  // splitmix64 -> [0, 1)
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return static_cast<double>(z >> 11) * 0x1p-53;
+  };
+  auto fill = [&](int off, int fan_in, int fan_out, int count) {  // Glorot uniform
+    const double a = std::sqrt(6.0 / (fan_in + fan_out));
+    for (int i = 0; i < count; ++i) p[off + i] = static_cast<float>((2.0 * u() - 1.0) * a);
+  };
+  fill(CV_W1, CV_IN, CV_H, CV_H * CV_IN);
+  fill(CV_W2, CV_H, CV_H, CV_H * CV_H);
+  fill(CV_W3, CV_H, CV_H, CV_H * CV_H);
+  fill(CV_W4, CV_H, 1, CV_H);
+}
+
+void cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg, myql_cv_net *net) {
+  if (cfg->batch < CV_TROWS || cfg->batch % CV_TROWS != 0) throw Fail(MYQL_E_INVALID_ARG, "batch must be a multiple of 32");
+  if (cfg->steps < 1 || cfg->train_paths < 1) throw Fail(MYQL_E_INVALID_ARG, "need steps >= 1 and train_paths >= 1");
+  const CvNorm nm = cv_norm(jobs, n);
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  std::vector<std::unique_ptr<CvRecord>> recs;
+  std::vector<CvSet> sets;
+  for (int64_t i = 0; i < n; ++i) {
+    myql_mc_config tc = *jobs[i].config;
+    tc.num_paths = cfg->train_paths;
+    tc.seed = cfg->train_seed;
+    tc.precision = MYQL_FP32;
+    tc.num_streams = 0;
+    recs.push_back(std::make_unique<CvRecord>());
+    cv_record<float>(c, jobs[i], tc, static_cast<int>(cfg->points), *recs.back());
+    sets.push_back(recs.back()->set);
+  }
+  DevBuf d_sets, prm, mom, var, gpart, lpart, hist, step;
+  d_sets.reserve(sets.size() * sizeof(CvSet));
+  HIPCHECK(hipMemcpyAsync(d_sets.p, sets.data(), sets.size() * sizeof(CvSet), hipMemcpyHostToDevice, c->stream));
+  std::vector<float> p0;
+  cv_init_params(cfg->init_seed, p0);
+  const int nb = static_cast<int>(cfg->batch / CV_TROWS);
+  prm.reserve(CV_NPARAM * sizeof(float));
+  mom.reserve(CV_NPARAM * sizeof(float));
+  var.reserve(CV_NPARAM * sizeof(float));
+  gpart.reserve(static_cast<size_t>(nb) * CV_NPARAM * sizeof(float));
+  lpart.reserve(nb * sizeof(float));
+  hist.reserve(cfg->steps * sizeof(float));
+  step.reserve(sizeof(int));
+  HIPCHECK(hipMemcpyAsync(prm.p, p0.data(), CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemsetAsync(mom.p, 0, CV_NPARAM * sizeof(float), c->stream));
+  HIPCHECK(hipMemsetAsync(var.p, 0, CV_NPARAM * sizeof(float), c->stream));
+  HIPCHECK(hipMemsetAsync(step.p, 0, sizeof(int), c->stream));
+  CvTrainArgs a{prm.as<float>(), mom.as<float>(), var.as<float>(), gpart.as<float>(), lpart.as<float>(),
+                hist.as<float>(), d_sets.as<CvSet>(), static_cast<int>(n), nm, cfg->train_seed, step.as<int>(),
+                static_cast<int>(cfg->batch), static_cast<float>(cfg->lr), true, nullptr, nullptr};
+  // one Adam step (train, reduce + update, counter) captured as a graph and
+  // replayed; the step counter lives on the device, so replays need no
+  // argument changes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = launch_cv_train_step(a, c->stream);
+  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  HIPCHECK(le);
+  HIPCHECK(ce);
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(ie);
+  }
+  hipError_t ge = hipSuccess;
+  for (uint32_t it = 0; it < cfg->steps && ge == hipSuccess; ++it) ge = hipGraphLaunch(exec, c->stream);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  HIPCHECK(ge);
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  std::vector<float> h(cfg->steps);
+  HIPCHECK(hipMemcpyAsync(net->params, prm.p, CV_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(h.data(), hist.p, cfg->steps * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  const uint32_t tail = std::max<uint32_t>(1, std::min<uint32_t>(50, cfg->steps));
+  double l = 0.0;
+  for (uint32_t i = cfg->steps - tail; i < cfg->steps; ++i) l += h[i];
+  net->final_loss = l / tail;
+  net->train_ms = ms;
+  const float nv[6] = {nm.ts, nm.xs, nm.vs0, nm.vs1, nm.logS0, nm.S0};
+  std::memcpy(net->norm, nv, sizeof(nv));
+}
+
+template <typename R>
+void cv_price_one(myql_ctx *c, const myql_mc_job &jb, const myql_cv_config *cfg, const DevBuf &prm, const CvNorm &nm,
+                  myql_cv_result *cr, double &ms_total) {
+  CvRecord rec;
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  cv_record<R>(c, jb, *jb.config, static_cast<int>(cfg->points), rec);
+  const CvSet &S = rec.set;
+  const uint64_t M = S.M;
+  const int blocks = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(c->n_cu) * 8,
+                                                                                (M + 3) / 4)));
+  DevBuf part, sums;
+  part.reserve(static_cast<size_t>(blocks) * (BLOCK / 64) * 64 * 4 * sizeof(double));
+  sums.reserve(64 * 4 * sizeof(double));
+  HIPCHECK(launch_cv_eval(prm.as<float>(), nm, S, part.as<double>(), blocks, sums.as<double>(), c->stream));
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  std::vector<double> h(S.nk * 4);
+  HIPCHECK(hipMemcpyAsync(h.data(), sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  ms_total += ms;
+  const double invM = 1.0 / static_cast<double>(M), df = S.df;
+  myql_mc_result &res = *jb.result;
+  for (int k = 0; k < S.nk; ++k) {
+    const double mc = h[4 * k] * invM, vc = std::max(h[4 * k + 1] * invM - mc * mc, 0.0);
+    const double mp = h[4 * k + 2] * invM, vp = std::max(h[4 * k + 3] * invM - mp * mp, 0.0);
+    const bool use = vc < vp;
+    if (res.price) res.price[k] = (use ? mc : mp) * df;
+    if (res.price_std_err) res.price_std_err[k] = std::sqrt((use ? vc : vp) * invM) * df;
+    if (cr) {
+      if (cr->plain_price) cr->plain_price[k] = mp * df;
+      if (cr->plain_std_err) cr->plain_std_err[k] = std::sqrt(vp * invM) * df;
+      if (cr->var_ratio) cr->var_ratio[k] = vc > 0.0 ? vp / vc : (vp > 0.0 ? INFINITY : 1.0);
+      if (cr->used_cv) cr->used_cv[k] = use ? 1 : 0;
+    }
+  }
+  res.total_steps = M * jb.config->time_steps;
+  res.device_ms = ms;
+}
+
+void cv_price(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg, const myql_cv_net *net,
+              myql_cv_result *cr) {
+  CvNorm nm;
+  nm.ts = net->norm[0];
+  nm.xs = net->norm[1];
+  nm.vs0 = net->norm[2];
+  nm.vs1 = net->norm[3];
+  nm.logS0 = net->norm[4];
+  nm.S0 = net->norm[5];
+  DevBuf prm;
+  prm.reserve(CV_NPARAM * sizeof(float));
+  HIPCHECK(hipMemcpyAsync(prm.p, net->params, CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  double ms = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (jobs[i].config->precision == MYQL_FP32)
+      cv_price_one<float>(c, jobs[i], cfg, prm, nm, cr ? cr + i : nullptr, ms);
+    else
+      cv_price_one<double>(c, jobs[i], cfg, prm, nm, cr ? cr + i : nullptr, ms);
+  }
+  (void)ms;
+}
+
 int fail(myql_ctx *c, const Fail &e) {
   if (c) c->err = e.what();
   return e.code;
@@ -1312,6 +1571,97 @@ int myql_debug_poissinv(myql_ctx *c, const double *lambda, const double *u, int6
 int myql_debug_chi2_quantile(myql_ctx *c, const double *dof, const double *p, int64_t n, double *out) {
   if (!dof || !p || !out) return MYQL_E_INVALID_ARG;
   return debug_map(c, 2, dof, p, n, out, nullptr);
+}
+
+int myql_cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n_jobs, const myql_cv_config *cfg, myql_cv_net *net) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!net) throw Fail(MYQL_E_INVALID_ARG, "null net");
+    cv_validate(jobs, n_jobs, cfg);
+    cv_train(c, jobs, n_jobs, cfg, net);
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_mc_calculate_batch_cv(myql_ctx *c, const myql_mc_job *jobs, int64_t n_jobs, const myql_cv_config *cfg,
+                               const myql_cv_net *net, myql_cv_result *cv_results) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!net) throw Fail(MYQL_E_INVALID_ARG, "null net");
+    cv_validate(jobs, n_jobs, cfg);
+    cv_price(c, jobs, n_jobs, cfg, net, cv_results);
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_debug_cv_input_grad(myql_ctx *c, const float *params, const float *feats, int64_t n, float *out) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!params || !feats || !out || n < 1) throw Fail(MYQL_E_INVALID_ARG, "bad arguments");
+    DevBuf p, f, o;
+    p.reserve(CV_NPARAM * sizeof(float));
+    f.reserve(n * CV_IN * sizeof(float));
+    o.reserve(n * 3 * sizeof(float));
+    HIPCHECK(hipMemcpyAsync(p.p, params, CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(f.p, feats, n * CV_IN * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(launch_cv_grad_debug(p.as<float>(), f.as<float>(), static_cast<int>(n), o.as<float>(), c->stream));
+    HIPCHECK(hipMemcpyAsync(out, o.p, n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_debug_cv_param_grad(myql_ctx *c, const float *params, const float *feats, const float *targets, int64_t n,
+                             float *grad, double *loss) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!params || !feats || !targets || !grad || n < CV_TROWS || n % CV_TROWS != 0)
+      throw Fail(MYQL_E_INVALID_ARG, "bad arguments (n must be a positive multiple of 32)");
+    const int nb = static_cast<int>(n / CV_TROWS);
+    DevBuf p, f, t, gp, lp, stp;
+    p.reserve(CV_NPARAM * sizeof(float));
+    f.reserve(n * CV_IN * sizeof(float));
+    t.reserve(n * sizeof(float));
+    gp.reserve(static_cast<size_t>(nb) * CV_NPARAM * sizeof(float));
+    lp.reserve(nb * sizeof(float));
+    stp.reserve(sizeof(int));
+    HIPCHECK(hipMemcpyAsync(p.p, params, CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(f.p, feats, n * CV_IN * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(t.p, targets, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemsetAsync(stp.p, 0, sizeof(int), c->stream));
+    CvTrainArgs a{p.as<float>(), nullptr, nullptr, gp.as<float>(), lp.as<float>(), nullptr, nullptr, 0, CvNorm{},
+                  0, stp.as<int>(), static_cast<int>(n), 0.f, false, f.as<float>(), t.as<float>()};
+    HIPCHECK(launch_cv_train_step(a, c->stream));
+    std::vector<float> g(static_cast<size_t>(nb) * CV_NPARAM), l(nb);
+    HIPCHECK(hipMemcpyAsync(g.data(), gp.p, g.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(l.data(), lp.p, nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    double ls = 0.0;
+    for (int b = 0; b < nb; ++b) ls += l[b];
+    for (int i = 0; i < CV_NPARAM; ++i) {
+      double s = 0.0;
+      for (int b = 0; b < nb; ++b) s += g[static_cast<size_t>(b) * CV_NPARAM + i];
+      grad[i] = static_cast<float>(s);
+    }
+    if (loss) *loss = ls / static_cast<double>(n);
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
 }
 
 }  // extern "C"

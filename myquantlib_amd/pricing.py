@@ -106,6 +106,11 @@ def mc_config(num_paths: int, time_steps: int, seed: int = 42, fd_bump: float = 
                     stream_layout, num_streams, flags)
 
 
+def cv_config(train_paths: int = 1 << 20, train_seed: int = 7, steps: int = 2000, batch: int = 8192,
+              lr: float = 1e-3, points: int = 8, init_seed: int = 1) -> "capi.CVConfig":
+    return capi.CVConfig(train_paths, train_seed, steps, batch, lr, points, init_seed)
+
+
 def mlmc_config(epsilon: float = 0.005, L_min: int = 2, L_max: int = 15, N0: int = 10000,
                 base_steps: int = 1, alpha: float = 2.0, beta: float = 1.0,
                 adaptive_rates: bool = True, seed: int = 42, fd_bump: float = 1e-4,
@@ -211,10 +216,9 @@ class Engine:
         self._check(rc)
         return buf.as_dict()
 
-    def mc_batch(self, jobs: list) -> list:
-        """jobs: list of (model, schemes, instr_spec, mode, cfg, S0, r, q)."""
-        keep, arr = [], (capi.MCJob * len(jobs))()
-        bufs = []
+    @staticmethod
+    def _job_array(jobs: list):
+        keep, arr, bufs = [], (capi.MCJob * len(jobs))(), []
         for i, (model, schemes, instr, mode, cfg, S0, r, q) in enumerate(jobs):
             s = instr.struct()
             b = ResultBuffers(s.n_strikes)
@@ -228,9 +232,67 @@ class Engine:
             j.config = C.pointer(cfg)
             j.S0, j.r, j.q = S0, r, q
             j.result = C.pointer(b.struct)
+        return arr, bufs, keep
+
+    def mc_batch(self, jobs: list) -> list:
+        """jobs: list of (model, schemes, instr_spec, mode, cfg, S0, r, q)."""
+        arr, bufs, _keep = self._job_array(jobs)
         rc = self.lib.myql_mc_calculate_batch(self.ctx, arr, len(jobs))
         self._check(rc)
         return [b.as_dict() for b in bufs]
+
+    # ---- learned control variate -----------------------------------------
+    def cv_train(self, jobs: list, cfg: "capi.CVConfig") -> "capi.CVNet":
+        """Train the control-variate network for the option sets of `jobs`
+        (same tuples as mc_batch; their path counts and seeds are unused)."""
+        arr, _bufs, _keep = self._job_array(jobs)
+        net = capi.CVNet()
+        self._check(self.lib.myql_cv_train(self.ctx, arr, len(jobs), C.byref(cfg), C.byref(net)))
+        return net
+
+    def mc_batch_cv(self, jobs: list, cfg: "capi.CVConfig", net: "capi.CVNet") -> list:
+        """Price `jobs` with the control variate; each result dict also holds
+        plain_price, plain_std_err, var_ratio and used_cv per strike."""
+        arr, bufs, _keep = self._job_array(jobs)
+        extra, crs = [], (capi.CVResult * len(jobs))()
+        pd = C.POINTER(C.c_double)
+        for i, b in enumerate(bufs):
+            e = {"plain_price": np.full(b.n, np.nan), "plain_std_err": np.full(b.n, np.nan),
+                 "var_ratio": np.full(b.n, np.nan), "used_cv": np.zeros(b.n, dtype=np.int32)}
+            extra.append(e)
+            crs[i].plain_price = e["plain_price"].ctypes.data_as(pd)
+            crs[i].plain_std_err = e["plain_std_err"].ctypes.data_as(pd)
+            crs[i].var_ratio = e["var_ratio"].ctypes.data_as(pd)
+            crs[i].used_cv = e["used_cv"].ctypes.data_as(C.POINTER(C.c_int32))
+        self._check(self.lib.myql_mc_calculate_batch_cv(self.ctx, arr, len(jobs), C.byref(cfg),
+                                                        C.byref(net), crs))
+        out = []
+        for b, e in zip(bufs, extra):
+            d = b.as_dict()
+            d.update({k: v.copy() for k, v in e.items()})
+            out.append(d)
+        return out
+
+    def cv_input_grad(self, params: np.ndarray, feats: np.ndarray) -> np.ndarray:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros((f.shape[0], 3), dtype=np.float32)
+        pf = C.POINTER(C.c_float)
+        self._check(self.lib.myql_debug_cv_input_grad(self.ctx, p.ctypes.data_as(pf), f.ctypes.data_as(pf),
+                                                      f.shape[0], out.ctypes.data_as(pf)))
+        return out
+
+    def cv_param_grad(self, params: np.ndarray, feats: np.ndarray, targets: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        g = np.zeros(capi.CV_NPARAM, dtype=np.float32)
+        loss = C.c_double(0.0)
+        pf = C.POINTER(C.c_float)
+        self._check(self.lib.myql_debug_cv_param_grad(self.ctx, p.ctypes.data_as(pf), f.ctypes.data_as(pf),
+                                                      t.ctypes.data_as(pf), f.shape[0],
+                                                      g.ctypes.data_as(pf), C.byref(loss)))
+        return g, loss.value
 
     def mlmc(self, model: Model, schemes, instr: InstrumentSpec, mode: int, cfg: MLMCConfig,
              S0: float, r: float, q: float) -> dict:
