@@ -12,9 +12,14 @@ sums FP64. Inputs are synthetic (no market data).
 Metric: path-steps/s = (sum over maturities of paths x steps x 7 Greek
 variants) / wall time, the 7 variants being the reference algorithm's
 base, v0 up/down, T up/down and r up/down replays. Also reported: ms to a
-1e-4 relative standard error for plain FP32 and plain FP64 GPU MC.
+1e-4 relative standard error (time x (worst se/price over options with
+price >= 0.1 / 1e-4)^2) for price-only plain FP32, plain FP64 and the learned
+martingale control variate (training included) on the strip's 512 options,
+the same for the Full-Greek runs, and the control variate on a Bates-Kou
+SchemeNV strip; the CPU baseline; the roofline of the path kernel.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths-log2 24] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths-log2 24] [--no-cpu] [--no-fp64]
+                [--no-cv] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -62,17 +67,62 @@ def model():
     return make_model(1, [Heston(2.0, 0.04, 1.0, -0.7, 0.04)], jump_kind=jk, jump=jp)
 
 
-def jobs(paths: int, precision: int, layout: int = 0, streams: int = 0):
+def jobs(paths: int, precision: int, layout: int = 0, streams: int = 0, mode: int = None,
+         seed: int = 42, bates_kou: bool = False):
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import InstrumentSpec, mc_config
-    m = model()
+    m = model_bates_kou() if bates_kou else model()
+    scheme = capi.SCHEME_NV if bates_kou else capi.SCHEME_NCI
+    mode = capi.GREEK_FULL if mode is None else mode
     out = []
     for T in MATURITIES:
         inst = InstrumentSpec(strikes=STRIKES, maturity=T)
-        cfg = mc_config(paths, int(round(100 * T)), seed=42, precision=precision,
+        cfg = mc_config(paths, int(round(100 * T)), seed=seed, precision=precision,
                         stream_layout=layout, num_streams=streams)
-        out.append((m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg, S0, R, Q))
+        out.append((m, [scheme], inst, mode, cfg, S0, R, Q))
     return out
+
+
+def model_bates_kou():
+    """Bates-Kou with SchemeNV (sigma^2 <= 4 kappa theta): Heston kappa 2,
+    theta 0.04, sigma 0.5, rho -0.7, v0 0.04; Kou lambda 1, p 0.4, eta1 10,
+    eta2 5 -- the learned control variate's second workload."""
+    from myquantlib_amd.pricing import Heston, kou, make_model
+    jk, jp = kou(1.0, 0.4, 10.0, 5.0)
+    return make_model(1, [Heston(2.0, 0.04, 0.5, -0.7, 0.04)], jump_kind=jk, jump=jp)
+
+
+def cv_leg(eng, eval_paths: int, bates_kou: bool):
+    """Price-only plain FP32 and learned-CV FP32 on the strip's 512 options:
+    ms to 1e-4 relative SE for each, the CV's time including its training
+    (2^20 training paths per maturity, seed 7, 2,000 Adam steps of 8,192
+    rows, lr 1e-3, 8 control points; evaluation seed 42)."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import cv_config
+    J = jobs(eval_paths, capi.FP32, mode=capi.GREEK_NONE, bates_kou=bates_kou)
+    eng.mc_batch(J)
+    t0 = time.perf_counter()
+    plain = eng.mc_batch(J)
+    tp = 1e3 * (time.perf_counter() - t0)
+    cfg = cv_config(train_paths=1 << 20, train_seed=7, steps=2000, batch=8192, lr=1e-3, points=8)
+    t0 = time.perf_counter()
+    net = eng.cv_train(jobs(1024, capi.FP32, mode=capi.GREEK_NONE, seed=7, bates_kou=bates_kou), cfg)
+    tt = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    res = eng.mc_batch_cv(J, cfg, net)
+    tc = 1e3 * (time.perf_counter() - t0)
+    mp, wp = ms_to_target(plain, tp)
+    mc, wc = ms_to_target(res, tc + tt)
+    vr = np.concatenate([r["var_ratio"] for r in res])
+    used = int(sum(int(np.sum(r["used_cv"])) for r in res))
+    print(f"# cv leg ({'bates-kou NV' if bates_kou else 'bates NCI strip'}): plain {tp:.0f} ms, "
+          f"cv {tc:.0f} ms + train {tt:.0f} ms, median variance ratio {np.median(vr):.2f}",
+          file=sys.stderr, flush=True)
+    return {"eval_paths_per_maturity": eval_paths, "plain_fp32_ms": tp, "plain_fp32_ms_to_1e-4": mp,
+            "plain_worst_rel_se": wp, "learned_cv_fp32_ms": tc, "train_ms": tt,
+            "learned_cv_fp32_ms_to_1e-4": mc, "cv_worst_rel_se": wc,
+            "variance_ratio_median": float(np.median(vr)), "variance_ratio_max": float(np.max(vr)),
+            "options_using_cv": used, "options": int(vr.size), "train_loss": net.final_loss}
 
 
 def path_steps(paths: int) -> int:
@@ -147,6 +197,7 @@ def main():
     ap.add_argument("--paths-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cv", action="store_true", help="skip the price-only and control-variate legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -193,6 +244,25 @@ def main():
         ms64, worst64 = ms_to_target(r64, w64)
         print(f"# fp64 strip: {w64:.1f} ms, worst rel se {worst64:.2e}", file=sys.stderr, flush=True)
 
+    # price-only strip (GreekMode None): plain FP32 / FP64 and the learned CV
+    cv = {}
+    po32 = po64 = None
+    if not args.no_cv:
+        Jn = jobs(paths, capi.FP32, mode=capi.GREEK_NONE)
+        eng.mc_batch(Jn)
+        t0 = time.perf_counter()
+        rn = eng.mc_batch(Jn)
+        po32, _ = ms_to_target(rn, 1e3 * (time.perf_counter() - t0))
+        if not args.no_fp64:
+            Jn64 = jobs(paths, capi.FP64, mode=capi.GREEK_NONE)
+            eng.mc_batch(Jn64)
+            t0 = time.perf_counter()
+            rn64 = eng.mc_batch(Jn64)
+            po64, _ = ms_to_target(rn64, 1e3 * (time.perf_counter() - t0))
+        cv_paths = min(paths, 1 << 22)
+        cv["bates_strip"] = cv_leg(eng, cv_paths, bates_kou=False)
+        cv["bates_kou_nv_strip"] = cv_leg(eng, cv_paths, bates_kou=True)
+
     kern_s = np.mean(dev_ms) / 1e3
     achieved = total * FLOP_PER_PATH_STEP / kern_s / 1e12
     out = {
@@ -215,7 +285,13 @@ def main():
             "paths_per_maturity": paths,
             "path_steps_per_step": total,
         },
-        "ms_to_1e-4_rel_se": {"plain_fp32": t32, "plain_fp64": ms64, "learned_cv_fp32": None},
+        # price-only runs of the strip's 512 options (GreekMode None); the CV
+        # time includes its training; "full_greeks_*" are the Full-mode runs
+        "ms_to_1e-4_rel_se": {
+            "plain_fp32": po32, "plain_fp64": po64,
+            "learned_cv_fp32": cv["bates_strip"]["learned_cv_fp32_ms_to_1e-4"] if cv else None,
+            "full_greeks_fp32": t32, "full_greeks_fp64": ms64},
+        "learned_cv": cv or None,
         "sample_steps_per_s": value / VARIANTS,
         "executed_variant_fraction": EXECUTED_VARIANTS / VARIANTS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,

@@ -128,6 +128,8 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
     R xs[1] = {x};
     euro_jump<R, 1>(J, PP, xs, g);
     set.ST[p] = static_cast<double>(dexp<R>(xs[0]));
+    // the compensated jump factor of the European's single terminal jump
+    if (set.dJ) set.dJ[p] = static_cast<float>(expm1(static_cast<double>(xs[0]) - static_cast<double>(x)));
   }
 }
 
@@ -288,6 +290,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     for (uint64_t p = wave; p < M; p += n_waves) {
       double cv = 0.0;
+      float ux_last = 0.f;
       for (int j = 0; j < set.nj; ++j) {
         const float x = set.rec[(j * CV_REC + 0) * M + p];
         const float v1 = set.rec[(j * CV_REC + 1) * M + p];
@@ -320,7 +323,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
         cv_grad(L, f, g1, g2, g3, lane);
         const float ux = nm.S0 * nm.xs * g1, u1 = nm.S0 * nm.vs0 * g2, u2 = nm.S0 * nm.vs1 * g3;
         cv += set.disc_of[j] * static_cast<double>(ux * ex + u1 * e1 + u2 * e2);
+        ux_last = ux;
       }
+      // terminal jump: u_x dS/S at the last control point times the
+      // compensated jump factor (independent of the path, mean 0)
+      if (set.dJ) cv += set.disc_of[set.nj - 1] * static_cast<double>(ux_last * set.dJ[p]);
       if (live) {
         const double P = cv_payoff(set.payoff, set.call, set.ST[p], K);
         const double Pa = P - cv;

@@ -27,6 +27,7 @@ struct CvSet {
   // dW_v2 at control point j; ST[p] terminal spot
   float *rec;
   double *ST;
+  float *dJ;                    // e^{jump total} - 1 of the terminal jump (mean 0), or null
   uint64_t M;
   int nj;                       // control points
   int step_of[CV_MAXJ + 1];     // first step of each block, step_of[nj] = steps

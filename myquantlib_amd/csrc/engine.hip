@@ -1147,7 +1147,7 @@ CvNorm cv_norm(const myql_mc_job *jobs, int64_t n) {
 // Owner of one recorded option set.
 struct CvRecord {
   CvSet set{};
-  DevBuf rec, st, strikes, job;
+  DevBuf rec, st, dj, strikes, job;
 };
 
 // Record `paths` base paths of job `jb` (seed, stream layout and precision
@@ -1204,6 +1204,11 @@ void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, in
   out.st.reserve(M * sizeof(double));
   S.rec = out.rec.as<float>();
   S.ST = out.st.as<double>();
+  S.dJ = nullptr;
+  if (m.jump_kind != MYQL_JUMP_NONE) {
+    out.dj.reserve(M * sizeof(float));
+    S.dJ = out.dj.as<float>();
+  }
   HIPCHECK(launch_cv_paths<R>(nf, out.job.as<Job<R>>(), S, J.S, c->stream));
 }
 
