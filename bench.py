@@ -189,6 +189,40 @@ def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def mlmc_leg(eng):
+    """Multilevel MC of an up-and-out call under Heston (kappa 2, theta 0.04,
+    sigma 0.3, rho -0.7, v0 0.04; SchemeNV), K 100, B 130, T 1: levels 0..6
+    with 4 * 2^l fine steps, epsilon = 1e-4 x the price of a plain pilot run,
+    Giles' sample sizes, FP32 paths, FP64 level sums, seed 42; each level a
+    replayed HIP graph."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import Heston, InstrumentSpec, make_model, mc_config, mlmc_config
+    m = make_model(1, [Heston(2.0, 0.04, 0.3, -0.7, 0.04)])
+    inst = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=130.0, maturity=1.0,
+                          barrier_dir=capi.BARRIER_UP, barrier_action=capi.BARRIER_OUT)
+    pilot = eng.mc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE,
+                   mc_config(1 << 20, 256, seed=1, precision=capi.FP32), S0, R, Q)
+    eps = 1e-4 * float(pilot["price"][0])
+    cfg = mlmc_config(epsilon=eps, L_min=6, L_max=6, N0=10000, base_steps=4, seed=42,
+                      precision=capi.FP32)
+    eng.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE,
+             mlmc_config(epsilon=100 * eps, L_min=6, L_max=6, N0=1000, base_steps=4, seed=3,
+                         precision=capi.FP32), S0, R, Q)  # warm-up: tables, code objects
+    t0 = time.perf_counter()
+    res = eng.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE, cfg, S0, R, Q)
+    wall = 1e3 * (time.perf_counter() - t0)
+    print(f"# mlmc up-and-out call: {wall:.0f} ms, price {res['price']:.6f} +- "
+          f"{res['price_std_err']:.2e}, {res['total_steps']:.3e} path-steps", file=sys.stderr,
+          flush=True)
+    return {"workload": "Heston NV up-and-out call K 100 B 130 T 1, levels 0..6, base 4 steps, "
+                        "eps 1e-4 x pilot price, FP32 paths, FP64 sums, seed 42",
+            "epsilon": eps, "price": res["price"], "price_std_err": res["price_std_err"],
+            "ms": wall, "device_ms": res["device_ms"], "total_path_steps": res["total_steps"],
+            "path_steps_per_s": res["total_steps"] / (wall / 1e3),
+            "paths_per_level": [int(x) for x in res["paths_per_level"]],
+            "alpha": res["alpha_estimated"], "beta": res["beta_estimated"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +232,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the price-only and control-variate legs")
+    ap.add_argument("--no-mlmc", action="store_true", help="skip the multilevel leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -263,6 +298,8 @@ def main():
         cv["bates_strip"] = cv_leg(eng, cv_paths, bates_kou=False)
         cv["bates_kou_nv_strip"] = cv_leg(eng, cv_paths, bates_kou=True)
 
+    mlmc = None if args.no_mlmc else mlmc_leg(eng)
+
     kern_s = np.mean(dev_ms) / 1e3
     achieved = total * FLOP_PER_PATH_STEP / kern_s / 1e12
     out = {
@@ -292,6 +329,7 @@ def main():
             "learned_cv_fp32": cv["bates_strip"]["learned_cv_fp32_ms_to_1e-4"] if cv else None,
             "full_greeks_fp32": t32, "full_greeks_fp64": ms64},
         "learned_cv": cv or None,
+        "mlmc": mlmc,
         "sample_steps_per_s": value / VARIANTS,
         "executed_variant_fraction": EXECUTED_VARIANTS / VARIANTS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,

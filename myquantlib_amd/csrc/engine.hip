@@ -300,7 +300,7 @@ struct myql_ctx {
   std::map<std::tuple<uint64_t, int, uint64_t>, std::unique_ptr<DevBuf>> streams;
   std::map<std::tuple<uint64_t, int, int, int>, std::unique_ptr<DevBuf>> nci;  // (d bits, rows, prec, n_fb)
   DevBuf jobs, units, partial, sums, meta, counter, tmp, accg;
-  DevBuf mlmc_streams, mlmc_sums, mlmc_job;
+  DevBuf mlmc_streams, mlmc_job;
 };
 
 namespace {
@@ -921,6 +921,29 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
 // MLMC: Giles' adaptive driver on the host (MultiLevelMonteCarloPricer.hpp
 // :46-241) over level kernels that keep every stream's state on the device.
 // ---------------------------------------------------------------------------
+// Pinned host memory (hipHostMalloc) of the MLMC graphs' copy nodes.
+struct HostBuf {
+  void *p = nullptr;
+  void reserve(size_t bytes) { HIPCHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault)); }
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  template <typename T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// One level's launch sequence -- dN upload, level kernel, block-sum download
+// -- captured once as a HIP graph and replayed on every visit of the level
+// by the Giles loop (the per-visit path count goes through pinned memory).
+struct LevelGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~LevelGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
 template <typename R>
 struct MlmcRun {
   myql_ctx *c;
@@ -932,8 +955,27 @@ struct MlmcRun {
   double S0, r, q;
   uint64_t S;
   double device_ms = 0.0;
+  unsigned nb = 0;        // blocks of a level launch
+  size_t job_stride = 0;  // bytes per level in c->mlmc_job: Job<R> + strike
+  HostBuf h_dn, h_sums;
+  DevBuf d_dn, d_sums;
+  std::vector<std::unique_ptr<LevelGraph>> graphs;
 
-  void level(size_t l, uint64_t dN, double &sum, double &sq) {
+  void prepare() {
+    nb = static_cast<unsigned>((S + BLOCK - 1) / BLOCK);
+    const size_t L = cfg.L_max + 1;
+    job_stride = (sizeof(Job<R>) + 255) / 256 * 256 + 256;
+    c->mlmc_job.reserve(L * job_stride);
+    h_dn.reserve(L * sizeof(uint64_t));
+    h_sums.reserve(L * 2 * nb * sizeof(double));
+    d_dn.reserve(L * sizeof(uint64_t));
+    d_sums.reserve(L * 2 * nb * sizeof(double));
+    graphs.resize(L);
+  }
+
+  // Job of level l (fine/coarse steppers) in its own device slot, and the
+  // level's captured launch sequence.
+  void build_level(size_t l) {
     Job<R> J;
     fill_instrument(J, m, schemes, in, mode, S0, r, cfg.fd_bump, false);
     const bool pd = in.kind != MYQL_INSTR_EUROPEAN;
@@ -959,39 +1001,58 @@ struct MlmcRun {
     J.var_list[1] = 1;
     J.n_groups = 1;
     J.group_size[0] = 1;
-    J.M = dN;
+    J.M = 0;
     J.S = S;
     J.steps = steps;
     J.nk = 1;
     J.nkp = 1;
-    const size_t jb = (sizeof(Job<R>) + 255) / 256 * 256;
-    c->mlmc_job.reserve(jb + 64);
-    double *d_k = reinterpret_cast<double *>(static_cast<char *>(c->mlmc_job.p) + jb);
+    char *slot = static_cast<char *>(c->mlmc_job.p) + l * job_stride;
+    double *d_k = reinterpret_cast<double *>(slot + (sizeof(Job<R>) + 255) / 256 * 256);
     J.strikes = d_k;
-    HIPCHECK(hipMemcpyAsync(c->mlmc_job.p, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(slot, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(d_k, in.strikes, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    const unsigned nb = static_cast<unsigned>((S + BLOCK - 1) / BLOCK);
     const size_t smem = static_cast<size_t>(J.n_fields) * MAXV * BLOCK * sizeof(R);
     if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
-    HIPCHECK(hipEventRecord(c->ev0, c->stream));
-    const Job<R> *dj = c->mlmc_job.as<Job<R>>();
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    const Job<R> *dj = reinterpret_cast<const Job<R> *>(slot);
     uint64_t *st = c->mlmc_streams.as<uint64_t>();
-    double *bs = c->mlmc_sums.as<double>();
+    double *bs = d_sums.as<double>() + l * 2 * nb;
+    const uint64_t *dn = d_dn.as<uint64_t>() + l;
     const int cp = coupled ? 1 : 0;
-    if (nf == 0)
-      hipLaunchKernelGGL((mlmc_level_kernel<R, 0>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dN, cp, bs);
-    else if (nf == 1)
-      hipLaunchKernelGGL((mlmc_level_kernel<R, 1>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dN, cp, bs);
-    else
-      hipLaunchKernelGGL((mlmc_level_kernel<R, 2>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dN, cp, bs);
-    HIPCHECK(hipGetLastError());
+    auto &G = graphs[l];
+    G = std::make_unique<LevelGraph>();
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(d_dn.as<uint64_t>() + l, h_dn.as<uint64_t>() + l, sizeof(uint64_t),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+      if (nf == 0)
+        hipLaunchKernelGGL((mlmc_level_kernel<R, 0>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs);
+      else if (nf == 1)
+        hipLaunchKernelGGL((mlmc_level_kernel<R, 1>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs);
+      else
+        hipLaunchKernelGGL((mlmc_level_kernel<R, 2>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_sums.as<double>() + l * 2 * nb, bs, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &G->g);
+    HIPCHECK(e);
+    HIPCHECK(ce);
+    HIPCHECK(hipGraphInstantiate(&G->exec, G->g, nullptr, nullptr, 0));
+  }
+
+  void level(size_t l, uint64_t dN, double &sum, double &sq) {
+    if (!graphs[l]) build_level(l);
+    h_dn.as<uint64_t>()[l] = dN;
+    HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    HIPCHECK(hipGraphLaunch(graphs[l]->exec, c->stream));
     HIPCHECK(hipEventRecord(c->ev1, c->stream));
-    std::vector<double> h(2 * nb);
-    HIPCHECK(hipMemcpyAsync(h.data(), bs, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     device_ms += ms;
+    const double *h = h_sums.as<double>() + l * 2 * nb;
     for (unsigned b = 0; b < nb; ++b) {
       sum += h[2 * b];
       sq += h[2 * b + 1];
@@ -1003,7 +1064,7 @@ struct MlmcRun {
     const uint64_t *init = stream_states(c, cfg.seed, cfg.stream_layout, S);
     c->mlmc_streams.reserve(S * 32);
     HIPCHECK(hipMemcpyAsync(c->mlmc_streams.p, init, S * 32, hipMemcpyDeviceToDevice, c->stream));
-    c->mlmc_sums.reserve(2 * ((S + BLOCK - 1) / BLOCK) * sizeof(double) + 64);
+    prepare();
     const size_t Lmax = cfg.L_max;
     size_t L = cfg.L_min;
     std::vector<uint64_t> N(Lmax + 1, 0);

@@ -162,13 +162,15 @@ __device__ void sim_coupled(const Job<R> *__restrict__ J, Rng &g, R *__restrict_
 // Price dN new paths of one level; per-block FP64 (sum, sum of squares).
 template <typename R, int NF>
 __global__ void __launch_bounds__(BLOCK) mlmc_level_kernel(const Job<R> *__restrict__ J,
-                                                           uint64_t *__restrict__ streams, uint64_t dN,
-                                                           int coupled, double *__restrict__ block_sums) {
+                                                           uint64_t *__restrict__ streams,
+                                                           const uint64_t *__restrict__ dNp, int coupled,
+                                                           double *__restrict__ block_sums) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double s_red[2 * (BLOCK / 64)];
   R *stage = reinterpret_cast<R *>(smem);
   const int tid = threadIdx.x;
   const uint64_t S = J->S;
+  const uint64_t dN = *dNp;  // new paths of this level (device memory: the launch is a replayed graph)
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * BLOCK + tid;
   double sy = 0.0, sy2 = 0.0;
   if (s < S) {
