@@ -189,6 +189,98 @@ def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def _xoshiro_uniforms(seed: int):
+    """uniform(0,1) draws of xoshiro256++ seeded by splitmix64 (the engine's
+    generator), for the parameter sets of the batched workload."""
+    M64 = (1 << 64) - 1
+    x, st = seed, []
+    for _ in range(4):
+        x = (x + 0x9E3779B97F4A7C15) & M64
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+        st.append(z ^ (z >> 31))
+
+    def rotl(v, k):
+        return ((v << k) | (v >> (64 - k))) & M64
+
+    while True:
+        out = (rotl((st[0] + st[3]) & M64, 23) + st[0]) & M64
+        t = (st[1] << 17) & M64
+        st[2] ^= st[0]
+        st[3] ^= st[1]
+        st[1] ^= st[2]
+        st[0] ^= st[3]
+        st[2] ^= t
+        st[3] = rotl(st[3], 45)
+        u = out * 2.0 ** -64
+        if u < 1.0:
+            yield u
+
+
+def double_heston_sets(n_sets: int = 1024, seed: int = 2024, log2_paths=(14, 20)):
+    """Parameter sets of the batched workload, drawn in this order per set:
+    for factor 1 then factor 2: kappa ~ U[0.5, 5], theta ~ U[0.01, 0.09],
+    sigma ~ U[0.1, 1], rho ~ U[-0.9, 0], v0 ~ U[0.01, 0.09]; then the path
+    count 2^u, u ~ U[14, 20] (rounded). Scheme per factor: SchemeNV where
+    sigma^2 <= 4 kappa theta, else SchemeExact."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import Heston
+    g = _xoshiro_uniforms(seed)
+    out = []
+    for _ in range(n_sets):
+        hs, schemes = [], []
+        for _f in range(2):
+            k = 0.5 + 4.5 * next(g)
+            th = 0.01 + 0.08 * next(g)
+            sg = 0.1 + 0.9 * next(g)
+            rh = -0.9 * next(g)
+            v0 = 0.01 + 0.08 * next(g)
+            hs.append(Heston(k, th, sg, rh, v0))
+            schemes.append(capi.SCHEME_NV if sg * sg <= 4.0 * k * th else capi.SCHEME_EXACT)
+        lo, hi = log2_paths
+        paths = int(round(2.0 ** (lo + (hi - lo) * next(g))))
+        out.append((hs, schemes, paths))
+    return out
+
+
+def batched_leg(eng, n_sets: int = 1024, steps: int = 252):
+    """1,024 Double-Heston parameter sets, each pricing an arithmetic Asian
+    call (K 100) and a down-and-out call (K 100, B 80), T 1, 252 steps, price
+    only, FP32 paths; the 2,048 pricing calls go to the device as one batch
+    (one persistent launch per kernel instantiation). All sets use seed 42,
+    so a set's two instruments see identical draws, as two reference calls
+    with the same seed would."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec, make_model, mc_config
+    sets = double_heston_sets(n_sets)
+    asian = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=1.0)
+    dao = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0,
+                         barrier_dir=capi.BARRIER_DOWN, barrier_action=capi.BARRIER_OUT)
+    J = []
+    for hs, schemes, paths in sets:
+        m = make_model(2, hs)
+        # 16,384 streams per call (paths >= 2^14): one shared stream table
+        cfg = mc_config(paths, steps, seed=42, precision=capi.FP32, num_streams=1 << 14)
+        J.append((m, schemes, asian, capi.GREEK_NONE, cfg, S0, R, Q))
+        J.append((m, schemes, dao, capi.GREEK_NONE, cfg, S0, R, Q))
+    eng.mc_batch(J[:64])  # warm-up: code objects, stream states
+    t0 = time.perf_counter()
+    res = eng.mc_batch(J)
+    wall = 1e3 * (time.perf_counter() - t0)
+    ps = sum(2 * paths * steps for _h, _s, paths in sets)
+    n_exact = sum(s.count(capi.SCHEME_EXACT) for _h, s, _p in sets)
+    print(f"# batched double heston: {len(J)} calls, {wall:.0f} ms, {ps:.3e} path-steps",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{n_sets} Double-Heston sets x (arithmetic Asian call K 100, down-and-out "
+                        f"call K 100 B 80), T 1, {steps} steps, 2^U[14,20] paths per set, price only, "
+                        "FP32 paths, one batch",
+            "calls": len(J), "paths": int(sum(p for _h, _s, p in sets)), "path_steps": ps,
+            "ms": wall, "device_ms": res[0]["device_ms"], "path_steps_per_s": ps / (wall / 1e3),
+            "factors_exact": n_exact, "factors_nv": 2 * n_sets - n_exact,
+            "price_range": [float(min(r["price"][0] for r in res)), float(max(r["price"][0] for r in res))]}
+
+
 def mlmc_leg(eng):
     """Multilevel MC of an up-and-out call under Heston (kappa 2, theta 0.04,
     sigma 0.3, rho -0.7, v0 0.04; SchemeNV), K 100, B 130, T 1: levels 0..6
@@ -233,6 +325,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the price-only and control-variate legs")
     ap.add_argument("--no-mlmc", action="store_true", help="skip the multilevel leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched Double-Heston leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -299,6 +392,7 @@ def main():
         cv["bates_kou_nv_strip"] = cv_leg(eng, cv_paths, bates_kou=True)
 
     mlmc = None if args.no_mlmc else mlmc_leg(eng)
+    batched = None if args.no_batched else batched_leg(eng)
 
     kern_s = np.mean(dev_ms) / 1e3
     achieved = total * FLOP_PER_PATH_STEP / kern_s / 1e12
@@ -330,6 +424,7 @@ def main():
             "full_greeks_fp32": t32, "full_greeks_fp64": ms64},
         "learned_cv": cv or None,
         "mlmc": mlmc,
+        "batched_double_heston": batched,
         "sample_steps_per_s": value / VARIANTS,
         "executed_variant_fraction": EXECUTED_VARIANTS / VARIANTS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,

@@ -305,14 +305,20 @@ struct myql_ctx {
 
 namespace {
 
-void set_device(myql_ctx *c) { HIPCHECK(hipSetDevice(c->device)); }
+// Called at the start of every pricing entry point: the stream-state and
+// NCI-table caches are bounded between calls only, since the jobs of one
+// call keep pointers into them until it returns.
+void set_device(myql_ctx *c) {
+  HIPCHECK(hipSetDevice(c->device));
+  if (c->streams.size() > 16) c->streams.clear();
+  if (c->nci.size() > 32) c->nci.clear();
+}
 
 // S stream states for (seed, layout), cached per context.
 const uint64_t *stream_states(myql_ctx *c, uint64_t seed, int layout, uint64_t S) {
   auto key = std::make_tuple(seed, layout, S);
   auto it = c->streams.find(key);
   if (it != c->streams.end()) return it->second->as<uint64_t>();
-  if (c->streams.size() > 16) c->streams.clear();
   auto buf = std::make_unique<DevBuf>();
   buf->reserve(S * 32);
   const int tb = 256;
@@ -375,7 +381,6 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   const auto key = std::make_tuple(dbits, rows, prec, n_fb);
   auto it = c->nci.find(key);
   if (it != c->nci.end()) return info(*it->second);
-  if (c->nci.size() > 32) c->nci.clear();
   const int64_t n = static_cast<int64_t>(rows) * NCI_GRID;
   c->tmp.reserve(n * sizeof(double));
   auto buf = std::make_unique<DevBuf>();

@@ -80,8 +80,12 @@ class InstrumentSpec:
     _s: Instrument = field(default=None, repr=False)
 
     def struct(self) -> Instrument:
-        self._k = np.ascontiguousarray(np.asarray(self.strikes, dtype=np.float64).reshape(-1))
+        """A C struct for this instrument. Each struct keeps its own strike
+        array alive, so one spec can back several jobs of a batch."""
+        k = np.ascontiguousarray(np.asarray(self.strikes, dtype=np.float64).reshape(-1))
+        self._k = k
         s = Instrument()
+        s._strikes_ref = k
         s.kind, s.payoff, s.option_type = self.kind, self.payoff, self.option_type
         s.fixed_strike = int(self.fixed_strike)
         s.barrier_dir, s.barrier_action = self.barrier_dir, self.barrier_action
@@ -92,8 +96,8 @@ class InstrumentSpec:
         s.rebate_rate, s.payoff_eps = self.rebate_rate, self.payoff_eps
         s.past_avg_log, s.past_avg_S, s.t_elapsed = self.past_avg_log, self.past_avg_S, self.t_elapsed
         s.hist_min_S, s.hist_max_S = self.hist_min_S, self.hist_max_S
-        s.strikes = self._k.ctypes.data_as(C.POINTER(C.c_double))
-        s.n_strikes = len(self._k)
+        s.strikes = k.ctypes.data_as(C.POINTER(C.c_double))
+        s.n_strikes = len(k)
         self._s = s
         return s
 

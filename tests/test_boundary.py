@@ -48,6 +48,9 @@ STRUCTS = {
     "myql_mc_job": capi.MCJob,
     "myql_mlmc_config": capi.MLMCConfig,
     "myql_mlmc_result": capi.MLMCResult,
+    "myql_cv_config": capi.CVConfig,
+    "myql_cv_net": capi.CVNet,
+    "myql_cv_result": capi.CVResult,
 }
 
 
@@ -126,3 +129,20 @@ def test_cpp_type_mismatches_fail_to_compile(tmp_path, snippet, needle):
                           str(src)], capture_output=True, text=True)
     assert out.returncode != 0
     assert needle in out.stderr
+
+
+def test_instrument_structs_keep_their_strikes_alive():
+    """One InstrumentSpec backing several jobs of a batch: every struct it
+    returned must keep pointing at live strikes."""
+    import gc
+
+    from myquantlib_amd.pricing import InstrumentSpec
+    spec = InstrumentSpec(strikes=[90.0, 100.0, 110.0])
+    structs = [spec.struct() for _ in range(4)]
+    spec.strikes = [1.0, 2.0, 3.0]
+    spec.struct()
+    gc.collect()
+    junk = [bytearray(1 << 16) for _ in range(64)]  # reuse freed memory if any
+    for s in structs:
+        assert [s.strikes[i] for i in range(3)] == [90.0, 100.0, 110.0]
+    del junk

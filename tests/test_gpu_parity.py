@@ -228,3 +228,30 @@ def test_mlmc_graph_replays_are_deterministic(engine):
         assert a["n_levels"] == b["n_levels"]
         assert np.array_equal(a["paths_per_level"], b["paths_per_level"])
         assert a["price"] == b["price"] and a["price_std_err"] == b["price_std_err"]
+
+
+def test_batched_double_heston_sets(engine, oracle):
+    """Parameter sets of the batched Double-Heston workload (NV or Exact per
+    factor by the Feller-type test), an arithmetic Asian and a down-and-out
+    call per set on identical draws: one batch equals the single calls bit
+    for bit, and FP64 results equal the oracle on identical draws."""
+    import bench
+    sets = bench.double_heston_sets(6, log2_paths=(10, 12))
+    asian = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=1.0)
+    dao = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0,
+                         barrier_dir=capi.BARRIER_DOWN, barrier_action=capi.BARRIER_OUT)
+    jobs = []
+    for hs, schemes, paths in sets:
+        m = make_model(2, hs)
+        cfg = mc_config(paths, 32, seed=42, precision=capi.FP64, num_streams=256)
+        jobs += [(m, schemes, asian, capi.GREEK_NONE, cfg, 100.0, 0.02, 0.0),
+                 (m, schemes, dao, capi.GREEK_NONE, cfg, 100.0, 0.02, 0.0)]
+    assert {s for _h, sch, _p in sets for s in sch} == {capi.SCHEME_NV, capi.SCHEME_EXACT}
+    batch = engine.mc_batch(jobs)
+    for j, b in zip(jobs, batch):
+        single = engine.mc(*j)
+        assert np.array_equal(single["price"], b["price"])
+        assert np.array_equal(single["price_std_err"], b["price_std_err"])
+    for j, b in list(zip(jobs, batch))[:4]:
+        c = oracle.mc(*j)
+        assert rel_err(b["price"], c["price"]) < 1e-9
