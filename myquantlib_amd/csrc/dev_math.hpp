@@ -180,12 +180,36 @@ __device__ __forceinline__ double giles_w(double U) {
   return U > V ? -W : W;
 }
 
+// Giles' central asymptotic estimate (lam > 4, |W| < 3): continuous
+// quantile S and its error bound Del. Shared by the full inversion and the
+// lockstep variants' shortcut so both take identical arithmetic.
+__device__ __forceinline__ void giles_asym_S(double lam, double ilam, double W, double &S, double &Del) {
+  const double sl = sqrt(lam);
+  S = sl * W + (1.0 / 3.0 + (1.0 / 6.0) * W * W) * (1.0 - W / (12.0 * sl));
+  Del = 1.0 / 160.0;
+  Del = 1.0 / 80.0 + Del * (W * W);
+  Del = 1.0 / 40.0 + Del * (W * W);
+  Del = Del * ilam;
+  S = lam + (S + Del);
+}
+
+// Distances of U from the CDF values around the returned count N, from the
+// bottom-up summation: up = F(N) - U >= 0, lo = U - F(N-1) > 0 (lo = U for
+// N = 0), and the probabilities p(N), p(N-1). valid = false when the count
+// came from elsewhere (asymptotic branch, rounding-ambiguous correction).
+struct PoisMargin {
+  bool valid;
+  double up, lo, p_hi, p_lo;
+};
+
 // DEFER: instead of running the rare branches (the tail Newton iteration for
 // |W| >= 3 and the Temme / Stirling correction near an integer boundary), set
 // *rare and return early; the caller then runs the full inversion for those
 // cases (poissinv), so the result is the same.
 template <bool DEFER = false>
-__device__ __forceinline__ double poissinv_w(double lam, double U, double W, bool *rare = nullptr) {
+__device__ __forceinline__ double poissinv_w(double lam, double U, double W, bool *rare = nullptr,
+                                             PoisMargin *mg = nullptr) {
+  if (mg) mg->valid = false;
   if (U <= 0.0) return 0.0;
   if (U >= 1.0) return __builtin_nan("");
   const double V = 1.0 - U;
@@ -194,13 +218,7 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
   if (lam > 4.0) {
     double S;
     if (fabs(W) < 3.0) {
-      const double sl = sqrt(lam);
-      S = sl * W + (1.0 / 3.0 + (1.0 / 6.0) * W * W) * (1.0 - W / (12.0 * sl));
-      Del = 1.0 / 160.0;
-      Del = 1.0 / 80.0 + Del * (W * W);
-      Del = 1.0 / 40.0 + Del * (W * W);
-      Del = Del * ilam;
-      S = lam + (S + Del);
+      giles_asym_S(lam, ilam, W, S, Del);
     } else {
       if constexpr (DEFER) {
         *rare = true;
@@ -298,12 +316,30 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
     double Tt = exp(0.5 * lam);
     Del = 0.0;
     if (U > 0.5) Del = Tt * (1e-13 * Tt);
+    const double T0 = Tt;
     double Sm = 1.0 - Tt * (U * Tt) + Del;
+    // Sm_n = (n!/lam^n) e^lam (F(n) - U): Pr tracks n!/lam^n
+    double Smp = Sm, Pr = 1.0;
     while (Sm < 0.0) {
       X += 1.0;
       Tt = X * ilam;
       Del = Tt * Del;
+      Smp = Sm;
+      Pr *= Tt;
       Sm = Tt * Sm + 1.0;
+    }
+    if (mg && !(Sm < 2.0 * Del)) {
+      const double ph = 1.0 / (Pr * T0 * T0);  // p(N)
+      mg->valid = true;
+      mg->p_hi = ph;
+      mg->up = Sm * ph;
+      if (X > 0.0) {
+        mg->p_lo = ph * X * ilam;  // p(N-1)
+        mg->lo = -Smp * mg->p_lo;
+      } else {
+        mg->p_lo = 0.0;
+        mg->lo = U;
+      }
     }
     if (Sm < 2.0 * Del) {
       Del = 1e13 * Del;
@@ -327,6 +363,20 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
 
 __device__ static __noinline__ double poissinv(double lam, double U) {
   return poissinv_w(lam, U, (lam > 4.0 && U > 0.0 && U < 1.0) ? giles_w(U) : 0.0);
+}
+
+// The count Giles' method returns from its central asymptotic branch alone
+// (lam > 4, |W| < 3, S clear of the correction band, floor(S) >= 10), for
+// 0 < U < 1; false when any other branch would decide.
+__device__ __forceinline__ bool giles_asym_only(double lam, double W, int &N) {
+  if (!(lam > 4.0 && fabs(W) < 3.0)) return false;
+  double S, Del;
+  giles_asym_S(lam, 1.0 / lam, W, S, Del);
+  const double X = floor(S);
+  if (S > 10.0 && S < X + 2.0 * Del) return false;
+  if (X < 10.0) return false;
+  N = static_cast<int>(X);
+  return true;
 }
 
 // Poisson(mean) by sequential inversion on one uniform (Boost's method for

@@ -347,10 +347,17 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     const R h00 = R(2) * t3 - R(3) * t2 + R(1), h10 = t3 - R(2) * t2 + t;
     const R h01 = R(-2) * t3 + R(3) * t2, h11 = t3 - t2;
     const R *__restrict__ tab = static_cast<const R *>(J->nci[f]);
-    // Poisson counts of all slots from the shared uniform u1: Giles' method
-    // per slot, with the normal quantile W of u1 computed once. The slot
-    // loop stays rolled (one inlined copy of the method); the slot's
-    // intensity and count move through select chains, not memory.
+    // Poisson counts of all slots from the shared uniform u1, each equal to
+    // what Giles' method returns for that slot's intensity:
+    //  - slot 0 runs the method (normal quantile W of u1 computed once); its
+    //    bottom-up summation also yields the CDF margins F(N0) - u1 and
+    //    u1 - F(N0 - 1);
+    //  - a bumped slot k differs by d = |lam_k - lam_0|, which moves F(n) by
+    //    at most d p(n; lam) (dF/dlam = -p): when both margins exceed that
+    //    bound its count is N0; when slot 0 took the asymptotic branch, slot k
+    //    reruns that branch alone with the same W;
+    //  - anything else (and the method's rare branches) is finished by a
+    //    full inversion on wave-packed items (giles_rare).
     int Ns[NV];
     if constexpr (NV == 1) {
       Ns[0] = static_cast<int>(poissinv(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1));
@@ -363,20 +370,38 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         big = big || lams[k] > 4.0;
         Ns[k] = 0;
       }
-      const double W = (big && u1 > 0.0) ? giles_w(u1) : 0.0;
-      unsigned rare = 0;
-#pragma unroll 1
-      for (int k = 0; k < NV; ++k) {
-        double lam = lams[0];
+      if (u1 > 0.0) {  // u1 == 0: every count is 0
+        const double W = big ? giles_w(u1) : 0.0;
+        PoisMargin mg;
+        bool r0 = false;
+        const int n0 = static_cast<int>(poissinv_w<true>(lams[0], u1, W, &r0, &mg));
+        unsigned pend = r0 ? 1u : 0u;
+        Ns[0] = n0;
 #pragma unroll
-        for (int j = 1; j < NV; ++j) lam = (k == j) ? lams[j] : lam;
-        bool r = false;
-        const int n = static_cast<int>(poissinv_w<true>(lam, u1, W, &r));
-        rare |= r ? (1u << k) : 0u;
-#pragma unroll
-        for (int j = 0; j < NV; ++j) Ns[j] = (k == j) ? n : Ns[j];
+        for (int k = 1; k < NV; ++k) {
+          const double dl = fabs(lams[k] - lams[0]);
+          bool ok = false;
+          int n = n0;
+          if (mg.valid) {
+            if (dl == 0.0) {
+              ok = true;
+            } else if (lams[0] > 0.0) {
+              // p(n; lam) <= p(n; lam0) e^x on the interval, x = d (1 + n/lam0),
+              // and e^x <= 1 + 2x for x <= 1/2
+              const double x = dl * (1.0 + static_cast<double>(n0) / lams[0]);
+              if (x <= 0.5) {
+                const double g = dl * (1.0 + 2.0 * x);
+                ok = mg.up > g * mg.p_hi + 1e-12 && (n0 == 0 || mg.lo > g * mg.p_lo + 1e-12);
+              }
+            }
+          } else if (!r0) {
+            ok = giles_asym_only(lams[k], W, n);
+          }
+          if (ok) Ns[k] = n;
+          else pend |= 1u << k;
+        }
+        if (__any(pend != 0u)) giles_rare<NV>(lams, u1, pend, Ns);
       }
-      if (__any(rare != 0u)) giles_rare<NV>(lams, u1, rare, Ns);
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
