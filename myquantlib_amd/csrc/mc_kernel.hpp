@@ -620,7 +620,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   {
     double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg = 0, sg2 = 0;
     if (live) {
-#pragma unroll 1
+#pragma unroll 4
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double p, pu = 0, pd = 0;
@@ -651,7 +651,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   for (int f = 0; f < J->nvega; ++f) {
     double sv = 0, sv2 = 0, sn = 0, sn2 = 0;
     if (live) {
-#pragma unroll 1
+#pragma unroll 4
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
@@ -679,7 +679,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   {
     double st = 0, st2 = 0, sc = 0, sc2 = 0;
     if (live) {
-#pragma unroll 1
+#pragma unroll 4
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
@@ -701,7 +701,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   {
     double sr = 0, sr2 = 0;
     if (live) {
-#pragma unroll 1
+#pragma unroll 4
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, bp, dum0, dum1;
