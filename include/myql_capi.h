@@ -22,6 +22,9 @@
  *   myql_create/destroy      (no counterpart: the reference has no device
  *                            state; a context owns a HIP stream, cached
  *                            stream states and NCI tables)
+ *   myql_cv_train,           learned martingale control variate (no
+ *   myql_mc_calculate_batch_cv  counterpart: README.md:332 lists control
+ *                            variates as future work)
  *
  * Ownership: the caller owns every pointer it passes; the library never
  * frees or retains caller memory past the call. Calls are synchronous on the
