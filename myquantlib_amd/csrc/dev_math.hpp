@@ -197,9 +197,13 @@ __device__ __forceinline__ void giles_asym_S(double lam, double ilam, double W, 
 // bottom-up summation: up = F(N) - U >= 0, lo = U - F(N-1) > 0 (lo = U for
 // N = 0), and the probabilities p(N), p(N-1). valid = false when the count
 // came from elsewhere (asymptotic branch, rounding-ambiguous correction).
+// asym = true when the central asymptotic branch alone decided N = floor(S),
+// with S and its error term Del kept.
 struct PoisMargin {
   bool valid;
   double up, lo, p_hi, p_lo;
+  bool asym;
+  double S, Del;
 };
 
 // DEFER: instead of running the rare branches (the tail Newton iteration for
@@ -209,7 +213,10 @@ struct PoisMargin {
 template <bool DEFER = false>
 __device__ __forceinline__ double poissinv_w(double lam, double U, double W, bool *rare = nullptr,
                                              PoisMargin *mg = nullptr) {
-  if (mg) mg->valid = false;
+  if (mg) {
+    mg->valid = false;
+    mg->asym = false;
+  }
   if (U <= 0.0) return 0.0;
   if (U >= 1.0) return __builtin_nan("");
   const double V = 1.0 - U;
@@ -243,6 +250,11 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
       S = S + Del;
     }
     X = floor(S);
+    if (mg && fabs(W) < 3.0 && X >= 10.0 && !(S < X + 2.0 * Del)) {
+      mg->asym = true;
+      mg->S = S;
+      mg->Del = Del;
+    }
     if (S > 10.0 && S < X + 2.0 * Del) {
       if constexpr (DEFER) {
         *rare = true;

@@ -323,6 +323,73 @@ __device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, u
   }
 }
 
+// Count of a bumped slot (intensity lamk) from slot 0's inversion (intensity
+// lam0 = 1/il0, count n0 not deferred, margins mg) and the shared normal
+// quantile W; -1 when undecided, and the slot then takes the full inversion.
+//  - Equal intensities: n0 (same inputs, same arithmetic).
+//  - Slot 0 decided by the summation: dF(n)/dlam = -p(n; lam) < 0, so raising
+//    lam only endangers F(n0) >= U and lowering it only U > F(n0 - 1). With
+//    d = |lamk - lam0| and x = d (1 + n0/lam0) <= 1/2, p(n; lam) <= p(n; lam0)
+//    e^x <= p(n; lam0) (1 + 2x) between the intensities, so a margin above
+//    d (1 + 2x) p keeps n0. Otherwise the margin moves by the integral of
+//    p(n; lam) over the interval, p d (1 +- (n/lam0 - 1) d/2) to second
+//    order, with the Taylor remainder at most d^3/6 max|p''| and
+//    |p''| <= p e^x ((2n/lam0 + 1)^2 + 4n/lam0^2) for lam >= lam0/2; a moved
+//    margin clear of that bound gives n0 or the neighbouring count, whose own
+//    probability is bounded below (p(n0 + 1; lamk) >= p(n0 + 1; lam0)(1 - d)
+//    when rising, p(n0 - 1; lamk) >= p(n0 - 1; lam0)(1 - (n0 - 1) d/lam0)
+//    when falling).
+//  - Slot 0 decided by the central asymptotic branch: there |dS/dlam| < 1.82
+//    (|W| < 3, lam > 4), so the slot's S lies within 2d of S0; when that
+//    interval stays clear of the integers and of the correction band
+//    [X, X + 2 Del) (Del scaling as 1/lam), floor(S) is n0. Else the slot
+//    reruns the branch with the shared W.
+// Giles' method returns the exact quantile except where U lies within
+// rounding of a CDF value; the 1e-12 slack keeps those cases out, so a count
+// decided here equals the method's.
+__device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, double il0, double W,
+                                            const PoisMargin &mg) {
+  const double dl = fabs(lamk - lam0);
+  if (dl == 0.0) return n0;
+  if (mg.valid) {
+    if (!(lam0 > 0.0 && dl <= 0.5 * lam0)) return -1;
+    const double x = dl * (1.0 + static_cast<double>(n0) * il0);
+    if (!(x <= 0.5)) return -1;
+    const double g = dl * (1.0 + 2.0 * x);
+    const bool rise = lamk > lam0;
+    if (rise) {
+      if (!(n0 == 0 || mg.lo > 1e-12)) return -1;
+      if (mg.up > g * mg.p_hi + 1e-12) return n0;
+    } else {
+      if (!(mg.up > 1e-12)) return -1;
+      if (n0 == 0 || mg.lo > g * mg.p_lo + 1e-12) return n0;
+    }
+    const double n = static_cast<double>(rise ? n0 : n0 - 1);
+    const double p0 = rise ? mg.p_hi : mg.p_lo;
+    const double q = 2.0 * n * il0, c = n * il0 - 1.0;
+    const double B = dl * dl * dl * (1.0 / 6.0) * p0 * (1.0 + 2.0 * x) * ((q + 1.0) * (q + 1.0) + 2.0 * q * il0) +
+                     1e-12;
+    if (rise) {
+      const double m = mg.up - p0 * dl * (1.0 + 0.5 * c * dl);  // F(n0; lamk) - U
+      if (m > B) return n0;
+      const double n1 = static_cast<double>(n0 + 1);
+      if (m < -B && mg.p_hi * lam0 * (1.0 - dl) + (m - B) * n1 > 1e-12 * n1) return n0 + 1;
+    } else {
+      const double m = mg.lo - p0 * dl * (1.0 - 0.5 * c * dl);  // U - F(n0 - 1; lamk)
+      if (m > B) return n0;
+      if (m < -B && mg.p_lo * (1.0 - n * dl * il0) + (m - B) > 1e-12) return n0 - 1;
+    }
+    return -1;
+  }
+  if (mg.asym && dl <= 0.5 * lam0 && lam0 - dl > 4.0) {
+    const double f0 = mg.S - static_cast<double>(n0);
+    const double e = 2.0 * dl + 1e-9;
+    if (f0 + e < 1.0 && f0 - e >= 2.0 * mg.Del * (1.0 + 2.0 * dl * il0)) return n0;
+  }
+  int n = n0;
+  return giles_asym_only(lamk, W, n) ? n : -1;
+}
+
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
 template <typename R, int NV, int NFX>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
@@ -352,10 +419,10 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     //  - slot 0 runs the method (normal quantile W of u1 computed once); its
     //    bottom-up summation also yields the CDF margins F(N0) - u1 and
     //    u1 - F(N0 - 1);
-    //  - a bumped slot k differs by d = |lam_k - lam_0|, which moves F(n) by
-    //    at most d p(n; lam) (dF/dlam = -p): when both margins exceed that
-    //    bound its count is N0; when slot 0 took the asymptotic branch, slot k
-    //    reruns that branch alone with the same W;
+    //  - a bumped slot k takes N0 when its intensity equals slot 0's, else
+    //    its count follows from those margins moved by the intensity change,
+    //    or from slot 0's asymptotic estimate, or from rerunning the
+    //    asymptotic branch alone with the same W (bumped_count);
     //  - anything else (and the method's rare branches) is finished by a
     //    full inversion on wave-packed items (giles_rare).
     int Ns[NV];
@@ -377,27 +444,11 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         const int n0 = static_cast<int>(poissinv_w<true>(lams[0], u1, W, &r0, &mg));
         unsigned pend = r0 ? 1u : 0u;
         Ns[0] = n0;
+        const double il0 = 1.0 / lams[0];
 #pragma unroll
         for (int k = 1; k < NV; ++k) {
-          const double dl = fabs(lams[k] - lams[0]);
-          bool ok = false;
-          int n = n0;
-          if (mg.valid) {
-            if (dl == 0.0) {
-              ok = true;
-            } else if (lams[0] > 0.0) {
-              // p(n; lam) <= p(n; lam0) e^x on the interval, x = d (1 + n/lam0),
-              // and e^x <= 1 + 2x for x <= 1/2
-              const double x = dl * (1.0 + static_cast<double>(n0) / lams[0]);
-              if (x <= 0.5) {
-                const double g = dl * (1.0 + 2.0 * x);
-                ok = mg.up > g * mg.p_hi + 1e-12 && (n0 == 0 || mg.lo > g * mg.p_lo + 1e-12);
-              }
-            }
-          } else if (!r0) {
-            ok = giles_asym_only(lams[k], W, n);
-          }
-          if (ok) Ns[k] = n;
+          const int n = r0 ? -1 : bumped_count(lams[0], lams[k], n0, il0, W, mg);
+          if (n >= 0) Ns[k] = n;
           else pend |= 1u << k;
         }
         if (__any(pend != 0u)) giles_rare<NV>(lams, u1, pend, Ns);

@@ -135,6 +135,24 @@ def test_bates_nci_full_strip(engine, oracle, prec, tol, flags):
         compare(g, c, ["price", "delta"], tol)
 
 
+@pytest.mark.parametrize("bump", [1e-3, 3e-2])
+def test_bates_nci_bumped_slot_counts(engine, oracle, bump):
+    """The lockstep kernel gives each bumped Greek slot (v0 +-, T +-) its
+    Poisson count from slot 0's inversion -- CDF margins moved by the
+    intensity change to second order, slot 0's asymptotic estimate, or a
+    rerun of the asymptotic branch -- and only falls back to the full
+    inversion when those leave the count open. The oracle inverts every slot
+    in full, as the reference's replays do. Larger v0 and T bumps send more
+    slots through the second-order and fallback paths; counts must still
+    agree exactly (FP64 results to 1e-9)."""
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
+    inst = InstrumentSpec(strikes=np.linspace(70, 130, 8), maturity=0.25)
+    cfg = mc_config(1 << 14, 100, seed=11, vol_bump=bump, t_bump=bump, num_streams=1024)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL, cfg)
+    compare(g, c, FULL, 1e-9)
+
+
 @pytest.mark.parametrize("scheme", [capi.SCHEME_NV, capi.SCHEME_NCI, capi.SCHEME_EXACT])
 @pytest.mark.parametrize("mode", [capi.GREEK_NONE, capi.GREEK_FULL])
 def test_schemes_fp64(engine, oracle, scheme, mode):
