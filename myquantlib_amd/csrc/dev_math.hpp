@@ -134,7 +134,7 @@ __device__ __forceinline__ float normcdfinv_f(double p) {
   const bool lower = p <= 0.02425;
   // tail argument formed in FP64 so 1 - p keeps its precision near 1
   const float q = static_cast<float>(lower ? p : 1.0 - p);
-  const float r = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(q));
+  const float r = __builtin_amdgcn_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(q));
   const float x =
       (((((float(MYQL_ACKLAM_C0) * r + float(MYQL_ACKLAM_C1)) * r + float(MYQL_ACKLAM_C2)) * r +
          float(MYQL_ACKLAM_C3)) * r + float(MYQL_ACKLAM_C4)) * r + float(MYQL_ACKLAM_C5)) *
@@ -490,6 +490,12 @@ __device__ static __noinline__ double gamma_inv(double a, double p, double q, do
       double den = 1.0 - 0.5 * r * ((a - 1.0) / x - 1.0);
       if (den <= 0.1) den = 1.0;
       xn = x - r / den;
+      // converged: a step this small can round onto the bracket end x
+      // itself, which must not be taken for a step out of the bracket
+      if (fabs(xn - x) <= 1e-14 * xn) {
+        if (xn > lo && xn < hi) x = xn;
+        break;
+      }
     }
     if (!(xn > lo && xn < hi)) {
       if (isinf(hi)) xn = (x > 0.0 ? 2.0 * x : 1.0);

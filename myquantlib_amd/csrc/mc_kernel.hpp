@@ -27,7 +27,7 @@ __device__ __forceinline__ R dsqrt(R x);
 template <>
 __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 template <>
-__device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }  // v_sqrt_f32, 1 ulp
 
 template <typename R>
 __device__ __forceinline__ R dexp(R x);
@@ -454,10 +454,21 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         if (__any(pend != 0u)) giles_rare<NV>(lams, u1, pend, Ns);
       }
     }
+    // the table (chi2 quantiles of d + 2N degrees of freedom) and u2 are
+    // common to all slots, so slots sharing slot 0's count share its
+    // interpolated quantile and differ only in their scale factor
+    const bool main0 = Ns[0] <= P[0]->nmax[f] && u2 < 0.999;
+    R y0 = R(0);
+    if (main0) {
+      const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
+      y0 = h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3];
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const int N = Ns[k];
-      if (N <= P[k]->nmax[f] && u2 < 0.999) {
+      if (main0 && N == Ns[0] && N <= P[k]->nmax[f]) {
+        vn[k][f] = y0 * P[k]->Kr[f][2];
+      } else if (N <= P[k]->nmax[f] && u2 < 0.999) {
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
       } else if (N <= P[k]->nmax[f] && J->nci_tail[f] != nullptr) {

@@ -343,6 +343,13 @@ static inline double gamma_inv(double a, double p, double q) {
       double denom = 1.0 - 0.5 * r * curv;
       if (denom <= 0.1) denom = 1.0;  // fall back to Newton
       xn = x - r / denom;
+      // converged: a step this small can round onto the bracket end x
+      // itself, which must not be taken for a step out of the bracket
+      if (std::fabs(xn - x) <= 1e-14 * xn) {
+        if (xn > lo && xn < hi) x = xn;
+        ORC_OPS(16);
+        break;
+      }
     } else {
       xn = -1.0;
     }

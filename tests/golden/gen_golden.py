@@ -70,6 +70,15 @@ def main():
     pn = np.concatenate([10.0 ** -np.arange(1, 16), np.linspace(0.001, 0.999, 99),
                          1 - 10.0 ** -np.arange(1, 16)])
     g["ndtri"] = {"p": pn.tolist(), "z": special.ndtri(pn).tolist()}
+    # degrees of freedom d + 2N over the whole NCI table (N up to 1023)
+    rt = np.random.default_rng(20261203)
+    dof_t = np.concatenate([[849.0313428193025, 1264.3770645306565, 2046.32], rt.uniform(120.0, 2100.0, 397)])
+    p_t = rt.uniform(0.0, 1.0, dof_t.size)
+    p_t[:3] = [0.7674501184822347, 0.5044304231979618, 0.5]
+    p_t[3:23] = 0.999
+    p_t[23:43] = 1e-6
+    p_t[43:63] = 1.0 - 1e-9
+    g["chi2_table_range"] = {"dof": dof_t.tolist(), "p": p_t.tolist(), "q": stats.chi2.ppf(p_t, dof_t).tolist()}
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json")
     with open(out, "w") as f:
         json.dump(g, f, indent=1)

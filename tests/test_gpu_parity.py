@@ -91,7 +91,8 @@ def test_poissinv_device_exact(engine, oracle):
 
 def test_chi2_quantile_device(engine, oracle):
     rng = np.random.default_rng(2)
-    dof = np.concatenate([rng.uniform(0.05, 2, 500), rng.uniform(2, 200, 500)])
+    # up to the last NCI table row's d + 2N (N = 1023)
+    dof = np.concatenate([rng.uniform(0.05, 2, 500), rng.uniform(2, 200, 500), rng.uniform(200, 2100, 1000)])
     p = rng.uniform(0, 1, dof.size)
     p[:20] = 0.999
     p[20:40] = 1 - 1e-9

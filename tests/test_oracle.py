@@ -136,6 +136,16 @@ def test_chi2_quantile_golden(oracle):
     assert oracle.chi2_quantile(0.32, 0.0) == 0.0
 
 
+def test_chi2_quantile_golden_table_range(oracle):
+    """Degrees of freedom d + 2N up to the last NCI table row (N = 1023),
+    including two cases where a converged Halley step once rounded onto the
+    bracket end and was taken for a step out of the bracket."""
+    g = GOLDEN["chi2_table_range"]
+    got = np.array([oracle.chi2_quantile(d, p) for d, p in zip(g["dof"], g["p"])])
+    ref = np.array(g["q"])
+    assert np.max(np.abs(got - ref) / ref) < 1e-12
+
+
 def test_incomplete_gamma_golden(oracle):
     g = GOLDEN["igamma"]
     P = np.array([oracle.gamma_p(a, x) for a, x in zip(g["a"], g["x"])])
