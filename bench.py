@@ -18,8 +18,12 @@ martingale control variate (training included) on the strip's 512 options,
 the same for the Full-Greek runs, and the control variate on a Bates-Kou
 SchemeNV strip; the CPU baseline; the roofline of the path kernel.
 
+The strip kernel's launch shape (streams per work unit, strike-loop unroll,
+grid factor) is autotuned on an untimed first call (MYQL_FLAG_AUTOTUNE) and
+reported under config.launch_shape.
+
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths-log2 24] [--no-cpu] [--no-fp64]
-                [--no-cv] [--copy-probe]
+                [--no-cv] [--no-mlmc] [--no-batched] [--no-autotune] [--shape B,U,G] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -68,7 +72,7 @@ def model():
 
 
 def jobs(paths: int, precision: int, layout: int = 0, streams: int = 0, mode: int = None,
-         seed: int = 42, bates_kou: bool = False):
+         seed: int = 42, bates_kou: bool = False, flags: int = 0):
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import InstrumentSpec, mc_config
     m = model_bates_kou() if bates_kou else model()
@@ -78,7 +82,7 @@ def jobs(paths: int, precision: int, layout: int = 0, streams: int = 0, mode: in
     for T in MATURITIES:
         inst = InstrumentSpec(strikes=STRIKES, maturity=T)
         cfg = mc_config(paths, int(round(100 * T)), seed=seed, precision=precision,
-                        stream_layout=layout, num_streams=streams)
+                        stream_layout=layout, num_streams=streams, flags=flags)
         out.append((m, [scheme], inst, mode, cfg, S0, R, Q))
     return out
 
@@ -326,6 +330,12 @@ def main():
     ap.add_argument("--no-cv", action="store_true", help="skip the price-only and control-variate legs")
     ap.add_argument("--no-mlmc", action="store_true", help="skip the multilevel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Double-Heston leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the strip kernel at its default launch shape (256-stream units, "
+                         "strike loop unrolled by 4, grid 4x) instead of autotuning it")
+    ap.add_argument("--shape", default=None,
+                    help="BLOCK,UNROLL,GRID: fix the strip kernel's launch shape (no autotuning), "
+                         "e.g. to profile the shape an autotuned run chose")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -342,7 +352,19 @@ def main():
     from myquantlib_amd.pricing import Engine
     eng = Engine(0)
     paths = 1 << args.paths_log2
-    J32 = jobs(paths, capi.FP32)
+    flags = 0 if (args.no_autotune or args.shape) else capi.FLAG_AUTOTUNE
+    if args.shape:
+        blk, unr, gf = (int(x) for x in args.shape.split(","))
+        for prec in (capi.FP32, capi.FP64):
+            eng.set_launch_shape(prec, 1, 5, False, blk, unr, gf)
+    J32 = jobs(paths, capi.FP32, flags=flags)
+    if flags:
+        # tuning pass (untimed): trial launches of every shape at 1/8 of the
+        # samples, then the call itself at the chosen shape
+        t0 = time.perf_counter()
+        eng.mc_batch(J32)
+        print(f"# autotune fp32: {time.perf_counter() - t0:.3f} s; {eng.tuning_report().strip()}",
+              file=sys.stderr, flush=True)
     for i in range(args.warmup):
         t0 = time.perf_counter()
         eng.mc_batch(J32)
@@ -364,7 +386,7 @@ def main():
 
     ms64 = None
     if not args.no_fp64:
-        J64 = jobs(paths, capi.FP64)
+        J64 = jobs(paths, capi.FP64, flags=flags)
         eng.mc_batch(J64)
         t0 = time.perf_counter()
         r64 = eng.mc_batch(J64)
@@ -415,6 +437,8 @@ def main():
                         "GreekMode::Full CRN, FP32 paths, FP64 sums, seed 42",
             "paths_per_maturity": paths,
             "path_steps_per_step": total,
+            "launch_shape": ("autotuned: " + eng.tuning_report().strip()) if flags else
+                            (f"fixed {args.shape} (block,unroll,grid)" if args.shape else "default 256,4,4"),
         },
         # price-only runs of the strip's 512 options (GreekMode None); the CV
         # time includes its training; "full_greeks_*" are the Full-mode runs

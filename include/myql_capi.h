@@ -22,6 +22,8 @@
  *   myql_create/destroy      (no counterpart: the reference has no device
  *                            state; a context owns a HIP stream, cached
  *                            stream states and NCI tables)
+ *   myql_tuning_report       launch shapes picked by MYQL_FLAG_AUTOTUNE (no
+ *                            counterpart: the reference has no launch shape)
  *   myql_cv_train,           learned martingale control variate (no
  *   myql_mc_calculate_batch_cv  counterpart: README.md:332 lists control
  *                            variates as future work)
@@ -96,8 +98,19 @@ enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
  *      flag the table gets rows up to n_ext = ceil(l + 6 sqrt(l)), l the
  *      Poisson intensity at v = max(v0, 25 theta) (at most 1023 rows), so the
  *      exact inversion -- thousands of cycles, run by the whole wavefront --
- *      is left for u >= 0.999 and essentially never for the row count. */
-enum { MYQL_FLAG_REFERENCE_NCI_ROWS = 1 };
+ *      is left for u >= 0.999 and essentially never for the row count.
+ *  AUTOTUNE  the first pricing call of a context that meets a lockstep
+ *      European kernel (per path precision, factor count) times its
+ *      launch shapes -- 256, 128 or 64 streams per work unit (block size), strike
+ *      loop over staged samples unrolled by 4 or 1, 4 or 2 persistent blocks
+ *      per resident slot -- on trial launches of that call's own jobs with an
+ *      eighth of each stream's samples, keeps the fastest for the context and
+ *      uses it from then on. Streams, draws and paths are those of the
+ *      default launch; only the FP64 summation order of the per-unit partial
+ *      sums can differ (results agree to ~1e-15 relative). Without the flag
+ *      every kernel runs its default shape (256, 4, 4). Paths per lane stay
+ *      num_paths / num_streams: changing them would change the streams. */
+enum { MYQL_FLAG_REFERENCE_NCI_ROWS = 1, MYQL_FLAG_AUTOTUNE = 2 };
 
 /* ---- model (ModelParams.hpp:7-47, ASVJmodel.hpp:14-87) ----------------- */
 typedef struct {
@@ -204,6 +217,11 @@ myql_ctx *myql_create(int device);
 void myql_destroy(myql_ctx *ctx);
 /* Message of the last failed call on this context ("" if none). */
 const char *myql_last_error(const myql_ctx *ctx);
+/* Launch shapes the autotuner (MYQL_FLAG_AUTOTUNE) chose on this context, one
+ * line per kernel with its trial times, written NUL-terminated into buf
+ * (truncated to len bytes). No counterpart in the reference, whose OpenMP
+ * loop has no launch shape. */
+int myql_tuning_report(const myql_ctx *ctx, char *buf, int64_t len);
 
 int myql_mc_calculate(myql_ctx *ctx, const myql_model *model,
                       const int32_t schemes[2], const myql_instrument *instr,
@@ -286,6 +304,14 @@ int myql_debug_poissinv(myql_ctx *ctx, const double *lambda, const double *u,
 /* Device chi-squared quantile (inverse regularised incomplete gamma). */
 int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
                              int64_t n, double *out);
+/* Fix the launch shape of one path kernel on this context, as if the
+ * autotuner had chosen it: precision MYQL_FP32/FP64, factor count, lockstep
+ * width (5 for 0/1-factor, 7 for 2-factor Europeans), path dependence,
+ * block (256, 128 or 64), unroll (4 or 1), grid factor (>= 1). MYQL_E_UNSUPPORTED
+ * for a shape that is not compiled. */
+int myql_debug_set_launch_shape(myql_ctx *ctx, int32_t precision, int32_t n_factors,
+                                int32_t lockstep_width, int32_t path_dependent,
+                                int32_t block, int32_t unroll, int32_t grid_factor);
 /* Control-variate network on n caller feature rows (n x 6, scaled features):
  * the BF16 MFMA input gradient d net/d f_{1,2,3} used by the pricing kernel
  * (out: n x 3), and the FP32 training gradient of mean (net - target)^2 over

@@ -25,6 +25,7 @@ REBATE_NONE, REBATE_AT_HIT, REBATE_AT_MATURITY = range(3)
 FP64, FP32 = range(2)
 STREAMS_JUMP, STREAMS_SEEDED = range(2)
 FLAG_REFERENCE_NCI_ROWS = 1
+FLAG_AUTOTUNE = 2
 
 
 class HestonParams(C.Structure):
@@ -111,10 +112,11 @@ class CVResult(C.Structure):
 
 # Every extern "C" symbol declared in include/myql_capi.h.
 EXPORTED = [
-    "myql_create", "myql_destroy", "myql_last_error", "myql_mc_calculate",
+    "myql_create", "myql_destroy", "myql_last_error", "myql_tuning_report", "myql_mc_calculate",
     "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_cv_train",
     "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
-    "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_cv_input_grad",
+    "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_set_launch_shape",
+    "myql_debug_cv_input_grad",
     "myql_debug_cv_param_grad",
 ]
 
@@ -135,6 +137,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_destroy.argtypes = [vp]
     lib.myql_last_error.restype = C.c_char_p
     lib.myql_last_error.argtypes = [vp]
+    lib.myql_tuning_report.restype = C.c_int
+    lib.myql_tuning_report.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.myql_mc_calculate.restype = C.c_int
     lib.myql_mc_calculate.argtypes = [vp, C.POINTER(Model), C.POINTER(C.c_int32),
                                       C.POINTER(Instrument), C.c_int32, C.POINTER(MCConfig),
@@ -155,6 +159,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_poissinv.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
     lib.myql_debug_chi2_quantile.restype = C.c_int
     lib.myql_debug_chi2_quantile.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
+    lib.myql_debug_set_launch_shape.restype = C.c_int
+    lib.myql_debug_set_launch_shape.argtypes = [vp] + [C.c_int32] * 7
     pf = C.POINTER(C.c_float)
     lib.myql_cv_train.restype = C.c_int
     lib.myql_cv_train.argtypes = [vp, C.POINTER(MCJob), C.c_int64, C.POINTER(CVConfig),

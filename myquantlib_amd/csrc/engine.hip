@@ -8,6 +8,7 @@
 #include <array>
 #include <bitset>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -301,6 +302,15 @@ struct myql_ctx {
   std::map<std::tuple<uint64_t, int, int, int>, std::unique_ptr<DevBuf>> nci;  // (d bits, rows, prec, n_fb)
   DevBuf jobs, units, partial, sums, meta, counter, tmp, accg;
   DevBuf mlmc_streams, mlmc_job;
+  // launch shapes chosen by the autotuner per (precision, factor count,
+  // lockstep width, path dependence), with the trial times behind them
+  struct Tuned {
+    myql::McShape shape;
+    int gridf;
+    std::string trials;
+  };
+  std::map<std::tuple<int, int, int, bool>, Tuned> tuned;
+  DevBuf tjobs, tunits, tcounter;
 };
 
 namespace {
@@ -774,26 +784,39 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   std::vector<double> strikes;
   for (size_t i = 0; i < specs.size(); ++i) built.push_back(build_job<R>(c, specs[i], jobs[i], strikes));
 
-  // slots and sums layout
+  // kernel groups (one persistent launch each) and their launch shapes:
+  // the tuned shape where the autotuner has chosen one, else the default
+  using GKey = std::tuple<int, int, bool>;
+  const int prec = sizeof(R) == 4 ? MYQL_FP32 : MYQL_FP64;
+  bool autotune = false;
+  for (const BuiltJob &b : built) autotune = autotune || (b.spec.cfg->flags & MYQL_FLAG_AUTOTUNE);
+  auto shape_of = [&](const GKey &k) -> std::pair<McShape, int> {
+    auto it = c->tuned.find(std::make_tuple(prec, std::get<0>(k), std::get<1>(k), std::get<2>(k)));
+    if (it != c->tuned.end()) return {it->second.shape, it->second.gridf};
+    return {MC_DEFAULT_SHAPE, 4};
+  };
+  auto gkey = [&](size_t i) { return GKey{built[i].nf, built[i].nv, built[i].pd}; };
+  auto slots_for = [&](size_t i, int blk) { return static_cast<int>((jobs[i].S + blk - 1) / blk); };
+
+  // slots and sums layout; partials are laid out for the smallest block so
+  // that any shape (trial or chosen) fits
   int soff = 0;
   std::vector<int> n_slots(jobs.size()), nks(jobs.size()), out_off(jobs.size());
   for (size_t i = 0; i < jobs.size(); ++i) {
-    n_slots[i] = built[i].n_slots;
+    n_slots[i] = slots_for(i, shape_of(gkey(i)).first.blk);
     nks[i] = built[i].nk;
     out_off[i] = soff;
     soff += 2 * NSTAT * built[i].nk;
   }
-  size_t partial_elems = 0;
-  for (size_t i = 0; i < jobs.size(); ++i)
-    partial_elems += static_cast<size_t>(n_slots[i]) * 2 * NSTAT * nks[i];
-  // lay partials out per job contiguously: offsets in units of that job's width
+  int min_blk = BLOCK;
+  for (const McShape &sh : MC_TUNE_SHAPES) min_blk = std::min(min_blk, sh.blk);
   std::vector<size_t> pbase(jobs.size());
-  {
-    size_t o = 0;
-    for (size_t i = 0; i < jobs.size(); ++i) {
-      pbase[i] = o;
-      o += static_cast<size_t>(n_slots[i]) * 2 * NSTAT * nks[i];
-    }
+  size_t partial_elems = 0;
+  uint64_t stream_total = 0;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    pbase[i] = partial_elems;
+    partial_elems += static_cast<size_t>(slots_for(i, min_blk)) * 2 * NSTAT * nks[i];
+    stream_total += jobs[i].S + BLOCK;
   }
   c->partial.reserve(std::max<size_t>(partial_elems, 1) * sizeof(double));
   c->sums.reserve(std::max(soff, 1) * sizeof(double));
@@ -811,28 +834,100 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   if (strike_bytes)
     HIPCHECK(hipMemcpyAsync(d_strikes, strikes.data(), strike_bytes, hipMemcpyHostToDevice, c->stream));
   const Job<R> *d_jobs = c->jobs.as<Job<R>>();
+  // per-unit accumulators: 2*NSTAT*blk doubles per unit of blk streams
+  c->accg.reserve(static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double));
+
+  std::map<GKey, std::vector<size_t>> members;
+  for (size_t i = 0; i < jobs.size(); ++i) members[gkey(i)].push_back(i);
+  // units of one group for block size blk, most expensive job first
+  auto units_for = [&](const std::vector<size_t> &idx, int blk) {
+    std::vector<Unit> u;
+    for (size_t i : idx)
+      for (int b = 0; b < slots_for(i, blk); ++b) u.push_back({static_cast<int>(i), b});
+    std::stable_sort(u.begin(), u.end(), [&](const Unit &a, const Unit &b) { return built[a.job].cost > built[b.job].cost; });
+    return u;
+  };
+  auto staging = [&](const GKey &k, int blk) {
+    int nfields = 1;
+    for (size_t i : members[k]) nfields = std::max(nfields, jobs[i].n_fields);
+    const size_t smem = static_cast<size_t>(nfields) * MAXV * blk * sizeof(R);
+    if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
+    return smem;
+  };
+  // Persistent grid: gridf x the resident blocks per CU (the occupancy
+  // query, but never fewer than the VGPR budget allows: 512 per SIMD lane,
+  // blk/64 waves per block, 4 SIMDs); surplus blocks find the queue empty.
+  auto grid_for = [&](const GKey &k, McShape sh, int gridf, size_t smem, int n_units) {
+    const int bpc = std::max(mc_blocks_per_cu<R>(std::get<0>(k), std::get<1>(k), std::get<2>(k), sh, smem),
+                             2 * BLOCK / sh.blk);
+    return std::max(1, std::min(n_units, gridf * bpc * c->n_cu));
+  };
+
+  // Autotuner: time every shape on trial launches of this call's own jobs
+  // with an eighth of each stream's samples, keep the fastest.
+  if (autotune) {
+    for (auto &kv : members) {
+      const GKey &k = kv.first;
+      const auto tkey = std::make_tuple(prec, std::get<0>(k), std::get<1>(k), std::get<2>(k));
+      if (!mc_shape_tunable(std::get<0>(k), std::get<1>(k), std::get<2>(k)) || c->tuned.count(tkey)) continue;
+      std::vector<Job<R>> tj = jobs;
+      for (Job<R> &j : tj) j.M = j.S * std::max<uint64_t>(1, (j.M / j.S) / 8);
+      c->tjobs.reserve(jobs_bytes);
+      HIPCHECK(hipMemcpyAsync(c->tjobs.p, tj.data(), jobs_bytes, hipMemcpyHostToDevice, c->stream));
+      c->tcounter.reserve(64);
+      std::string log;
+      float best = 0.f;
+      McShape best_sh = MC_DEFAULT_SHAPE;
+      int best_gf = 4;
+      for (const McShape &sh : MC_TUNE_SHAPES) {
+        const std::vector<Unit> u = units_for(kv.second, sh.blk);
+        c->tunits.reserve(u.size() * sizeof(Unit));
+        HIPCHECK(hipMemcpyAsync(c->tunits.p, u.data(), u.size() * sizeof(Unit), hipMemcpyHostToDevice, c->stream));
+        const size_t smem = staging(k, sh.blk);
+        const int n_units = static_cast<int>(u.size());
+        for (int rep = 0; rep < 3; ++rep) {  // rep 0 warms the kernel up, untimed
+          const int gridf = rep == 2 ? 1 : 4;
+          HIPCHECK(hipMemsetAsync(c->tcounter.p, 0, 64, c->stream));
+          HIPCHECK(hipEventRecord(c->ev0, c->stream));
+          HIPCHECK(launch_mc<R>(std::get<0>(k), std::get<1>(k), std::get<2>(k), sh, c->tjobs.as<Job<R>>(),
+                                c->tunits.as<Unit>(), n_units, c->tcounter.as<unsigned>(), c->accg.as<double>(),
+                                smem, grid_for(k, sh, gridf, smem, n_units), c->stream));
+          HIPCHECK(hipEventRecord(c->ev1, c->stream));
+          HIPCHECK(hipEventSynchronize(c->ev1));
+          float ms = 0.f;
+          HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+          if (rep == 0) continue;
+          char line[96];
+          std::snprintf(line, sizeof line, "%s%d/%d/%dx %.3f ms", log.empty() ? "" : ", ", sh.blk, sh.unr, gridf,
+                        static_cast<double>(ms));
+          log += line;
+          if (best == 0.f || ms < best) {
+            best = ms;
+            best_sh = sh;
+            best_gf = gridf;
+          }
+        }
+      }
+      c->tuned[tkey] = {best_sh, best_gf, log};
+    }
+    for (size_t i = 0; i < jobs.size(); ++i) n_slots[i] = slots_for(i, shape_of(gkey(i)).first.blk);
+  }
 
   // units grouped by kernel instantiation, most expensive first
-  std::map<std::tuple<int, int, bool>, std::vector<Unit>> groups;
-  for (size_t i = 0; i < jobs.size(); ++i)
-    for (int b = 0; b < n_slots[i]; ++b)
-      groups[{built[i].nf, built[i].nv, built[i].pd}].push_back({static_cast<int>(i), b});
-  size_t max_units = 0;
-  for (auto &kv : groups) max_units = std::max(max_units, kv.second.size());
-  c->units.reserve(groups.size() * max_units * sizeof(Unit) + 64);
-  c->counter.reserve(groups.size() * 64);
-  HIPCHECK(hipMemsetAsync(c->counter.p, 0, groups.size() * 64, c->stream));
   std::vector<Unit> all_units;
   std::vector<std::pair<size_t, size_t>> ranges;
-  for (auto &kv : groups) {
-    auto &u = kv.second;
-    std::stable_sort(u.begin(), u.end(), [&](const Unit &a, const Unit &b) { return built[a.job].cost > built[b.job].cost; });
+  std::vector<GKey> order;
+  for (auto &kv : members) {
+    const std::vector<Unit> u = units_for(kv.second, shape_of(kv.first).first.blk);
     ranges.push_back({all_units.size(), u.size()});
     all_units.insert(all_units.end(), u.begin(), u.end());
+    order.push_back(kv.first);
   }
+  c->units.reserve(all_units.size() * sizeof(Unit) + 64);
+  c->counter.reserve(members.size() * 64);
+  HIPCHECK(hipMemsetAsync(c->counter.p, 0, members.size() * 64, c->stream));
   HIPCHECK(hipMemcpyAsync(c->units.p, all_units.data(), all_units.size() * sizeof(Unit), hipMemcpyHostToDevice,
                           c->stream));
-  c->accg.reserve(all_units.size() * 2 * NSTAT * BLOCK * sizeof(double));
 
   // meta for the reduction: per job the element offset of its partials,
   // its unit count, strike count and output offset
@@ -849,25 +944,18 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   HIPCHECK(hipMemcpyAsync(d_mint, mint.data(), mint.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
 
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
-  size_t gi = 0;
-  for (auto &kv : groups) {
-    const int nf = std::get<0>(kv.first), nv = std::get<1>(kv.first);
-    const bool pdk = std::get<2>(kv.first);
-    int nfields = 1;
-    for (size_t i = 0; i < jobs.size(); ++i)
-      if (built[i].nf == nf && built[i].nv == nv && built[i].pd == pdk) nfields = std::max(nfields, jobs[i].n_fields);
-    const size_t smem = static_cast<size_t>(nfields) * MAXV * BLOCK * sizeof(R);
-    if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
-    // Resident blocks per CU: the occupancy query, but never fewer than the
-    // VGPR budget allows (512 per SIMD lane, 4 waves per block, 4 SIMDs);
-    // surplus blocks of the persistent grid find the queue empty and exit.
-    const int bpc = std::max(mc_blocks_per_cu<R>(nf, nv, pdk, smem), 2);
+  size_t acc_off = 0;  // doubles
+  for (size_t gi = 0; gi < order.size(); ++gi) {
+    const GKey &k = order[gi];
+    const auto sg = shape_of(k);
+    const McShape sh = sg.first;
+    const size_t smem = staging(k, sh.blk);
     const int n_units = static_cast<int>(ranges[gi].second);
-    const int grid = std::max(1, std::min(n_units, 4 * bpc * c->n_cu));
-    HIPCHECK(launch_mc<R>(nf, nv, pdk, d_jobs, c->units.as<Unit>() + ranges[gi].first, n_units,
-                          c->counter.as<unsigned>() + 16 * gi,
-                          c->accg.as<double>() + ranges[gi].first * 2 * NSTAT * BLOCK, smem, grid, c->stream));
-    ++gi;
+    HIPCHECK(launch_mc<R>(std::get<0>(k), std::get<1>(k), std::get<2>(k), sh, d_jobs,
+                          c->units.as<Unit>() + ranges[gi].first, n_units, c->counter.as<unsigned>() + 16 * gi,
+                          c->accg.as<double>() + acc_off, smem, grid_for(k, sh, sg.second, smem, n_units),
+                          c->stream));
+    acc_off += static_cast<size_t>(n_units) * 2 * NSTAT * sh.blk;
   }
   {
     int maxw = 0;
@@ -1519,6 +1607,26 @@ void myql_destroy(myql_ctx *c) {
 
 const char *myql_last_error(const myql_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+int myql_tuning_report(const myql_ctx *c, char *buf, int64_t len) {
+  if (!c || !buf || len <= 0) return MYQL_E_INVALID_ARG;
+  std::string out;
+  for (const auto &kv : c->tuned) {
+    const int prec = std::get<0>(kv.first), nf = std::get<1>(kv.first), nv = std::get<2>(kv.first);
+    const bool pd = std::get<3>(kv.first);
+    char line[160];
+    std::snprintf(line, sizeof line, "%s nf=%d nv=%d %s: block %d unroll %d grid %dx; trials (block/unroll/grid) ",
+                  prec == MYQL_FP32 ? "fp32" : "fp64", nf, nv, pd ? "path-dependent" : "european",
+                  kv.second.shape.blk, kv.second.shape.unr, kv.second.gridf);
+    out += line;
+    out += kv.second.trials;
+    out += "\n";
+  }
+  const size_t n = std::min(out.size(), static_cast<size_t>(len - 1));
+  std::memcpy(buf, out.data(), n);
+  buf[n] = '\0';
+  return MYQL_OK;
+}
+
 int myql_mc_calculate(myql_ctx *c, const myql_model *model, const int32_t schemes[2], const myql_instrument *instr,
                       int32_t greek_mode, const myql_mc_config *cfg, double S0, double r, double q,
                       myql_mc_result *res) {
@@ -1637,6 +1745,28 @@ int myql_debug_normcdfinv(myql_ctx *c, const double *p, int64_t n, double *out64
 int myql_debug_poissinv(myql_ctx *c, const double *lambda, const double *u, int64_t n, double *out) {
   if (!lambda || !u || !out) return MYQL_E_INVALID_ARG;
   return debug_map(c, 1, lambda, u, n, out, nullptr);
+}
+
+int myql_debug_set_launch_shape(myql_ctx *c, int32_t precision, int32_t n_factors, int32_t lockstep_width,
+                                int32_t path_dependent, int32_t block, int32_t unroll, int32_t grid_factor) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  const bool pd = path_dependent != 0;
+  if ((precision != MYQL_FP32 && precision != MYQL_FP64) || grid_factor < 1 ||
+      !myql::mc_supported(n_factors, lockstep_width, pd)) {
+    c->err = "bad launch shape arguments";
+    return MYQL_E_INVALID_ARG;
+  }
+  bool ok = block == myql::MC_DEFAULT_SHAPE.blk && unroll == myql::MC_DEFAULT_SHAPE.unr;
+  if (myql::mc_shape_tunable(n_factors, lockstep_width, pd))
+    for (const myql::McShape &sh : myql::MC_TUNE_SHAPES) ok = ok || (sh.blk == block && sh.unr == unroll);
+  if (!ok) {
+    c->err = "launch shape not compiled for this kernel";
+    return MYQL_E_UNSUPPORTED;
+  }
+  c->tuned[std::make_tuple(static_cast<int>(precision), static_cast<int>(n_factors),
+                           static_cast<int>(lockstep_width), pd)] = {{block, unroll}, grid_factor, "set"};
+  c->err.clear();
+  return MYQL_OK;
 }
 
 int myql_debug_chi2_quantile(myql_ctx *c, const double *dof, const double *p, int64_t n, double *out) {

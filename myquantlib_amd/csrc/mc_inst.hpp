@@ -1,0 +1,57 @@
+// Instantiations of the path kernels for one path precision, MYQL_R, which
+// the including translation unit defines (mc_fp32.hip: float, mc_fp64.hip:
+// double).
+#include "mc_kernel.hpp"
+#include "mc_launch.hpp"
+
+namespace myql {
+
+// (factor count, lockstep width, path dependence, block, unroll): every
+// kernel at the default shape, the lockstep European kernels also at the
+// other shapes of MC_TUNE_SHAPES.
+#define MYQL_MC_CASES(X)                                                                                  \
+  X(0, 1, false, 256, 4) X(0, 5, false, 256, 4) X(1, 1, false, 256, 4) X(1, 5, false, 256, 4)             \
+  X(2, 1, false, 256, 4) X(2, 7, false, 256, 4) X(0, 1, true, 256, 4) X(0, 7, true, 256, 4)               \
+  X(1, 1, true, 256, 4) X(1, 7, true, 256, 4) X(2, 1, true, 256, 4) X(2, 9, true, 256, 4)                 \
+  X(0, 5, false, 256, 1) X(1, 5, false, 256, 1) X(2, 7, false, 256, 1)                                    \
+  X(0, 5, false, 128, 4) X(1, 5, false, 128, 4) X(2, 7, false, 128, 4)                                    \
+  X(0, 5, false, 128, 1) X(1, 5, false, 128, 1) X(2, 7, false, 128, 1)                                    \
+  X(0, 5, false, 64, 4) X(1, 5, false, 64, 4) X(2, 7, false, 64, 4)                                       \
+  X(0, 5, false, 64, 1) X(1, 5, false, 64, 1) X(2, 7, false, 64, 1)
+
+template <>
+hipError_t launch_mc<MYQL_R>(int nf, int nv, bool pd, McShape shape, const Job<MYQL_R> *jobs, const Unit *units,
+                             int n_units, unsigned *counter, double *accg, size_t smem, int grid,
+                             hipStream_t stream) {
+#define MYQL_LAUNCH(NF_, NV_, PD_, B_, U_)                                                                   \
+  if (nf == NF_ && nv == NV_ && pd == PD_ && shape.blk == B_ && shape.unr == U_) {                            \
+    if (smem > 64 * 1024)                                                                                    \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(mc_paths_kernel<MYQL_R, NF_, NV_, PD_, B_, U_>), \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));         \
+    hipLaunchKernelGGL((mc_paths_kernel<MYQL_R, NF_, NV_, PD_, B_, U_>), dim3(grid), dim3(B_), smem, stream,  \
+                       jobs, units, n_units, counter, accg);                                                 \
+    return hipGetLastError();                                                                                \
+  }
+  MYQL_MC_CASES(MYQL_LAUNCH)
+#undef MYQL_LAUNCH
+  return hipErrorInvalidValue;
+}
+
+template <>
+int mc_blocks_per_cu<MYQL_R>(int nf, int nv, bool pd, McShape shape, size_t smem) {
+  int nb = 0;
+#define MYQL_OCC(NF_, NV_, PD_, B_, U_)                                                                       \
+  if (nf == NF_ && nv == NV_ && pd == PD_ && shape.blk == B_ && shape.unr == U_) {                             \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mc_paths_kernel<MYQL_R, NF_, NV_, PD_, B_, U_>, B_,  \
+                                                     smem) != hipSuccess)                                      \
+      nb = 1;                                                                                                  \
+    return nb > 0 ? nb : 1;                                                                                    \
+  }
+  MYQL_MC_CASES(MYQL_OCC)
+#undef MYQL_OCC
+  return 1;
+}
+
+#undef MYQL_MC_CASES
+
+}  // namespace myql

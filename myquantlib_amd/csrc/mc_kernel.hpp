@@ -129,7 +129,7 @@ __device__ __forceinline__ void trk_update(const Job<R> *__restrict__ J, Trk<R> 
 }
 
 // Finalize and stage one variant's terminal data in LDS.
-template <typename R>
+template <typename R, int BLK = BLOCK>
 __device__ __forceinline__ void finalize_stage(const Job<R> *__restrict__ J, const Trk<R> &s, R x, R dt,
                                                R T, R *__restrict__ stage, int v, int tid) {
   R aux = 0;
@@ -150,13 +150,13 @@ __device__ __forceinline__ void finalize_stage(const Job<R> *__restrict__ J, con
       break;
     }
   }
-  stage[(F_SPOT * MAXV + v) * BLOCK + tid] = dexp<R>(x);
-  if (J->n_fields > F_AUX) stage[(F_AUX * MAXV + v) * BLOCK + tid] = aux;
-  if (J->n_fields > F_FLAGS) stage[(F_FLAGS * MAXV + v) * BLOCK + tid] = static_cast<R>(s.hits);
+  stage[(F_SPOT * MAXV + v) * BLK + tid] = dexp<R>(x);
+  if (J->n_fields > F_AUX) stage[(F_AUX * MAXV + v) * BLK + tid] = aux;
+  if (J->n_fields > F_FLAGS) stage[(F_FLAGS * MAXV + v) * BLK + tid] = static_cast<R>(s.hits);
   if (J->n_fields > F_HT0) {
-    stage[(F_HT0 * MAXV + v) * BLOCK + tid] = s.ht[0];
-    stage[(F_HT1 * MAXV + v) * BLOCK + tid] = s.ht[1];
-    stage[(F_HT2 * MAXV + v) * BLOCK + tid] = s.ht[2];
+    stage[(F_HT0 * MAXV + v) * BLK + tid] = s.ht[0];
+    stage[(F_HT1 * MAXV + v) * BLK + tid] = s.ht[1];
+    stage[(F_HT2 * MAXV + v) * BLK + tid] = s.ht[2];
   }
 }
 
@@ -500,7 +500,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
 // Simulate one variant group for the lane's current sample and stage it.
 // PD selects path-dependent trackers; European kernels carry no tracker
 // state at all.
-template <typename R, int NF, int NV, bool PD>
+template <typename R, int NF, int NV, bool PD, int BLK = BLOCK>
 __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfirst, Rng &g,
                                           R *__restrict__ stage, int tid) {
   constexpr int NFX = NF > 0 ? NF : 1;
@@ -566,9 +566,9 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     if constexpr (PD) {
-      finalize_stage(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
+      finalize_stage<R, BLK>(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
     } else {
-      stage[(F_SPOT * MAXV + vid[k]) * BLOCK + tid] = dexp<R>(x[k]);
+      stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = dexp<R>(x[k]);
     }
   }
 }
@@ -591,15 +591,15 @@ __device__ __forceinline__ double payoff(const Job<R> *__restrict__ J, bool smoo
 
 // base / up / dn payoffs of staged variant v at sample s for path-dependent
 // instruments (Asian.hpp:168-206, Lookback.hpp:229-268, Barrier.hpp:76-110)
-template <typename R>
+template <typename R, int BLK = BLOCK>
 __device__ static __noinline__ void eval3_pd(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
                                              int s, double K, bool bumps, double &p, double &pu, double &pd) {
-  const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLOCK + s]);
+  const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLK + s]);
   const double mu = J->mult_up, md = J->mult_dn;
   const bool smooth = J->mode != 0;
   const int trk = J->tracker;
   if (trk != 4) {
-    const double A = static_cast<double>(stage[(F_AUX * MAXV + v) * BLOCK + s]);
+    const double A = static_cast<double>(stage[(F_AUX * MAXV + v) * BLK + s]);
     if (J->fixed_strike) {
       p = payoff(J, smooth, A, K);
       if (bumps) {
@@ -614,14 +614,14 @@ __device__ static __noinline__ void eval3_pd(const Job<R> *__restrict__ J, const
       }
     }
   } else {
-    const int hits = static_cast<int>(stage[(F_FLAGS * MAXV + v) * BLOCK + s]);
+    const int hits = static_cast<int>(stage[(F_FLAGS * MAXV + v) * BLK + s]);
     auto bar = [&](double sp, int idx) {
       const bool hit = (hits >> idx) & 1;
       double reb = 0.0;
       if (J->rebate_timing != 0) {
         reb = J->rebate;
         if (J->rebate_timing == 1 && hit) {
-          const double ht = static_cast<double>(stage[((F_HT0 + idx) * MAXV + v) * BLOCK + s]);
+          const double ht = static_cast<double>(stage[((F_HT0 + idx) * MAXV + v) * BLK + s]);
           reb = J->rebate * exp(J->rebate_rate * (J->maturity - ht));
         }
       }
@@ -639,14 +639,14 @@ __device__ static __noinline__ void eval3_pd(const Job<R> *__restrict__ J, const
 
 // base / up / dn payoffs of staged variant v at sample s (calculate_to_buffer;
 // European.hpp:40-74 scales the terminal spot by (S0 +- h)/S0)
-template <typename R>
+template <typename R, int BLK = BLOCK>
 __device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
                                       int s, double K, bool bumps, double &p, double &pu, double &pd) {
   if (J->tracker != 0) {
-    eval3_pd(J, stage, v, s, K, bumps, p, pu, pd);
+    eval3_pd<R, BLK>(J, stage, v, s, K, bumps, p, pu, pd);
     return;
   }
-  const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLOCK + s]);
+  const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLK + s]);
   const bool smooth = J->mode != 0;
   p = payoff(J, smooth, S, K);
   if (bumps) {
@@ -661,13 +661,14 @@ __device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__r
 // staged samples -- price/delta/gamma, then vega+vanna per factor, then
 // theta+charm, then rho -- so at most six FP64 sums are live at once; each
 // pass adds its batch sums into the lane's own slots in global memory
-// (accg[a*BLOCK + tid]), so the path simulation holds no accumulators.
+// (accg[a*BLK + tid]), so the path simulation holds no accumulators.
+template <int BLK>
 __device__ __forceinline__ void flush(double *__restrict__ accg, int a, int tid, bool first, double x) {
-  double *slot = accg + a * BLOCK + tid;
+  double *slot = accg + a * BLK + tid;
   *slot = first ? x : *slot + x;
 }
 
-template <typename R>
+template <typename R, int BLK, int UNR>
 __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
                                            const int *__restrict__ valid, int tid, double *__restrict__ accg,
                                            bool first) {
@@ -682,11 +683,11 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   {
     double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg = 0, sg2 = 0;
     if (live) {
-#pragma unroll 4
+#pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double p, pu = 0, pd = 0;
-        eval3(J, stage, V_BASE, s, K, ess, p, pu, pd);
+        eval3<R, BLK>(J, stage, V_BASE, s, K, ess, p, pu, pd);
         sp += p;
         sp2 += p * p;
         if (ess) {
@@ -699,13 +700,13 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
         }
       }
     }
-    flush(accg, 2 * ST_PRICE, tid, first, sp);
-    flush(accg, 2 * ST_PRICE + 1, tid, first, sp2);
+    flush<BLK>(accg, 2 * ST_PRICE, tid, first, sp);
+    flush<BLK>(accg, 2 * ST_PRICE + 1, tid, first, sp2);
     if (ess) {
-      flush(accg, 2 * ST_DELTA, tid, first, sd);
-      flush(accg, 2 * ST_DELTA + 1, tid, first, sd2);
-      flush(accg, 2 * ST_GAMMA, tid, first, sg);
-      flush(accg, 2 * ST_GAMMA + 1, tid, first, sg2);
+      flush<BLK>(accg, 2 * ST_DELTA, tid, first, sd);
+      flush<BLK>(accg, 2 * ST_DELTA + 1, tid, first, sd2);
+      flush<BLK>(accg, 2 * ST_GAMMA, tid, first, sg);
+      flush<BLK>(accg, 2 * ST_GAMMA + 1, tid, first, sg2);
     }
   }
   if (!full) return;
@@ -713,12 +714,12 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   for (int f = 0; f < J->nvega; ++f) {
     double sv = 0, sv2 = 0, sn = 0, sn2 = 0;
     if (live) {
-#pragma unroll 4
+#pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
-        eval3(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
-        eval3(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
+        eval3<R, BLK>(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
+        eval3<R, BLK>(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
         const double vg = (ap - bp) * J->inv2dv * J->cv[f];
         const double vn = ((au - ad) - (bu - bd)) * (J->inv2dv * J->inv2h) * J->cv[f];
         sv += vg;
@@ -727,26 +728,26 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
         sn2 += vn * vn;
       }
     }
-    flush(accg, 2 * (ST_VEGA0 + f), tid, first, sv);
-    flush(accg, 2 * (ST_VEGA0 + f) + 1, tid, first, sv2);
-    flush(accg, 2 * (ST_VANNA0 + f), tid, first, sn);
-    flush(accg, 2 * (ST_VANNA0 + f) + 1, tid, first, sn2);
+    flush<BLK>(accg, 2 * (ST_VEGA0 + f), tid, first, sv);
+    flush<BLK>(accg, 2 * (ST_VEGA0 + f) + 1, tid, first, sv2);
+    flush<BLK>(accg, 2 * (ST_VANNA0 + f), tid, first, sn);
+    flush<BLK>(accg, 2 * (ST_VANNA0 + f) + 1, tid, first, sn2);
   }
   if (J->nvega < 2) {
-    flush(accg, 2 * ST_VEGA1, tid, first, 0.0);
-    flush(accg, 2 * ST_VEGA1 + 1, tid, first, 0.0);
-    flush(accg, 2 * ST_VANNA1, tid, first, 0.0);
-    flush(accg, 2 * ST_VANNA1 + 1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VEGA1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VEGA1 + 1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VANNA1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VANNA1 + 1, tid, first, 0.0);
   }
   {
     double st = 0, st2 = 0, sc = 0, sc2 = 0;
     if (live) {
-#pragma unroll 4
+#pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
-        eval3(J, stage, V_TUP, s, K, true, ap, au, ad);
-        eval3(J, stage, V_TDN, s, K, true, bp, bu, bd);
+        eval3<R, BLK>(J, stage, V_TUP, s, K, true, ap, au, ad);
+        eval3<R, BLK>(J, stage, V_TDN, s, K, true, bp, bu, bd);
         const double th = (ap * J->dfTu - bp * J->dfTd) * J->inv2dT;
         const double ch = ((au - ad) * J->dfTu - (bu - bd) * J->dfTd) * (J->inv2dT * J->inv2h);
         st += th;
@@ -755,49 +756,50 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
         sc2 += ch * ch;
       }
     }
-    flush(accg, 2 * ST_THETA, tid, first, st);
-    flush(accg, 2 * ST_THETA + 1, tid, first, st2);
-    flush(accg, 2 * ST_CHARM, tid, first, sc);
-    flush(accg, 2 * ST_CHARM + 1, tid, first, sc2);
+    flush<BLK>(accg, 2 * ST_THETA, tid, first, st);
+    flush<BLK>(accg, 2 * ST_THETA + 1, tid, first, st2);
+    flush<BLK>(accg, 2 * ST_CHARM, tid, first, sc);
+    flush<BLK>(accg, 2 * ST_CHARM + 1, tid, first, sc2);
   }
   {
     double sr = 0, sr2 = 0;
     if (live) {
-#pragma unroll 4
+#pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, bp, dum0, dum1;
         if (J->derive_r) {
           // European: the rate-bumped paths are the base path shifted by
           // +-r_bump*T in log space (only C0 of the log step depends on r)
-          const double S = static_cast<double>(stage[(F_SPOT * MAXV + V_BASE) * BLOCK + s]);
+          const double S = static_cast<double>(stage[(F_SPOT * MAXV + V_BASE) * BLK + s]);
           ap = payoff(J, mode != 0, S * J->erb_up, K);
           bp = payoff(J, mode != 0, S * J->erb_dn, K);
         } else {
-          eval3(J, stage, V_RUP, s, K, false, ap, dum0, dum1);
-          eval3(J, stage, V_RDN, s, K, false, bp, dum0, dum1);
+          eval3<R, BLK>(J, stage, V_RUP, s, K, false, ap, dum0, dum1);
+          eval3<R, BLK>(J, stage, V_RDN, s, K, false, bp, dum0, dum1);
         }
         const double rh = (ap * J->dfru - bp * J->dfrd) * J->inv2dr;
         sr += rh;
         sr2 += rh * rh;
       }
     }
-    flush(accg, 2 * ST_RHO, tid, first, sr);
-    flush(accg, 2 * ST_RHO + 1, tid, first, sr2);
+    flush<BLK>(accg, 2 * ST_RHO, tid, first, sr);
+    flush<BLK>(accg, 2 * ST_RHO + 1, tid, first, sr2);
   }
 }
 
 // Persistent work-queue kernel. `counter` must be zero at launch; `accg`
-// holds 2*NSTAT*BLOCK doubles per work unit.
-template <typename R, int NF, int NV, bool PD>
-__global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restrict__ jobs,
+// holds 2*NSTAT*BLK doubles per work unit of BLK streams; UNR unrolls the
+// strike loop over staged samples.
+template <typename R, int NF, int NV, bool PD, int BLK = BLOCK, int UNR = 4>
+__global__ void __launch_bounds__(BLK) mc_paths_kernel(const Job<R> *__restrict__ jobs,
                                                          const Unit *__restrict__ units, int n_units,
                                                          unsigned *__restrict__ counter,
                                                          double *__restrict__ accg_all) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_unit;
-  __shared__ int s_valid[BLOCK];
-  __shared__ double s_red[BLOCK];
+  __shared__ int s_valid[BLK];
+  __shared__ double s_red[BLK];
   R *stage = reinterpret_cast<R *>(smem);
   const int tid = threadIdx.x;
 
@@ -810,10 +812,10 @@ __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restric
     const int job = __builtin_amdgcn_readfirstlane(units[u].job);
     const int blk = __builtin_amdgcn_readfirstlane(units[u].blk);
     const Job<R> *__restrict__ J = jobs + job;
-    double *__restrict__ accg = accg_all + static_cast<size_t>(u) * (2 * NSTAT * BLOCK);
+    double *__restrict__ accg = accg_all + static_cast<size_t>(u) * (2 * NSTAT * BLK);
 
     const uint64_t S = J->S, M = J->M;
-    const uint64_t s = static_cast<uint64_t>(blk) * BLOCK + tid;
+    const uint64_t s = static_cast<uint64_t>(blk) * BLK + tid;
     const bool active = s < S;
     Rng g;
     if (active) {
@@ -826,27 +828,27 @@ __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restric
     }
     const uint64_t q = M / S, rem = M % S;
     const uint64_t count = active ? q + (s < rem ? 1 : 0) : 0;
-    const uint64_t n_it = q + (static_cast<uint64_t>(blk) * BLOCK < rem ? 1 : 0);
+    const uint64_t n_it = q + (static_cast<uint64_t>(blk) * BLK < rem ? 1 : 0);
 
     for (uint64_t it = 0; it < n_it; ++it) {
       const bool valid = it < count;
       if (valid) {
         if constexpr (NV > 1) {
           // lockstep: one group, the stream advances as the base path does
-          sim_group<R, NF, NV, PD>(J, 0, g, stage, tid);
+          sim_group<R, NF, NV, PD, BLK>(J, 0, g, stage, tid);
         } else {
           // replay: every variant restarts from the sample's stream state
           const Rng snap = g;
-          sim_group<R, NF, 1, PD>(J, 0, g, stage, tid);
+          sim_group<R, NF, 1, PD, BLK>(J, 0, g, stage, tid);
           for (int gi = 1; gi < J->n_groups; ++gi) {
             Rng gg = snap;
-            sim_group<R, NF, 1, PD>(J, J->group_first[gi], gg, stage, tid);
+            sim_group<R, NF, 1, PD, BLK>(J, J->group_first[gi], gg, stage, tid);
           }
         }
       }
       s_valid[tid] = valid ? 1 : 0;
       __syncthreads();
-      accumulate(J, stage, s_valid, tid, accg, it == 0);
+      accumulate<R, BLK, UNR>(J, stage, s_valid, tid, accg, it == 0);
       __syncthreads();
     }
 
@@ -858,13 +860,13 @@ __global__ void __launch_bounds__(BLOCK) mc_paths_kernel(const Job<R> *__restric
     const int na = mode == 2 ? 2 * NSTAT : (mode == 1 ? 2 * (ST_GAMMA + 1) : 2);
     double *out = J->partial + static_cast<size_t>(J->first_slot + blk) * (2 * NSTAT) * J->nk;
     for (int a = 0; a < 2 * NSTAT; ++a) {
-      double x = (a < na && n_it > 0) ? accg[a * BLOCK + tid] : 0.0;
+      double x = (a < na && n_it > 0) ? accg[a * BLK + tid] : 0.0;
       for (int off = nkp; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
       s_red[tid] = x;
       __syncthreads();
       if (wave == 0 && lane < J->nk) {
         double t = 0.0;
-        for (int w = 0; w < BLOCK / 64; ++w) t += s_red[w * 64 + lane];
+        for (int w = 0; w < BLK / 64; ++w) t += s_red[w * 64 + lane];
         out[a * J->nk + lane] = t;
       }
       __syncthreads();

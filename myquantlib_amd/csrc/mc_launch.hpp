@@ -1,5 +1,5 @@
 // Launch helpers for the path kernels; one translation unit per path
-// precision instantiates them (mc_fp32.hip, mc_fp64.hip).
+// precision instantiates them (mc_fp32.hip, mc_fp64.hip via mc_inst.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -11,12 +11,23 @@ namespace myql {
 //   European NF 0/1: 1, 5; NF 2: 1, 7. Path-dependent NF 0/1: 1, 7; NF 2: 1, 9.
 bool mc_supported(int nf, int nv, bool pd);
 
-// pd: path-dependent tracker kernels; accg: 2*NSTAT*BLOCK doubles per unit.
+// Launch shape of the path kernel: streams per work unit (= lanes per block)
+// and the unroll of the strike loop over staged samples. Every kernel has
+// the default shape; the lockstep European kernels (the strip workloads)
+// also have the others, among which the autotuner picks.
+struct McShape {
+  int blk, unr;
+};
+constexpr McShape MC_DEFAULT_SHAPE{BLOCK, 4};
+constexpr McShape MC_TUNE_SHAPES[6] = {{256, 4}, {256, 1}, {128, 4}, {128, 1}, {64, 4}, {64, 1}};
+inline bool mc_shape_tunable(int nf, int nv, bool pd) { return !pd && nv > 1 && nf >= 0 && nf <= 2; }
+
+// pd: path-dependent tracker kernels; accg: 2*NSTAT*shape.blk doubles per unit.
 template <typename R>
-hipError_t launch_mc(int nf, int nv, bool pd, const Job<R> *jobs, const Unit *units, int n_units,
+hipError_t launch_mc(int nf, int nv, bool pd, McShape shape, const Job<R> *jobs, const Unit *units, int n_units,
                      unsigned *counter, double *accg, size_t smem, int grid, hipStream_t stream);
 
 template <typename R>
-int mc_blocks_per_cu(int nf, int nv, bool pd, size_t smem);
+int mc_blocks_per_cu(int nf, int nv, bool pd, McShape shape, size_t smem);
 
 }  // namespace myql

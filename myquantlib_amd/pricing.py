@@ -210,6 +210,19 @@ class Engine:
             msg = self.lib.myql_last_error(self.ctx).decode()
             raise EngineError(rc, msg)
 
+    def set_launch_shape(self, precision: int, n_factors: int, lockstep_width: int, path_dependent: bool,
+                         block: int, unroll: int, grid_factor: int = 4):
+        """Fix one path kernel's launch shape (myql_debug_set_launch_shape)."""
+        self._check(self.lib.myql_debug_set_launch_shape(self.ctx, precision, n_factors, lockstep_width,
+                                                         int(path_dependent), block, unroll, grid_factor))
+
+    def tuning_report(self) -> str:
+        """Launch shapes chosen under FLAG_AUTOTUNE on this context (one line
+        per kernel, with the trial times); empty before any tuned call."""
+        buf = C.create_string_buffer(8192)
+        self._check(self.lib.myql_tuning_report(self.ctx, buf, len(buf)))
+        return buf.value.decode()
+
     def mc(self, model: Model, schemes, instr: InstrumentSpec, mode: int, cfg: MCConfig,
            S0: float, r: float, q: float) -> dict:
         s = instr.struct()
