@@ -99,11 +99,12 @@ enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
  *      Poisson intensity at v = max(v0, 25 theta) (at most 1023 rows), so the
  *      exact inversion -- thousands of cycles, run by the whole wavefront --
  *      is left for u >= 0.999 and essentially never for the row count.
- *  AUTOTUNE  the first pricing call of a context that meets a lockstep
- *      European kernel (per path precision, factor count) times its
- *      launch shapes -- 256, 128 or 64 streams per work unit (block size), strike
- *      loop over staged samples unrolled by 4 or 1, 4 or 2 persistent blocks
- *      per resident slot -- on trial launches of that call's own jobs with an
+ *  AUTOTUNE  the first pricing call of a context that meets a path kernel
+ *      (per path precision, factor count, lockstep width, path dependence)
+ *      times its launch shapes -- 256, 128 or 64 streams per work unit
+ *      (block size); for the lockstep European kernels also the strike loop
+ *      over staged samples unrolled by 4 or 1; 4 or 1 persistent blocks per
+ *      resident slot -- on trial launches of that call's own jobs with an
  *      eighth of each stream's samples, keeps the fastest for the context and
  *      uses it from then on. Streams, draws and paths are those of the
  *      default launch; only the FP64 summation order of the per-unit partial
@@ -306,8 +307,9 @@ int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
                              int64_t n, double *out);
 /* Fix the launch shape of one path kernel on this context, as if the
  * autotuner had chosen it: precision MYQL_FP32/FP64, factor count, lockstep
- * width (5 for 0/1-factor, 7 for 2-factor Europeans), path dependence,
- * block (256, 128 or 64), unroll (4 or 1), grid factor (>= 1). MYQL_E_UNSUPPORTED
+ * width (1 for replays; 5 / 7 for 0-1 / 2-factor lockstep Europeans; 7 / 9
+ * path-dependent), path dependence, block (256, 128 or 64), unroll (4; or 1
+ * for lockstep Europeans), grid factor (>= 1). MYQL_E_UNSUPPORTED
  * for a shape that is not compiled. */
 int myql_debug_set_launch_shape(myql_ctx *ctx, int32_t precision, int32_t n_factors,
                                 int32_t lockstep_width, int32_t path_dependent,

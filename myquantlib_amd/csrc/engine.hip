@@ -869,7 +869,8 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     for (auto &kv : members) {
       const GKey &k = kv.first;
       const auto tkey = std::make_tuple(prec, std::get<0>(k), std::get<1>(k), std::get<2>(k));
-      if (!mc_shape_tunable(std::get<0>(k), std::get<1>(k), std::get<2>(k)) || c->tuned.count(tkey)) continue;
+      if (c->tuned.count(tkey)) continue;
+      const int n_shapes = mc_tune_shape_count(std::get<0>(k), std::get<1>(k), std::get<2>(k));
       std::vector<Job<R>> tj = jobs;
       for (Job<R> &j : tj) j.M = j.S * std::max<uint64_t>(1, (j.M / j.S) / 8);
       c->tjobs.reserve(jobs_bytes);
@@ -879,7 +880,8 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
       float best = 0.f;
       McShape best_sh = MC_DEFAULT_SHAPE;
       int best_gf = 4;
-      for (const McShape &sh : MC_TUNE_SHAPES) {
+      for (int si = 0; si < n_shapes; ++si) {
+        const McShape sh = MC_TUNE_SHAPES[si];
         const std::vector<Unit> u = units_for(kv.second, sh.blk);
         c->tunits.reserve(u.size() * sizeof(Unit));
         HIPCHECK(hipMemcpyAsync(c->tunits.p, u.data(), u.size() * sizeof(Unit), hipMemcpyHostToDevice, c->stream));
@@ -1756,9 +1758,9 @@ int myql_debug_set_launch_shape(myql_ctx *c, int32_t precision, int32_t n_factor
     c->err = "bad launch shape arguments";
     return MYQL_E_INVALID_ARG;
   }
-  bool ok = block == myql::MC_DEFAULT_SHAPE.blk && unroll == myql::MC_DEFAULT_SHAPE.unr;
-  if (myql::mc_shape_tunable(n_factors, lockstep_width, pd))
-    for (const myql::McShape &sh : myql::MC_TUNE_SHAPES) ok = ok || (sh.blk == block && sh.unr == unroll);
+  bool ok = false;
+  for (int si = 0; si < myql::mc_tune_shape_count(n_factors, lockstep_width, pd); ++si)
+    ok = ok || (myql::MC_TUNE_SHAPES[si].blk == block && myql::MC_TUNE_SHAPES[si].unr == unroll);
   if (!ok) {
     c->err = "launch shape not compiled for this kernel";
     return MYQL_E_UNSUPPORTED;

@@ -7,17 +7,16 @@
 namespace myql {
 
 // (factor count, lockstep width, path dependence, block, unroll): every
-// kernel at the default shape, the lockstep European kernels also at the
-// other shapes of MC_TUNE_SHAPES.
+// kernel at blocks 256, 128, 64 with unroll 4, the lockstep European kernels
+// also with unroll 1 (MC_TUNE_SHAPES).
+#define MYQL_MC_KERNELS(X, B_, U_)                                                                      \
+  X(0, 1, false, B_, U_) X(0, 5, false, B_, U_) X(1, 1, false, B_, U_) X(1, 5, false, B_, U_)           \
+  X(2, 1, false, B_, U_) X(2, 7, false, B_, U_) X(0, 1, true, B_, U_) X(0, 7, true, B_, U_)             \
+  X(1, 1, true, B_, U_) X(1, 7, true, B_, U_) X(2, 1, true, B_, U_) X(2, 9, true, B_, U_)
+#define MYQL_MC_LOCKSTEP_EURO(X, B_, U_) X(0, 5, false, B_, U_) X(1, 5, false, B_, U_) X(2, 7, false, B_, U_)
 #define MYQL_MC_CASES(X)                                                                                  \
-  X(0, 1, false, 256, 4) X(0, 5, false, 256, 4) X(1, 1, false, 256, 4) X(1, 5, false, 256, 4)             \
-  X(2, 1, false, 256, 4) X(2, 7, false, 256, 4) X(0, 1, true, 256, 4) X(0, 7, true, 256, 4)               \
-  X(1, 1, true, 256, 4) X(1, 7, true, 256, 4) X(2, 1, true, 256, 4) X(2, 9, true, 256, 4)                 \
-  X(0, 5, false, 256, 1) X(1, 5, false, 256, 1) X(2, 7, false, 256, 1)                                    \
-  X(0, 5, false, 128, 4) X(1, 5, false, 128, 4) X(2, 7, false, 128, 4)                                    \
-  X(0, 5, false, 128, 1) X(1, 5, false, 128, 1) X(2, 7, false, 128, 1)                                    \
-  X(0, 5, false, 64, 4) X(1, 5, false, 64, 4) X(2, 7, false, 64, 4)                                       \
-  X(0, 5, false, 64, 1) X(1, 5, false, 64, 1) X(2, 7, false, 64, 1)
+  MYQL_MC_KERNELS(X, 256, 4) MYQL_MC_KERNELS(X, 128, 4) MYQL_MC_KERNELS(X, 64, 4)                       \
+  MYQL_MC_LOCKSTEP_EURO(X, 256, 1) MYQL_MC_LOCKSTEP_EURO(X, 128, 1) MYQL_MC_LOCKSTEP_EURO(X, 64, 1)
 
 template <>
 hipError_t launch_mc<MYQL_R>(int nf, int nv, bool pd, McShape shape, const Job<MYQL_R> *jobs, const Unit *units,
@@ -53,5 +52,7 @@ int mc_blocks_per_cu<MYQL_R>(int nf, int nv, bool pd, McShape shape, size_t smem
 }
 
 #undef MYQL_MC_CASES
+#undef MYQL_MC_KERNELS
+#undef MYQL_MC_LOCKSTEP_EURO
 
 }  // namespace myql

@@ -12,15 +12,18 @@ namespace myql {
 bool mc_supported(int nf, int nv, bool pd);
 
 // Launch shape of the path kernel: streams per work unit (= lanes per block)
-// and the unroll of the strike loop over staged samples. Every kernel has
-// the default shape; the lockstep European kernels (the strip workloads)
-// also have the others, among which the autotuner picks.
+// and the unroll of the strike loop over staged samples. Every kernel is
+// compiled at three block sizes with the loop unrolled by 4 (the first,
+// 256/4, is the default shape); the lockstep European kernels (the strip
+// workloads, whose strike loop dominates the staging phase) also with the
+// loop not unrolled. The autotuner picks among them.
 struct McShape {
   int blk, unr;
 };
 constexpr McShape MC_DEFAULT_SHAPE{BLOCK, 4};
-constexpr McShape MC_TUNE_SHAPES[6] = {{256, 4}, {256, 1}, {128, 4}, {128, 1}, {64, 4}, {64, 1}};
-inline bool mc_shape_tunable(int nf, int nv, bool pd) { return !pd && nv > 1 && nf >= 0 && nf <= 2; }
+constexpr McShape MC_TUNE_SHAPES[6] = {{256, 4}, {128, 4}, {64, 4}, {256, 1}, {128, 1}, {64, 1}};
+// number of leading MC_TUNE_SHAPES compiled for a kernel
+inline int mc_tune_shape_count(int nf, int nv, bool pd) { return (!pd && nv > 1 && nf >= 0 && nf <= 2) ? 6 : 3; }
 
 // pd: path-dependent tracker kernels; accg: 2*NSTAT*shape.blk doubles per unit.
 template <typename R>

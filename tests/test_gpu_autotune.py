@@ -71,3 +71,41 @@ def test_autotuned_strip_matches_default_and_repeats(engine):
                 assert np.array_equal(np.asarray(ra[f]), np.asarray(rb[f])), f
     finally:
         eng.close()
+
+
+def _case(kind):
+    h1, h2 = Heston(2.0, 0.04, 0.3, -0.7, 0.04), Heston(1.0, 0.03, 0.9, -0.5, 0.02)
+    if kind == "asian_lockstep":      # 1 factor, path-dependent lockstep (7 slots)
+        m, sch = make_model(1, [h1]), [capi.SCHEME_NV]
+        inst = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[90.0, 100.0, 110.0], maturity=1.0)
+        mode, key = capi.GREEK_FULL, (1, 7, True)
+    elif kind == "barrier_replay":    # 2 factors, Exact: one slot per replay
+        m, sch = make_model(2, [h1, h2]), [capi.SCHEME_NV, capi.SCHEME_EXACT]
+        inst = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0,
+                              barrier_dir=capi.BARRIER_DOWN, barrier_action=capi.BARRIER_OUT)
+        mode, key = capi.GREEK_NONE, (2, 1, True)
+    else:                             # 0 factors, European replay
+        m, sch = make_model(0, [], vol=0.2), []
+        inst = InstrumentSpec(strikes=[80.0, 100.0, 120.0], maturity=1.0)
+        mode, key = capi.GREEK_NONE, (0, 1, False)
+    cfg = mc_config(1 << 15, 60, seed=5, num_streams=1500)
+    return (m, sch, inst, mode, cfg, 100.0, 0.02, 0.0), key
+
+
+@pytest.mark.parametrize("kind", ["asian_lockstep", "barrier_replay", "european_0f"])
+def test_other_kernels_at_every_block_size(engine, kind):
+    job, (nf, nv, pd) = _case(kind)
+    ref = engine.mc_batch([job])
+    fields = FIELDS if job[3] == capi.GREEK_FULL else ["price", "price_std_err"]
+    eng = Engine(0)
+    try:
+        for blk in (128, 64):
+            eng.set_launch_shape(capi.FP64, nf, nv, pd, blk, 4, 4)
+            got = eng.mc_batch([job])
+            for f in fields:
+                x, y = np.atleast_2d(np.asarray(got[0][f], float)), np.atleast_2d(np.asarray(ref[0][f], float))
+                assert np.max(np.abs(x - y)) <= 1e-12 * max(float(np.max(np.abs(y))), 1e-300), (blk, f)
+        with pytest.raises(Exception):
+            eng.set_launch_shape(capi.FP64, nf, nv, pd, 64, 1, 4)   # unroll 1 only for lockstep Europeans
+    finally:
+        eng.close()
