@@ -390,6 +390,20 @@ __device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, do
   return giles_asym_only(lamk, W, n) ? n : -1;
 }
 
+// Giles' count for a single slot with the rare branches packed across the
+// wavefront (giles_rare); the same count as poissinv. Every active lane of
+// the wave must reach the call together.
+__device__ __forceinline__ int poisson_count_packed(double lam, double u) {
+  const double W = (lam > 4.0 && u > 0.0) ? giles_w(u) : 0.0;
+  bool rare = false;
+  int Ns[1] = {static_cast<int>(poissinv_w<true>(lam, u, W, &rare))};
+  if (__any(rare)) {
+    const double lams[1] = {lam};
+    giles_rare<1>(lams, u, rare ? 1u : 0u, Ns);
+  }
+  return Ns[0];
+}
+
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
 template <typename R, int NV, int NFX>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
@@ -427,7 +441,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     //    full inversion on wave-packed items (giles_rare).
     int Ns[NV];
     if constexpr (NV == 1) {
-      Ns[0] = static_cast<int>(poissinv(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1));
+      Ns[0] = poisson_count_packed(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1);
     } else {
       double lams[NV];
       bool big = false;
@@ -491,7 +505,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
   } else {  // Exact (Poisson-gamma mixture); variable draws, so never lockstep
     if constexpr (NV == 1) {
       const double lam = P[0]->Kd[f][0] * static_cast<double>(v[0][f]);
-      const int N = static_cast<int>(poissinv(lam, uniform(g)));
+      const int N = poisson_count_packed(lam, uniform(g));
       vn[0][f] = gamma_draw<R>(P[0]->Kd[f][1], N, g) * P[0]->Kr[f][2];
     }
   }
