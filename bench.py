@@ -254,8 +254,11 @@ def batched_leg(eng, n_sets: int = 1024, steps: int = 252, flags: int = 0):
     only, FP32 paths; the 2,048 pricing calls go to the device as one batch
     (one persistent launch per kernel instantiation). All sets use seed 42,
     so a set's two instruments see identical draws, as two reference calls
-    with the same seed would. With flags = FLAG_AUTOTUNE the warm-up call (the
-    first 128 sets) also picks the kernel's launch shape."""
+    with the same seed would; the engine detects that and simulates each
+    set's paths once, pricing the Asian from the barrier job's staged samples
+    (path_steps counts both calls, as the reference runs them). With flags =
+    FLAG_AUTOTUNE the warm-up call (the first 128 sets) also picks the
+    kernel's launch shape."""
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import InstrumentSpec, make_model, mc_config
     sets = double_heston_sets(n_sets)

@@ -311,6 +311,7 @@ struct myql_ctx {
   };
   std::map<std::tuple<int, int, int, bool>, Tuned> tuned;
   DevBuf tjobs, tunits, tcounter;
+  DevBuf vaccg;  // accumulators of fused (view) jobs
 };
 
 namespace {
@@ -559,6 +560,24 @@ struct JobSpec {
   int k0, nk;  // strike chunk [k0, k0+nk) of the instrument
 };
 
+// True when the two calls simulate identical paths: same model, schemes,
+// config (paths, steps, seed, bumps, precision, streams, table rows), spot,
+// rates, maturity and Greek mode, neither in flight. Their instruments may
+// differ; the paths do not depend on them.
+bool same_paths(const JobSpec &x, const JobSpec &y) {
+  const myql_mc_config &a = *x.cfg, &b = *y.cfg;
+  const int nf = x.model->n_factors;
+  if (std::memcmp(x.model, y.model, sizeof(myql_model)) != 0) return false;
+  for (int f = 0; f < nf; ++f)
+    if (x.schemes[f] != y.schemes[f]) return false;
+  const uint32_t keep = ~static_cast<uint32_t>(MYQL_FLAG_AUTOTUNE);
+  return a.num_paths == b.num_paths && a.time_steps == b.time_steps && a.seed == b.seed && a.fd_bump == b.fd_bump &&
+         a.vol_bump == b.vol_bump && a.t_bump == b.t_bump && a.r_bump == b.r_bump && a.precision == b.precision &&
+         a.stream_layout == b.stream_layout && a.num_streams == b.num_streams && (a.flags & keep) == (b.flags & keep) &&
+         x.S0 == y.S0 && x.r == y.r && x.q == y.q && x.mode == y.mode && x.instr->maturity == y.instr->maturity &&
+         x.instr->t_elapsed == 0.0 && y.instr->t_elapsed == 0.0;
+}
+
 struct BuiltJob {
   int nf, nv;        // kernel instantiation
   bool pd;
@@ -784,6 +803,29 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   std::vector<double> strikes;
   for (size_t i = 0; i < specs.size(); ++i) built.push_back(build_job<R>(c, specs[i], jobs[i], strikes));
 
+  // Fused instruments: an arithmetic Asian whose paths equal a barrier
+  // call's is priced from that call's staged samples (the simulation keeps
+  // both trackers' state) instead of simulating the same paths again.
+  std::vector<char> is_view(jobs.size(), 0);
+  {
+    std::vector<char> taken(jobs.size(), 0);
+    for (size_t b = 0; b < jobs.size(); ++b) {
+      if (taken[b] || built[b].spec.instr->kind != MYQL_INSTR_BARRIER) continue;
+      for (size_t a = 0; a < jobs.size(); ++a) {
+        if (a == b || taken[a] || built[a].spec.instr->kind != MYQL_INSTR_ARITH_ASIAN) continue;
+        if (built[a].nf != built[b].nf || built[a].nv != built[b].nv || built[a].pd != built[b].pd ||
+            !same_paths(built[a].spec, built[b].spec))
+          continue;
+        taken[a] = taken[b] = 1;
+        is_view[a] = 1;
+        jobs[b].fuse_asian = 1;
+        jobs[b].view = static_cast<int>(a) + 1;
+        jobs[b].n_fields = std::max(jobs[b].n_fields, jobs[a].n_fields);
+        break;
+      }
+    }
+  }
+
   // kernel groups (one persistent launch each) and their launch shapes:
   // the tuned shape where the autotuner has chosen one, else the default
   using GKey = std::tuple<int, int, bool>;
@@ -824,7 +866,10 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   const size_t jobs_bytes = jobs.size() * sizeof(Job<R>);
   c->jobs.reserve(jobs_bytes + strike_bytes + 256);
   double *d_strikes = reinterpret_cast<double *>(static_cast<char *>(c->jobs.p) + ((jobs_bytes + 255) / 256) * 256);
+  if (std::find(is_view.begin(), is_view.end(), 1) != is_view.end())
+    c->vaccg.reserve(static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double));
   for (size_t i = 0; i < jobs.size(); ++i) {
+    if (jobs[i].view) jobs[i].view_accg = c->vaccg.as<double>();
     jobs[i].strikes = d_strikes + reinterpret_cast<size_t>(jobs[i].strikes);
     // each job indexes its partials from its own base with first_slot = 0
     jobs[i].partial = c->partial.as<double>() + pbase[i];
@@ -837,8 +882,9 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   // per-unit accumulators: 2*NSTAT*blk doubles per unit of blk streams
   c->accg.reserve(static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double));
 
-  std::map<GKey, std::vector<size_t>> members;
-  for (size_t i = 0; i < jobs.size(); ++i) members[gkey(i)].push_back(i);
+  std::map<GKey, std::vector<size_t>> members;  // jobs with work units (views have none)
+  for (size_t i = 0; i < jobs.size(); ++i)
+    if (!is_view[i]) members[gkey(i)].push_back(i);
   // units of one group for block size blk, most expensive job first
   auto units_for = [&](const std::vector<size_t> &idx, int blk) {
     std::vector<Unit> u;

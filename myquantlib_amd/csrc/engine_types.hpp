@@ -74,6 +74,15 @@ struct Job {
   int nk, nkp;                     // strikes in this job, padded to a power of 2 (<= 64)
   double *partial;                 // per unit: 2*NSTAT x nk sums
   int first_slot;
+  // Fused instruments: a job whose paths are identical to this job's (same
+  // model, config, seed, maturity, Greek mode) is priced from this job's
+  // staged samples instead of simulating them again. fuse_asian = 1: the
+  // simulation also keeps the arithmetic-Asian state (this job a barrier,
+  // the view an arithmetic Asian); view = 1 + that job's index (0: none);
+  // view_accg: the view's per-lane accumulators, indexed by this launch's
+  // unit like the job's own.
+  int fuse_asian, view;
+  double *view_accg;
   VarParams<R> var[MAXV];
 };
 

@@ -80,6 +80,10 @@ __device__ __forceinline__ void trk_init(const Job<R> *__restrict__ J, Trk<R> &s
       }
       break;
   }
+  if (J->fuse_asian) {  // a fused arithmetic Asian's state, as case 2
+    s.sumlog = static_cast<R>(J->init_sumlog) + R(0.5) * x * dt;
+    s.sumS = static_cast<R>(J->init_sumS) + R(0.5) * dexp<R>(x) * dt;
+  }
 }
 
 template <typename R>
@@ -126,6 +130,10 @@ __device__ __forceinline__ void trk_update(const Job<R> *__restrict__ J, Trk<R> 
       }
       break;
   }
+  if (J->fuse_asian) {
+    s.sumlog += x * dt;
+    s.sumS += dexp<R>(x) * dt;
+  }
 }
 
 // Finalize and stage one variant's terminal data in LDS.
@@ -149,6 +157,10 @@ __device__ __forceinline__ void finalize_stage(const Job<R> *__restrict__ J, con
       aux = dexp<R>(e);
       break;
     }
+  }
+  if (J->fuse_asian) {
+    const R sS = s.sumS - R(0.5) * dexp<R>(x) * dt;
+    aux = sS / (T + static_cast<R>(J->t_elapsed));
   }
   stage[(F_SPOT * MAXV + v) * BLK + tid] = dexp<R>(x);
   if (J->n_fields > F_AUX) stage[(F_AUX * MAXV + v) * BLK + tid] = aux;
@@ -802,6 +814,31 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   }
 }
 
+// Per-unit partial sums of job J from its lanes' accumulator slots: the
+// sample groups of each strike by wave butterfly, then the waves in order;
+// one FP64 partial per unit, accumulator and strike.
+template <typename R, int BLK>
+__device__ __forceinline__ void unit_partials(const Job<R> *__restrict__ J, const double *__restrict__ accg,
+                                              int blk, bool any, double *__restrict__ s_red, int tid) {
+  const int nkp = J->nkp;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int mode = J->mode;
+  const int na = mode == 2 ? 2 * NSTAT : (mode == 1 ? 2 * (ST_GAMMA + 1) : 2);
+  double *out = J->partial + static_cast<size_t>(J->first_slot + blk) * (2 * NSTAT) * J->nk;
+  for (int a = 0; a < 2 * NSTAT; ++a) {
+    double x = (a < na && any) ? accg[a * BLK + tid] : 0.0;
+    for (int off = nkp; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
+    s_red[tid] = x;
+    __syncthreads();
+    if (wave == 0 && lane < J->nk) {
+      double t = 0.0;
+      for (int w = 0; w < BLK / 64; ++w) t += s_red[w * 64 + lane];
+      out[a * J->nk + lane] = t;
+    }
+    __syncthreads();
+  }
+}
+
 // Persistent work-queue kernel. `counter` must be zero at launch; `accg`
 // holds 2*NSTAT*BLK doubles per work unit of BLK streams; UNR unrolls the
 // strike loop over staged samples.
@@ -863,28 +900,16 @@ __global__ void __launch_bounds__(BLK) mc_paths_kernel(const Job<R> *__restrict_
       s_valid[tid] = valid ? 1 : 0;
       __syncthreads();
       accumulate<R, BLK, UNR>(J, stage, s_valid, tid, accg, it == 0);
+      if (J->view)
+        accumulate<R, BLK, UNR>(jobs + (J->view - 1), stage, s_valid, tid,
+                                J->view_accg + static_cast<size_t>(u) * (2 * NSTAT * BLK), it == 0);
       __syncthreads();
     }
 
-    // reduce the sample groups of each strike: wave butterfly, then the 4
-    // waves in order; one FP64 partial per unit, accumulator and strike
-    const int nkp = J->nkp;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int mode = J->mode;
-    const int na = mode == 2 ? 2 * NSTAT : (mode == 1 ? 2 * (ST_GAMMA + 1) : 2);
-    double *out = J->partial + static_cast<size_t>(J->first_slot + blk) * (2 * NSTAT) * J->nk;
-    for (int a = 0; a < 2 * NSTAT; ++a) {
-      double x = (a < na && n_it > 0) ? accg[a * BLK + tid] : 0.0;
-      for (int off = nkp; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
-      s_red[tid] = x;
-      __syncthreads();
-      if (wave == 0 && lane < J->nk) {
-        double t = 0.0;
-        for (int w = 0; w < BLK / 64; ++w) t += s_red[w * 64 + lane];
-        out[a * J->nk + lane] = t;
-      }
-      __syncthreads();
-    }
+    unit_partials<R, BLK>(J, accg, blk, n_it > 0, s_red, tid);
+    if (J->view)
+      unit_partials<R, BLK>(jobs + (J->view - 1), J->view_accg + static_cast<size_t>(u) * (2 * NSTAT * BLK), blk,
+                            n_it > 0, s_red, tid);
   }
 }
 
