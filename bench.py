@@ -289,6 +289,29 @@ def batched_leg(eng, n_sets: int = 1024, steps: int = 252, flags: int = 0):
             "price_range": [float(min(r["price"][0] for r in res)), float(max(r["price"][0] for r in res))]}
 
 
+def fourier_check(eng, mc_results):
+    """The strip's 512 options with Full Greeks by the GPU Gil-Pelaez pricer
+    (one launch), and the FP32 Monte Carlo strip against it: z = (MC - Fourier)
+    / MC standard error per option with price >= 0.1."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec
+    FJ = [(model(), InstrumentSpec(strikes=STRIKES, maturity=T), capi.GREEK_FULL, S0, R, Q) for T in MATURITIES]
+    eng.fourier_batch(FJ)
+    t0 = time.perf_counter()
+    fr = eng.fourier_batch(FJ)
+    ms = 1e3 * (time.perf_counter() - t0)
+    z = []
+    for mc, fo in zip(mc_results, fr):
+        ok = fo["price"] >= 0.1
+        z.extend(((mc["price"] - fo["price"]) / mc["price_std_err"])[ok])
+    z = np.abs(np.asarray(z))
+    print(f"# fourier strip (512 options, Full): {ms:.2f} ms; MC fp32 vs Fourier max |z| {z.max():.2f}, "
+          f"{np.mean(z < 3):.3f} within 3 se", file=sys.stderr, flush=True)
+    return {"options": 512, "greek_mode": "Full", "ms": ms, "device_ms": fr[0]["device_ms"],
+            "mc_fp32_vs_fourier_max_abs_z": float(z.max()), "mc_fp32_vs_fourier_frac_within_3se": float(np.mean(z < 3)),
+            "mc_fp32_vs_fourier_rms_z": float(np.sqrt(np.mean(z * z)))}
+
+
 def mlmc_leg(eng):
     """Multilevel MC of an up-and-out call under Heston (kappa 2, theta 0.04,
     sigma 0.3, rho -0.7, v0 0.04; SchemeNV), K 100, B 130, T 1: levels 0..6
@@ -334,6 +357,7 @@ def main():
     ap.add_argument("--no-cv", action="store_true", help="skip the price-only and control-variate legs")
     ap.add_argument("--no-mlmc", action="store_true", help="skip the multilevel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Double-Heston leg")
+    ap.add_argument("--no-fourier", action="store_true", help="skip the Gil-Pelaez leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the strip kernel at its default launch shape (256-stream units, "
                          "strike loop unrolled by 4, grid 4x) instead of autotuning it")
@@ -387,6 +411,8 @@ def main():
     t32, worst32 = ms_to_target(res, ms_per_step)
     print(f"# fp32 strip: {ms_per_step:.1f} ms/step, device {np.mean(dev_ms):.1f} ms, "
           f"worst rel se {worst32:.2e}", file=sys.stderr, flush=True)
+
+    fourier_leg = None if args.no_fourier else fourier_check(eng, res)
 
     ms64 = None
     if not args.no_fp64:
@@ -453,6 +479,7 @@ def main():
         "learned_cv": cv or None,
         "mlmc": mlmc,
         "batched_double_heston": batched,
+        "fourier": fourier_leg,
         "sample_steps_per_s": value / VARIANTS,
         "executed_variant_fraction": EXECUTED_VARIANTS / VARIANTS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,

@@ -22,6 +22,9 @@
  *   myql_create/destroy      (no counterpart: the reference has no device
  *                            state; a context owns a HIP stream, cached
  *                            stream states and NCI tables)
+ *   myql_fourier_calculate_batch  FourierPricer<...>::calculate, one call per
+ *                            job (reference: include/myql/pricers/fourier/
+ *                             FourierPricer.hpp:34-407)
  *   myql_tuning_report       launch shapes picked by MYQL_FLAG_AUTOTUNE (no
  *                            counterpart: the reference has no launch shape)
  *   myql_cv_train,           learned martingale control variate (no
@@ -236,6 +239,38 @@ int myql_mlmc_calculate(myql_ctx *ctx, const myql_model *model,
                         const int32_t schemes[2], const myql_instrument *instr,
                         int32_t greek_mode, const myql_mlmc_config *cfg,
                         double S0, double r, double q, myql_mlmc_result *res);
+
+/* ---- Gil-Pelaez Fourier pricer ------------------------------------------
+ * FourierPricer<Model, EuropeanOption<Payoff>, Mode>::calculate(S0, r, q,
+ * instr) (reference: include/myql/pricers/fourier/FourierPricer.hpp:34-407,
+ * engines/fourier/FourierEngine.hpp, kernels/GilPelaezKernel.hpp). European
+ * vanilla, cash-or-nothing and asset-or-nothing calls and puts, scalar or
+ * strike strip, any ASVJ model; price, and per Greek mode delta, gamma,
+ * vega[2], theta (dP/dT), rho, vanna[2], charm exactly as the reference
+ * assembles them from its integrals. */
+/* FourierEngine::Config (FourierEngine.hpp:8-11): start_bound_guess is the
+ * first point of the integration-bound search (start / sqrt(T) for T < 1,
+ * then x1.5 until both price integrands are below 1e-12). tolerance is the
+ * reference's adaptive-Simpson target; the engine integrates by composite
+ * Gauss-Legendre (64 geometric panels x 16 nodes), whose error is well below
+ * it, so the field is accepted and not used. */
+typedef struct {
+  double tolerance;
+  double start_bound_guess;
+} myql_fourier_config;
+
+typedef struct {
+  const myql_model *model;
+  const myql_instrument *instrument; /* kind MYQL_INSTR_EUROPEAN */
+  int32_t greek_mode;
+  const myql_fourier_config *config; /* NULL: {1e-8, 20} */
+  double S0, r, q;
+  myql_mc_result *result; /* price and Greeks per strike; the *_std_err
+                             arrays and total_steps are left alone */
+} myql_fourier_job;
+
+/* All jobs' strikes in one launch (one wavefront per option). */
+int myql_fourier_calculate_batch(myql_ctx *ctx, const myql_fourier_job *jobs, int64_t n_jobs);
 
 /* ---- learned martingale control variate ---------------------------------
  * No counterpart in the reference (its README.md:332 lists control variates

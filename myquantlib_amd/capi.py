@@ -75,6 +75,17 @@ class MCJob(C.Structure):
                 ("q", C.c_double), ("result", C.POINTER(MCResult))]
 
 
+class FourierConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("start_bound_guess", C.c_double)]
+
+
+class FourierJob(C.Structure):
+    _fields_ = [("model", C.POINTER(Model)), ("instrument", C.POINTER(Instrument)),
+                ("greek_mode", C.c_int32), ("config", C.POINTER(FourierConfig)),
+                ("S0", C.c_double), ("r", C.c_double), ("q", C.c_double),
+                ("result", C.POINTER(MCResult))]
+
+
 class MLMCConfig(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("L_min", C.c_uint64), ("L_max", C.c_uint64),
                 ("N0", C.c_uint64), ("base_steps", C.c_uint64), ("alpha", C.c_double),
@@ -113,7 +124,7 @@ class CVResult(C.Structure):
 # Every extern "C" symbol declared in include/myql_capi.h.
 EXPORTED = [
     "myql_create", "myql_destroy", "myql_last_error", "myql_tuning_report", "myql_mc_calculate",
-    "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_cv_train",
+    "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_fourier_calculate_batch", "myql_cv_train",
     "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
     "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_set_launch_shape",
     "myql_debug_cv_input_grad",
@@ -137,6 +148,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_destroy.argtypes = [vp]
     lib.myql_last_error.restype = C.c_char_p
     lib.myql_last_error.argtypes = [vp]
+    lib.myql_fourier_calculate_batch.restype = C.c_int
+    lib.myql_fourier_calculate_batch.argtypes = [vp, C.POINTER(FourierJob), C.c_int64]
     lib.myql_tuning_report.restype = C.c_int
     lib.myql_tuning_report.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.myql_mc_calculate.restype = C.c_int

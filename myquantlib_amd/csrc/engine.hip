@@ -24,6 +24,7 @@
 #include "mlmc_kernel.hpp"
 #include "cv_launch.hpp"
 #include "cv_types.hpp"
+#include "fourier_launch.hpp"
 
 using namespace myql;
 
@@ -312,6 +313,7 @@ struct myql_ctx {
   std::map<std::tuple<int, int, int, bool>, Tuned> tuned;
   DevBuf tjobs, tunits, tcounter;
   DevBuf vaccg;  // accumulators of fused (view) jobs
+  DevBuf fopts, fout;  // Fourier pricer options and integrals
 };
 
 namespace {
@@ -1598,6 +1600,165 @@ void run_specs(myql_ctx *c, const std::vector<JobSpec> &in_specs) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Gil-Pelaez Fourier pricer: options on the device, results assembled on the
+// host as FourierPricer::price_scalar does (FourierPricer.hpp:105-404).
+// ---------------------------------------------------------------------------
+namespace {
+
+void fourier_validate(const myql_fourier_job &j) {
+  if (!j.model || !j.instrument || !j.result) throw Fail(MYQL_E_INVALID_ARG, "null argument");
+  const myql_model &m = *j.model;
+  const myql_instrument &in = *j.instrument;
+  if (m.n_factors < 0 || m.n_factors > 2) throw Fail(MYQL_E_INVALID_ARG, "n_factors must be 0, 1 or 2");
+  if (m.jump_kind < MYQL_JUMP_NONE || m.jump_kind > MYQL_JUMP_KOU) throw Fail(MYQL_E_INVALID_ARG, "bad jump kind");
+  if (m.jump_kind == MYQL_JUMP_KOU && !(m.jump[2] > 1.0)) throw Fail(MYQL_E_INVALID_ARG, "Kou eta1 must exceed 1");
+  for (int f = 0; f < m.n_factors; ++f)
+    if (!(m.heston[f].sigma > 0.0) || !(m.heston[f].v0 >= 0.0)) throw Fail(MYQL_E_INVALID_ARG, "bad Heston factor");
+  if (in.kind != MYQL_INSTR_EUROPEAN) throw Fail(MYQL_E_UNSUPPORTED, "the Fourier pricer prices European options");
+  if (in.payoff < MYQL_PAYOFF_VANILLA || in.payoff > MYQL_PAYOFF_ASSET)
+    throw Fail(MYQL_E_INVALID_ARG, "bad payoff");
+  if (!(in.maturity > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "maturity must be positive");
+  if (in.n_strikes <= 0 || !in.strikes) throw Fail(MYQL_E_INVALID_ARG, "no strikes");
+  for (int64_t k = 0; k < in.n_strikes; ++k)
+    if (!(in.strikes[k] > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "strikes must be positive");
+  if (!(j.S0 > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "S0 must be positive");
+  if (j.greek_mode < MYQL_GREEK_NONE || j.greek_mode > MYQL_GREEK_FULL) throw Fail(MYQL_E_INVALID_ARG, "bad greek mode");
+}
+
+void fourier_run(myql_ctx *c, const myql_fourier_job *jobs, int64_t n_jobs) {
+  std::vector<FourierOption> opts;
+  for (int64_t ji = 0; ji < n_jobs; ++ji) {
+    const myql_fourier_job &j = jobs[ji];
+    fourier_validate(j);
+    const myql_model &m = *j.model;
+    const myql_instrument &in = *j.instrument;
+    const double start = j.config ? j.config->start_bound_guess : 20.0;
+    FourierOption o{};
+    o.model.nf = m.n_factors;
+    o.model.jump_kind = m.jump_kind;
+    o.model.vol = m.vol;
+    for (int f = 0; f < 2; ++f) {
+      const myql_heston_params &h = m.heston[f];
+      const double hp[5] = {h.kappa, h.theta, h.sigma, h.rho, h.v0};
+      for (int e = 0; e < 5; ++e) o.model.h[f][e] = hp[e];
+    }
+    for (int e = 0; e < 4; ++e) o.model.jump[e] = m.jump[e];
+    o.T = in.maturity;
+    o.drift = (j.r - j.q) * in.maturity;
+    o.start_bound = start / (in.maturity < 1.0 ? std::sqrt(in.maturity) : 1.0);
+    o.mode = j.greek_mode;
+    for (int64_t k = 0; k < in.n_strikes; ++k) {
+      o.logK = std::log(in.strikes[k] / j.S0);
+      opts.push_back(o);
+    }
+  }
+  const int n = static_cast<int>(opts.size());
+  const int ni = fourier_n_integrals();
+  c->fopts.reserve(std::max<size_t>(1, opts.size()) * sizeof(FourierOption));
+  c->fout.reserve(std::max<size_t>(1, opts.size()) * ni * sizeof(double));
+  HIPCHECK(hipMemcpyAsync(c->fopts.p, opts.data(), opts.size() * sizeof(FourierOption), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  HIPCHECK(launch_fourier(c->fopts.as<FourierOption>(), n, c->fout.as<double>(), c->stream));
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  std::vector<double> I(opts.size() * ni);
+  HIPCHECK(hipMemcpyAsync(I.data(), c->fout.p, I.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+
+  // integral layout (fourier_launch.hpp): [P1 | P2] x {P, Dx, Dxx, V0, V1, T, VDx0, VDx1, TDx}
+  enum { P = 0, DX, DXX, V0, V1, TH, VDX0, VDX1, TDX, NI };
+  size_t oi = 0;
+  for (int64_t ji = 0; ji < n_jobs; ++ji) {
+    const myql_fourier_job &j = jobs[ji];
+    const myql_model &m = *j.model;
+    const myql_instrument &in = *j.instrument;
+    myql_mc_result &res = *j.result;
+    const double S0 = j.S0, r = j.r, q = j.q, T = in.maturity;
+    const double df_r = std::exp(-r * T), df_q = std::exp(-q * T);
+    const bool call = in.option_type == MYQL_CALL;
+    const double sign = call ? 1.0 : -1.0;
+    const int nf = m.n_factors;
+    double chain[2] = {1.0, 1.0};  // vega per unit of sqrt(v0) (AffineTraits vega_chain_factor)
+    for (int f = 0; f < nf; ++f) chain[f] = 2.0 * std::sqrt(m.heston[f].v0);
+    const bool ess = j.greek_mode != MYQL_GREEK_NONE, full = j.greek_mode == MYQL_GREEK_FULL;
+    for (int64_t k = 0; k < in.n_strikes; ++k, ++oi) {
+      const double K = in.strikes[k];
+      const double *I1 = I.data() + oi * ni, *I2 = I1 + NI;
+      const double P1 = 0.5 + I1[P], P2 = 0.5 + I2[P];
+      auto put = [&](double *arr, double v) {
+        if (arr) arr[k] = v;
+      };
+      double price = 0, delta = 0, gamma = 0, vega[2] = {0, 0}, theta = 0, rho = 0, vanna[2] = {0, 0}, charm = 0;
+      if (in.payoff == MYQL_PAYOFF_VANILLA) {
+        price = call ? std::max(0.0, S0 * df_q * P1 - K * df_r * P2)
+                     : std::max(0.0, K * df_r * (1.0 - P2) - S0 * df_q * (1.0 - P1));
+        delta = call ? df_q * P1 : df_q * (P1 - 1.0);
+        gamma = df_q * I1[DX] / S0;
+        for (int f = 0; f < std::max(nf, 1); ++f) {
+          const double d1 = I1[V0 + f], d2 = I2[V0 + f];
+          vega[f] = (S0 * df_q * d1 - K * df_r * d2) * chain[f];
+          vanna[f] = df_q * d1 * chain[f];
+        }
+        const double dP1dT = I1[TH], dP2dT = I2[TH];
+        // dP/dT. For puts the reference writes K df_r (r (1 - P2) - dP2/dT)
+        // (FourierPricer.hpp:183-184); the T-derivative of K e^{-rT} (1 - P2)
+        // carries -r, which its own Monte Carlo theta estimator agrees with,
+        // so the discount term takes that sign here.
+        theta = call ? S0 * df_q * (-q * P1 + dP1dT) + K * df_r * (r * P2 - dP2dT)
+                     : K * df_r * (-r * (1.0 - P2) - dP2dT) + S0 * df_q * (q * (1.0 - P1) + dP1dT);
+        const double dP1dr = T * I1[DX], dP2dr = T * I2[DX];
+        const double disc = call ? T * K * df_r * P2 : -T * K * df_r * (1.0 - P2);
+        rho = S0 * df_q * dP1dr + disc - K * df_r * dP2dr;
+        charm = call ? df_q * (-q * P1 + dP1dT) : df_q * (-q * (P1 - 1.0) + dP1dT);
+      } else if (in.payoff == MYQL_PAYOFF_CASH) {
+        const double base = call ? P2 : 1.0 - P2;
+        price = df_r * base;
+        delta = sign * df_r * I2[DX] / S0;
+        gamma = sign * df_r * (I2[DXX] - I2[DX]) / (S0 * S0);
+        for (int f = 0; f < std::max(nf, 1); ++f) {
+          vega[f] = sign * df_r * I2[V0 + f] * chain[f];
+          vanna[f] = sign * df_r * I2[VDX0 + f] / S0 * chain[f];
+        }
+        theta = df_r * (sign * I2[TH] - r * base);
+        rho = df_r * (sign * T * I2[DX] - T * base);
+        charm = sign * df_r * (I2[TDX] - r * I2[DX]) / S0;
+      } else {  // asset-or-nothing
+        const double base = call ? P1 : 1.0 - P1;
+        price = S0 * df_q * base;
+        delta = df_q * (base + sign * I1[DX]);
+        gamma = sign * df_q * (I1[DX] + I1[DXX]) / S0;
+        for (int f = 0; f < std::max(nf, 1); ++f) {
+          vega[f] = S0 * df_q * sign * I1[V0 + f] * chain[f];
+          vanna[f] = df_q * sign * (I1[V0 + f] + I1[VDX0 + f]) * chain[f];
+        }
+        theta = S0 * df_q * (-q * base + sign * I1[TH]);
+        rho = S0 * df_q * sign * T * I1[DX];
+        charm = df_q * (-q * (base + sign * I1[DX]) + sign * I1[TH] + sign * I1[TDX]);
+      }
+      put(res.price, price);
+      if (ess) {
+        put(res.delta, delta);
+        put(res.gamma, gamma);
+      }
+      if (full) {
+        put(res.vega[0], vega[0]);
+        put(res.vega[1], nf >= 2 ? vega[1] : 0.0);
+        put(res.theta, theta);
+        put(res.rho, rho);
+        put(res.vanna[0], vanna[0]);
+        put(res.vanna[1], nf >= 2 ? vanna[1] : 0.0);
+        put(res.charm, charm);
+      }
+    }
+    res.device_ms = ms;
+  }
+}
+
+}  // namespace
+
 namespace myql {
 bool mc_supported(int nf, int nv, bool pd) {
   // lockstep widths: Europeans simulate base, vol bumps and maturity bumps
@@ -1703,6 +1864,19 @@ int myql_mc_calculate_batch(myql_ctx *c, const myql_mc_job *jobs, int64_t n_jobs
       specs.push_back({j.model, j.schemes, j.instrument, j.greek_mode, j.config, j.S0, j.r, j.q, j.result, 0, 0});
     }
     if (!specs.empty()) run_specs(c, specs);
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_fourier_calculate_batch(myql_ctx *c, const myql_fourier_job *jobs, int64_t n_jobs) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) throw Fail(MYQL_E_INVALID_ARG, "bad job list");
+    if (n_jobs > 0) fourier_run(c, jobs, n_jobs);
     c->err.clear();
     return MYQL_OK;
   } catch (const Fail &e) {

@@ -258,6 +258,31 @@ class Engine:
         self._check(rc)
         return [b.as_dict() for b in bufs]
 
+    # ---- Gil-Pelaez Fourier pricer -----------------------------------------
+    def fourier_batch(self, jobs: list, start_bound_guess: float = 20.0, tolerance: float = 1e-8) -> list:
+        """jobs: list of (model, instr_spec, mode, S0, r, q) for European
+        options; one launch for all strikes of all jobs. Result dicts carry
+        price and the Greeks of the mode (std errors stay NaN)."""
+        cfg = capi.FourierConfig(tolerance, start_bound_guess)
+        arr = (capi.FourierJob * len(jobs))()
+        bufs, keep = [], [cfg]
+        for i, (model, instr, mode, S0, r, q) in enumerate(jobs):
+            st = instr.struct()
+            buf = ResultBuffers(len(instr.strikes))
+            keep += [model, st]
+            bufs.append(buf)
+            arr[i].model = C.pointer(model)
+            arr[i].instrument = C.pointer(st)
+            arr[i].greek_mode = mode
+            arr[i].config = C.pointer(cfg)
+            arr[i].S0, arr[i].r, arr[i].q = S0, r, q
+            arr[i].result = C.pointer(buf.struct)
+        self._check(self.lib.myql_fourier_calculate_batch(self.ctx, arr, len(jobs)))
+        return [b.as_dict() for b in bufs]
+
+    def fourier(self, model, instr, mode: int, S0: float, r: float, q: float, **kw) -> dict:
+        return self.fourier_batch([(model, instr, mode, S0, r, q)], **kw)[0]
+
     # ---- learned control variate -----------------------------------------
     def cv_train(self, jobs: list, cfg: "capi.CVConfig") -> "capi.CVNet":
         """Train the control-variate network for the option sets of `jobs`

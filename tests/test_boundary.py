@@ -48,6 +48,8 @@ STRUCTS = {
     "myql_mc_job": capi.MCJob,
     "myql_mlmc_config": capi.MLMCConfig,
     "myql_mlmc_result": capi.MLMCResult,
+    "myql_fourier_config": capi.FourierConfig,
+    "myql_fourier_job": capi.FourierJob,
     "myql_cv_config": capi.CVConfig,
     "myql_cv_net": capi.CVNet,
     "myql_cv_result": capi.CVResult,
@@ -92,6 +94,7 @@ def test_null_context_is_rejected_without_device_work():
     st = lib.myql_mlmc_calculate(None, C.byref(m), schemes, C.byref(inst), 0, C.byref(mcfg), 100.0,
                                  0.0, 0.0, C.byref(mres))
     assert st == capi.E_INVALID_ARG
+    assert lib.myql_fourier_calculate_batch(None, None, 0) == capi.E_INVALID_ARG
     buf = C.create_string_buffer(64)
     assert lib.myql_tuning_report(None, buf, len(buf)) == capi.E_INVALID_ARG
     assert lib.myql_debug_set_launch_shape(None, capi.FP32, 1, 5, 0, 256, 4, 4) == capi.E_INVALID_ARG

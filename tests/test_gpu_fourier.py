@@ -1,0 +1,165 @@
+"""GPU Gil-Pelaez Fourier pricer against the oracle and closed forms.
+
+Prices: the engine's composite Gauss-Legendre integrals against the
+oracle's adaptive QUADPACK integrals of the same characteristic functions
+(itself pinned by Black-Scholes, Merton's series and the Heston
+deterministic-variance limit in test_oracle.py), to 1e-9 of S0. Greeks:
+Black-Scholes closed forms for the zero-factor model (pins every integral
+kernel and the reference's assembly), and central finite differences of the
+oracle's price for Heston, Bates-Kou and double Heston (pins the v0 and T
+derivatives of the Heston exponent).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from myquantlib_amd import capi
+from myquantlib_amd.pricing import Heston, InstrumentSpec, kou, make_model, merton
+from oracle import fourier
+
+pytestmark = pytest.mark.gpu
+
+H1 = Heston(2.0, 0.04, 0.3, -0.7, 0.04)
+H_HI = Heston(2.0, 0.04, 1.0, -0.7, 0.04)
+H2 = Heston(1.0, 0.03, 0.25, -0.5, 0.02)
+S0, R, Q = 100.0, 0.03, 0.01
+STRIKES = [60.0, 80.0, 95.0, 100.0, 105.0, 120.0, 150.0]
+
+MODELS = {
+    "bs": (0, [], 0.25, None),
+    "merton": (0, [], 0.2, merton(0.5, -0.1, 0.15)),
+    "heston": (1, [H1], 0.0, None),
+    "bates": (1, [H_HI], 0.0, merton(0.5, -0.1, 0.15)),
+    "bates_kou": (1, [Heston(2.0, 0.04, 0.5, -0.7, 0.04)], 0.0, kou(1.0, 0.4, 10.0, 5.0)),
+    "double_heston": (2, [H1, H2], 0.0, None),
+}
+
+
+def both(name):
+    nf, hs, vol, jump = MODELS[name]
+    jk, jp = jump if jump else (capi.JUMP_NONE, (0, 0, 0, 0))
+    return make_model(nf, hs, vol=vol, jump_kind=jk, jump=jp), fourier.AffineModel(nf, hs, vol, jk, jp)
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_prices_match_oracle(engine, name):
+    m, fm = both(name)
+    jobs, refs = [], []
+    for T in (0.1, 1.0, 2.5):
+        for payoff, otype in [(capi.PAYOFF_VANILLA, capi.CALL), (capi.PAYOFF_VANILLA, capi.PUT),
+                              (capi.PAYOFF_CASH, capi.CALL), (capi.PAYOFF_ASSET, capi.PUT)]:
+            inst = InstrumentSpec(strikes=STRIKES, maturity=T, payoff=payoff, option_type=otype)
+            jobs.append((m, inst, capi.GREEK_NONE, S0, R, Q))
+            refs.append([fourier.price(fm, S0, K, T, R, Q, payoff, otype) for K in STRIKES])
+    got = engine.fourier_batch(jobs)
+    for g, ref, job in zip(got, refs, jobs):
+        err = np.max(np.abs(g["price"] - np.asarray(ref)))
+        assert err < 1e-9 * S0, (job[1].maturity, job[1].payoff, job[1].option_type, err)
+
+
+def bs_greeks(S, K, T, r, q, vol, call):
+    sq = math.sqrt(T)
+    d1 = (math.log(S / K) + (r - q + 0.5 * vol * vol) * T) / (vol * sq)
+    d2 = d1 - vol * sq
+    n1, N1, N2 = stats.norm.pdf(d1), stats.norm.cdf(d1), stats.norm.cdf(d2)
+    dq, dr = math.exp(-q * T), math.exp(-r * T)
+    dd1_dT = (r - q + 0.5 * vol * vol) / (vol * sq) - d1 / (2.0 * T)
+    if call:
+        price, delta = S * dq * N1 - K * dr * N2, dq * N1
+        theta = S * dq * n1 * vol / (2 * sq) - q * S * dq * N1 + r * K * dr * N2   # dP/dT
+        rho = K * T * dr * N2
+        charm = dq * (-q * N1 + n1 * dd1_dT)
+    else:
+        price, delta = K * dr * (1 - N2) - S * dq * (1 - N1), dq * (N1 - 1)
+        theta = S * dq * n1 * vol / (2 * sq) + q * S * dq * (1 - N1) - r * K * dr * (1 - N2)
+        rho = -K * T * dr * (1 - N2)
+        charm = dq * (-q * (N1 - 1) + n1 * dd1_dT)
+    return dict(price=price, delta=delta, gamma=dq * n1 / (S * vol * sq), vega=S * dq * n1 * sq,
+                theta=theta, rho=rho, vanna=-dq * n1 * d2 / vol, charm=charm)
+
+
+@pytest.mark.parametrize("call", [True, False])
+def test_black_scholes_full_greeks_closed_form(engine, call):
+    m, _ = both("bs")
+    vol = 0.25
+    out = []
+    for T in (0.25, 1.0, 2.0):
+        inst = InstrumentSpec(strikes=STRIKES, maturity=T, option_type=capi.CALL if call else capi.PUT)
+        g = engine.fourier(m, inst, capi.GREEK_FULL, S0, R, Q)
+        for i, K in enumerate(STRIKES):
+            ref = bs_greeks(S0, K, T, R, Q, vol, call)
+            got = dict(price=g["price"][i], delta=g["delta"][i], gamma=g["gamma"][i], vega=g["vega"][0][i],
+                       theta=g["theta"][i], rho=g["rho"][i], vanna=g["vanna"][0][i], charm=g["charm"][i])
+            for k, v in ref.items():
+                out.append((T, K, k, got[k], v))
+                assert abs(got[k] - v) < 1e-8 * max(1.0, abs(v)), (T, K, k, got[k], v)
+        assert np.all(g["vega"][1] == 0.0) and np.all(g["vanna"][1] == 0.0)
+
+
+def test_digital_greeks_closed_form(engine):
+    """Cash-or-nothing call e^{-rT} N(d2): delta, gamma and vega in closed form."""
+    m, _ = both("bs")
+    vol, T = 0.25, 0.8
+    inst = InstrumentSpec(strikes=STRIKES, maturity=T, payoff=capi.PAYOFF_CASH)
+    g = engine.fourier(m, inst, capi.GREEK_FULL, S0, R, Q)
+    sq = math.sqrt(T)
+    for i, K in enumerate(STRIKES):
+        d1 = (math.log(S0 / K) + (R - Q + 0.5 * vol * vol) * T) / (vol * sq)
+        d2 = d1 - vol * sq
+        dr = math.exp(-R * T)
+        n2 = stats.norm.pdf(d2)
+        ref = dict(price=dr * stats.norm.cdf(d2), delta=dr * n2 / (S0 * vol * sq),
+                   gamma=-dr * n2 * d1 / (S0 * S0 * vol * vol * T), vega=-dr * n2 * d1 / vol)
+        got = dict(price=g["price"][i], delta=g["delta"][i], gamma=g["gamma"][i], vega=g["vega"][0][i])
+        for k, v in ref.items():
+            assert abs(got[k] - v) < 1e-9 * max(1.0, abs(v)), (K, k, got[k], v)
+
+
+def fd_greeks(fm, nf, hs, vol, jk, jp, K, T, payoff):
+    """Central differences of the oracle's Fourier price."""
+    def px(S=S0, T=T, r=R, hs=hs, vol=vol):
+        return fourier.price(fourier.AffineModel(nf, hs, vol, jk, jp), S, K, T, r, Q, payoff, 0)
+    p0 = px()
+    hS, hv, hT, hr = 0.05, 1e-4, 1e-4, 1e-4
+    out = dict(price=p0, delta=(px(S=S0 + hS) - px(S=S0 - hS)) / (2 * hS),
+               gamma=(px(S=S0 + hS) - 2 * p0 + px(S=S0 - hS)) / hS ** 2,
+               theta=(px(T=T + hT) - px(T=T - hT)) / (2 * hT), rho=(px(r=R + hr) - px(r=R - hr)) / (2 * hr))
+    for f in range(nf):
+        up = [Heston(*[h.v0 + hv if (j == f and a == "v0") else getattr(h, a)
+                       for a in ("kappa", "theta", "sigma", "rho", "v0")]) for j, h in enumerate(hs)]
+        dn = [Heston(*[h.v0 - hv if (j == f and a == "v0") else getattr(h, a)
+                       for a in ("kappa", "theta", "sigma", "rho", "v0")]) for j, h in enumerate(hs)]
+        out[f"vega{f}"] = (px(hs=up) - px(hs=dn)) / (2 * hv) * 2 * math.sqrt(hs[f].v0)
+    return out
+
+
+@pytest.mark.parametrize("name,payoff", [("heston", capi.PAYOFF_VANILLA), ("bates_kou", capi.PAYOFF_VANILLA),
+                                         ("double_heston", capi.PAYOFF_VANILLA), ("heston", capi.PAYOFF_CASH)])
+def test_greeks_match_oracle_finite_differences(engine, name, payoff):
+    nf, hs, vol, jump = MODELS[name]
+    jk, jp = jump if jump else (capi.JUMP_NONE, (0, 0, 0, 0))
+    m, fm = both(name)
+    T = 0.9
+    ks = [80.0, 100.0, 120.0]
+    g = engine.fourier(m, InstrumentSpec(strikes=ks, maturity=T, payoff=payoff), capi.GREEK_FULL, S0, R, Q)
+    for i, K in enumerate(ks):
+        ref = fd_greeks(fm, nf, hs, vol, jk, jp, K, T, payoff)
+        got = dict(price=g["price"][i], delta=g["delta"][i], gamma=g["gamma"][i], theta=g["theta"][i],
+                   rho=g["rho"][i])
+        for f in range(nf):
+            got[f"vega{f}"] = g["vega"][f][i]
+        for k, v in ref.items():
+            scale = {"gamma": 1e-2, "delta": 1e-2}.get(k, 1.0)
+            assert abs(got[k] - v) < 2e-5 * max(scale, abs(v)), (name, K, k, got[k], v)
+
+
+def test_bad_arguments_raise(engine):
+    m, _ = both("heston")
+    from myquantlib_amd.pricing import EngineError
+    with pytest.raises(EngineError):
+        engine.fourier(m, InstrumentSpec(strikes=[100.0], maturity=0.0), capi.GREEK_NONE, S0, R, Q)
+    with pytest.raises(EngineError):
+        engine.fourier(m, InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0),
+                       capi.GREEK_NONE, S0, R, Q)

@@ -372,3 +372,56 @@ def test_mlmc_vs_fourier(oracle):
     z = (res["price"] - ref) / res["price_std_err"]
     assert abs(z) < Z_MAX
     assert res["n_levels"] >= 3 and np.all(res["paths_per_level"][:res["n_levels"]] > 0)
+
+
+# ------------------------------------------- Fourier oracle, closed forms ----
+def _bs(S, K, T, r, q, vol, call=True):
+    return fourier.black_scholes(S, K, T, r, q, vol, 0 if call else 1)
+
+
+@pytest.mark.parametrize("T", [0.1, 1.0, 3.0])
+def test_fourier_black_scholes_and_digitals_closed_form(T):
+    """Zero-factor model: Gil-Pelaez prices equal Black-Scholes (vanilla),
+    e^{-rT} N(d2) (cash-or-nothing) and S e^{-qT} N(d1) (asset-or-nothing)."""
+    vol, r, q, S = 0.25, 0.03, 0.01, 100.0
+    fm = fourier_model(0, [], vol)
+    for K in (60.0, 90.0, 100.0, 120.0, 180.0):
+        d1 = (math.log(S / K) + (r - q + 0.5 * vol * vol) * T) / (vol * math.sqrt(T))
+        d2 = d1 - vol * math.sqrt(T)
+        for call in (True, False):
+            got = fourier.price(fm, S, K, T, r, q, 0, 0 if call else 1)
+            assert abs(got - _bs(S, K, T, r, q, vol, call)) < 1e-10 * S, (K, call)
+        assert abs(fourier.price(fm, S, K, T, r, q, 1, 0) - math.exp(-r * T) * stats.norm.cdf(d2)) < 1e-11
+        assert abs(fourier.price(fm, S, K, T, r, q, 2, 0) - S * math.exp(-q * T) * stats.norm.cdf(d1)) < 1e-9
+
+
+def test_fourier_merton_series_closed_form():
+    """Merton jump-diffusion: Gil-Pelaez equals Merton's Poisson-weighted
+    Black-Scholes series."""
+    vol, lam, mu, delta = 0.2, 0.5, -0.1, 0.15
+    r, q, S, T = 0.02, 0.0, 100.0, 1.0
+    jk, jp = merton(lam, mu, delta)
+    fm = fourier_model(0, [], vol, (jk, jp))
+    k = math.exp(mu + 0.5 * delta * delta) - 1.0
+    lp = lam * (1.0 + k)
+    for K in (70.0, 100.0, 130.0):
+        ref = 0.0
+        for n in range(60):
+            vn = math.sqrt(vol * vol + n * delta * delta / T)
+            rn = r - lam * k + n * math.log(1.0 + k) / T
+            w = math.exp(-lp * T) * (lp * T) ** n / math.factorial(n)
+            ref += w * _bs(S, K, T, rn, q, vn)
+        assert abs(fourier.price(fm, S, K, T, r, q) - ref) < 1e-9 * S, K
+
+
+def test_fourier_heston_deterministic_variance_limit():
+    """Heston with vol-of-vol -> 0 (rho = 0): the variance is deterministic,
+    so the price is Black-Scholes at the integrated variance
+    theta T + (v0 - theta)(1 - e^{-kappa T})/kappa (deviation O(sigma^2))."""
+    h = Heston(1.5, 0.05, 1e-3, 0.0, 0.02)
+    fm = fourier_model(1, [h])
+    r, q, S, T = 0.02, 0.01, 100.0, 1.5
+    iv = h.theta * T + (h.v0 - h.theta) * (1.0 - math.exp(-h.kappa * T)) / h.kappa
+    for K in (80.0, 100.0, 125.0):
+        ref = _bs(S, K, T, r, q, math.sqrt(iv / T))
+        assert abs(fourier.price(fm, S, K, T, r, q) - ref) < 1e-5, K
