@@ -10,6 +10,32 @@
 
 enum class GreekMode { None, Essential, Full };
 
+// FourierResult (reference PricingTypes.hpp:17-41): price and, per mode,
+// delta, gamma, vega[2], theta (dP/dT), rho, vanna[2], charm.
+template <GreekMode Mode, typename T = double>
+struct FourierResult;
+template <typename T>
+struct FourierResult<GreekMode::None, T> {
+  T price{};
+};
+template <typename T>
+struct FourierResult<GreekMode::Essential, T> {
+  T price{};
+  T delta{};
+  T gamma{};
+};
+template <typename T>
+struct FourierResult<GreekMode::Full, T> {
+  T price{};
+  T delta{};
+  T gamma{};
+  std::array<T, 2> vega{};
+  T theta{};
+  T rho{};
+  std::array<T, 2> vanna{};
+  T charm{};
+};
+
 template <GreekMode Mode, typename T>
 struct MonteCarloResult;
 

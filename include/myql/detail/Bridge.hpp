@@ -78,6 +78,29 @@ void set_jump(myql_model &m, const JumpParams &j) {
   }
 }
 
+template <typename JumpParams>
+inline constexpr int32_t jump_capi_id = std::is_same_v<JumpParams, MertonParams> ? MYQL_JUMP_MERTON
+                                        : std::is_same_v<JumpParams, KouParams>  ? MYQL_JUMP_KOU
+                                                                                 : MYQL_JUMP_NONE;
+
+// Model alone -> C-ABI model (the Fourier pricer needs no stepper).
+template <typename Model>
+myql_model affine_model_to_capi(const Model &model) {
+  myql_model m{};
+  m.n_factors = Model::num_variance_factors;
+  m.jump_kind = jump_capi_id<std::remove_cv_t<decltype(model.jump)>>;
+  if constexpr (Model::num_variance_factors == 0) {
+    m.vol = model.vol;
+  } else if constexpr (Model::num_variance_factors == 1) {
+    m.heston[0] = to_capi(model.heston);
+  } else {
+    m.heston[0] = to_capi(model.heston1);
+    m.heston[1] = to_capi(model.heston2);
+  }
+  set_jump(m, model.jump);
+  return m;
+}
+
 // Model + stepper -> C-ABI model and scheme pair, with the consistency
 // checks the reference gets from its types.
 template <typename Model, typename Stepper>

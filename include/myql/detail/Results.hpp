@@ -53,6 +53,24 @@ class ResultArrays {
   }
 
   template <GreekMode Mode, typename T>
+  void fill(FourierResult<Mode, T> &out) const {
+    out.price = get<T>(PRICE);
+    if constexpr (Mode != GreekMode::None) {
+      out.delta = get<T>(DELTA);
+      out.gamma = get<T>(GAMMA);
+    }
+    if constexpr (Mode == GreekMode::Full) {
+      for (int i = 0; i < 2; ++i) {
+        out.vega[i] = get<T>(VEGA0 + i);
+        out.vanna[i] = get<T>(VANNA0 + i);
+      }
+      out.theta = get<T>(THETA);
+      out.rho = get<T>(RHO);
+      out.charm = get<T>(CHARM);
+    }
+  }
+
+  template <GreekMode Mode, typename T>
   void fill(MonteCarloResult<Mode, T> &out) const {
     out.price = get<T>(PRICE);
     out.price_std_err = get<T>(PRICE_SE);

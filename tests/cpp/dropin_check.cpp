@@ -14,6 +14,7 @@
 #include <myql/models/asvj/core/ASVJmodel.hpp>
 #include <myql/models/asvj/core/ASVJstepper.hpp>
 #include <myql/models/asvj/core/CoupledASVJStepper.hpp>
+#include <myql/pricers/fourier/FourierPricer.hpp>
 #include <myql/pricers/mlmc/MultiLevelMonteCarloPricer.hpp>
 #include <myql/pricers/montecarlo/MonteCarloPricer.hpp>
 
@@ -93,6 +94,23 @@ int main() {
     const auto res = pricer.calculate(S0, r, q, Call(100.0, 1.0));
     std::printf("{\"case\": \"heston_mlmc_call\", \"price\": %.17g, \"price_std_err\": %.17g, \"levels\": %zu}\n",
                 res.price, res.price_std_err, res.paths_per_level.size());
+  }
+  {  // Gil-Pelaez: Bates strip with Full Greeks, and a cash-or-nothing put
+    BatesModel model(HestonParams{2.0, 0.04, 1.0, -0.7, 0.04}, MertonParams{0.5, -0.1, 0.15});
+    using Strip = EuropeanOption<PayoffVanilla<OptionType::Call>, std::vector<double>>;
+    FourierPricer<BatesModel, Strip, GreekMode::Full> pricer(model);
+    const auto res = pricer.calculate(S0, r, q, Strip({90.0, 100.0, 110.0}, 0.5));
+    std::printf("{\"case\": \"bates_fourier_strip\", ");
+    print_vec("price", res.price);
+    print_vec("delta", res.delta);
+    print_vec("vega0", res.vega[0]);
+    print_vec("theta", res.theta, true);
+    std::printf("}\n");
+    using Digital = EuropeanOption<PayoffCashOrNothing<OptionType::Put>>;
+    FourierPricer<BatesModel, Digital, GreekMode::Essential> dp(model);
+    const auto d = dp.calculate(S0, r, q, Digital(95.0, 1.0));
+    std::printf("{\"case\": \"bates_fourier_digital\", \"price\": %.17g, \"delta\": %.17g, \"gamma\": %.17g}\n",
+                d.price, d.delta, d.gamma);
   }
   {  // SchemeNV outside its domain raises the reference's exception
     HestonModel model(HestonParams{2.0, 0.04, 1.0, -0.7, 0.04});

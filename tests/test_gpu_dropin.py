@@ -91,3 +91,19 @@ def test_mlmc(cases, engine):
 
 def test_nv_feller_raises_reference_message(cases):
     assert cases["nv_feller"]["error"] == "SchemeNV violation: sigma^2 > 4*kappa*theta"
+
+
+def test_fourier_pricer(cases, engine):
+    jk, jp = merton(0.5, -0.1, 0.15)
+    model = make_model(1, [Heston(2.0, 0.04, 1.0, -0.7, 0.04)], jump_kind=jk, jump=jp)
+    got = cases["bates_fourier_strip"]
+    res = engine.fourier(model, InstrumentSpec(strikes=[90.0, 100.0, 110.0], maturity=0.5), capi.GREEK_FULL,
+                         S0, R, Q)
+    for f in ("price", "delta", "theta"):
+        same(got[f], res[f])
+    same(got["vega0"], res["vega"][0])
+    got = cases["bates_fourier_digital"]
+    res = engine.fourier(model, InstrumentSpec(strikes=[95.0], maturity=1.0, payoff=capi.PAYOFF_CASH,
+                                               option_type=capi.PUT), capi.GREEK_ESSENTIAL, S0, R, Q)
+    for f in ("price", "delta", "gamma"):
+        same(got[f], res[f])
