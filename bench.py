@@ -23,7 +23,7 @@ grid factor) is autotuned on an untimed first call (MYQL_FLAG_AUTOTUNE) and
 reported under config.launch_shape.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths-log2 24] [--no-cpu] [--no-fp64]
-                [--no-cv] [--no-mlmc] [--no-batched] [--no-autotune] [--shape B,U,G] [--copy-probe]
+                [--no-cv] [--no-mlmc] [--no-batched] [--no-autotune] [--shape B,U,G[,W4]] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -62,7 +62,7 @@ EXECUTED_VARIANTS = 5
 # HBM bytes per path-kernel launch (FETCH_SIZE + WRITE_SIZE from rocprofv3,
 # scaled by the same counters on a plain copy of known size), by
 # --paths-log2 of the measured run; None where not measured
-TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 1.592e11}
+TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 1.942e11}
 
 
 def model():
@@ -365,8 +365,8 @@ def main():
                     help="run the strip kernel at its default launch shape (256-stream units, "
                          "strike loop unrolled by 4, grid 4x) instead of autotuning it")
     ap.add_argument("--shape", default=None,
-                    help="BLOCK,UNROLL,GRID: fix the strip kernel's launch shape (no autotuning), "
-                         "e.g. to profile the shape an autotuned run chose")
+                    help="BLOCK,UNROLL,GRID[,FOUR_WAVE]: fix the strip kernel's launch shape (no "
+                         "autotuning), e.g. to profile the shape an autotuned run chose")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -385,9 +385,11 @@ def main():
     paths = 1 << args.paths_log2
     flags = 0 if (args.no_autotune or args.shape) else capi.FLAG_AUTOTUNE
     if args.shape:
-        blk, unr, gf = (int(x) for x in args.shape.split(","))
+        parts = [int(x) for x in args.shape.split(",")]
+        blk, unr, gf = parts[:3]
+        w4 = bool(parts[3]) if len(parts) > 3 else False
         for prec in (capi.FP32, capi.FP64):
-            eng.set_launch_shape(prec, 1, 5, False, blk, unr, gf)
+            eng.set_launch_shape(prec, 1, 5, False, blk, unr, gf, w4)
     J32 = jobs(paths, capi.FP32, flags=flags)
     if flags:
         # tuning pass (untimed): trial launches of every shape at 1/8 of the

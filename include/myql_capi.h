@@ -106,7 +106,8 @@ enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
  *      (per path precision, factor count, lockstep width, path dependence)
  *      times its launch shapes -- 256, 128 or 64 streams per work unit
  *      (block size); for the lockstep European kernels also the strike loop
- *      over staged samples unrolled by 4 or 1; 4 or 1 persistent blocks per
+ *      over staged samples unrolled by 4 or 1 and, at 64-stream units, a
+ *      variant compiled for four waves per SIMD; 4 or 1 persistent blocks per
  *      resident slot -- on trial launches of that call's own jobs with an
  *      eighth of each stream's samples, keeps the fastest for the context and
  *      uses it from then on. Streams, draws and paths are those of the
@@ -344,11 +345,13 @@ int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
  * autotuner had chosen it: precision MYQL_FP32/FP64, factor count, lockstep
  * width (1 for replays; 5 / 7 for 0-1 / 2-factor lockstep Europeans; 7 / 9
  * path-dependent), path dependence, block (256, 128 or 64), unroll (4; or 1
- * for lockstep Europeans), grid factor (>= 1). MYQL_E_UNSUPPORTED
- * for a shape that is not compiled. */
+ * for lockstep Europeans), grid factor (>= 1), four_wave (1: the variant
+ * compiled for four waves per SIMD; lockstep Europeans at block 64 only).
+ * MYQL_E_UNSUPPORTED for a shape that is not compiled. */
 int myql_debug_set_launch_shape(myql_ctx *ctx, int32_t precision, int32_t n_factors,
                                 int32_t lockstep_width, int32_t path_dependent,
-                                int32_t block, int32_t unroll, int32_t grid_factor);
+                                int32_t block, int32_t unroll, int32_t grid_factor,
+                                int32_t four_wave);
 /* Control-variate network on n caller feature rows (n x 6, scaled features):
  * the BF16 MFMA input gradient d net/d f_{1,2,3} used by the pricing kernel
  * (out: n x 3), and the FP32 training gradient of mean (net - target)^2 over

@@ -173,7 +173,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_chi2_quantile.restype = C.c_int
     lib.myql_debug_chi2_quantile.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
     lib.myql_debug_set_launch_shape.restype = C.c_int
-    lib.myql_debug_set_launch_shape.argtypes = [vp] + [C.c_int32] * 7
+    lib.myql_debug_set_launch_shape.argtypes = [vp] + [C.c_int32] * 8
     pf = C.POINTER(C.c_float)
     lib.myql_cv_train.restype = C.c_int
     lib.myql_cv_train.argtypes = [vp, C.POINTER(MCJob), C.c_int64, C.POINTER(CVConfig),

@@ -842,11 +842,10 @@ __device__ __forceinline__ void unit_partials(const Job<R> *__restrict__ J, cons
 // Persistent work-queue kernel. `counter` must be zero at launch; `accg`
 // holds 2*NSTAT*BLK doubles per work unit of BLK streams; UNR unrolls the
 // strike loop over staged samples.
-template <typename R, int NF, int NV, bool PD, int BLK = BLOCK, int UNR = 4>
-__global__ void __launch_bounds__(BLK) mc_paths_kernel(const Job<R> *__restrict__ jobs,
-                                                         const Unit *__restrict__ units, int n_units,
-                                                         unsigned *__restrict__ counter,
-                                                         double *__restrict__ accg_all) {
+template <typename R, int NF, int NV, bool PD, int BLK, int UNR>
+__device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, const Unit *__restrict__ units,
+                                              int n_units, unsigned *__restrict__ counter,
+                                              double *__restrict__ accg_all) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_unit;
   __shared__ int s_valid[BLK];
@@ -911,6 +910,24 @@ __global__ void __launch_bounds__(BLK) mc_paths_kernel(const Job<R> *__restrict_
       unit_partials<R, BLK>(jobs + (J->view - 1), J->view_accg + static_cast<size_t>(u) * (2 * NSTAT * BLK), blk,
                             n_it > 0, s_red, tid);
   }
+}
+
+template <typename R, int NF, int NV, bool PD, int BLK = BLOCK, int UNR = 4>
+__global__ void __launch_bounds__(BLK) mc_paths_kernel(const Job<R> *__restrict__ jobs,
+                                                         const Unit *__restrict__ units, int n_units,
+                                                         unsigned *__restrict__ counter,
+                                                         double *__restrict__ accg_all) {
+  mc_paths_body<R, NF, NV, PD, BLK, UNR>(jobs, units, n_units, counter, accg_all);
+}
+
+// The same kernel asked to fit four waves per SIMD (at most 128 VGPRs): the
+// compiler spills what does not fit, and the extra wave hides more of the
+// table gathers' latency; the autotuner weighs the two.
+template <typename R, int NF, int NV, bool PD, int BLK, int UNR>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
+mc_paths_kernel_occ4(const Job<R> *__restrict__ jobs, const Unit *__restrict__ units, int n_units,
+                     unsigned *__restrict__ counter, double *__restrict__ accg_all) {
+  mc_paths_body<R, NF, NV, PD, BLK, UNR>(jobs, units, n_units, counter, accg_all);
 }
 
 }  // namespace myql

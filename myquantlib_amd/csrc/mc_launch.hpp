@@ -11,19 +11,22 @@ namespace myql {
 //   European NF 0/1: 1, 5; NF 2: 1, 7. Path-dependent NF 0/1: 1, 7; NF 2: 1, 9.
 bool mc_supported(int nf, int nv, bool pd);
 
-// Launch shape of the path kernel: streams per work unit (= lanes per block)
-// and the unroll of the strike loop over staged samples. Every kernel is
-// compiled at three block sizes with the loop unrolled by 4 (the first,
-// 256/4, is the default shape); the lockstep European kernels (the strip
-// workloads, whose strike loop dominates the staging phase) also with the
-// loop not unrolled. The autotuner picks among them.
+// Launch shape of the path kernel: streams per work unit (= lanes per block),
+// the unroll of the strike loop over staged samples, and occ = 4 for the
+// variant compiled to fit four waves per SIMD (0: the compiler's choice).
+// Every kernel is compiled at three block sizes with the loop unrolled by 4
+// (the first, 256/4, is the default shape); the lockstep European kernels
+// (the strip workloads, whose strike loop dominates the staging phase) also
+// with the loop not unrolled and, at 64-stream units, in the four-wave
+// variant. The autotuner picks among them.
 struct McShape {
-  int blk, unr;
+  int blk, unr, occ;
 };
-constexpr McShape MC_DEFAULT_SHAPE{BLOCK, 4};
-constexpr McShape MC_TUNE_SHAPES[6] = {{256, 4}, {128, 4}, {64, 4}, {256, 1}, {128, 1}, {64, 1}};
+constexpr McShape MC_DEFAULT_SHAPE{BLOCK, 4, 0};
+constexpr McShape MC_TUNE_SHAPES[8] = {{256, 4, 0}, {128, 4, 0}, {64, 4, 0}, {256, 1, 0},
+                                       {128, 1, 0}, {64, 1, 0},  {64, 4, 4}, {64, 1, 4}};
 // number of leading MC_TUNE_SHAPES compiled for a kernel
-inline int mc_tune_shape_count(int nf, int nv, bool pd) { return (!pd && nv > 1 && nf >= 0 && nf <= 2) ? 6 : 3; }
+inline int mc_tune_shape_count(int nf, int nv, bool pd) { return (!pd && nv > 1 && nf >= 0 && nf <= 2) ? 8 : 3; }
 
 // pd: path-dependent tracker kernels; accg: 2*NSTAT*shape.blk doubles per unit.
 template <typename R>

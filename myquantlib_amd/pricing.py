@@ -211,10 +211,11 @@ class Engine:
             raise EngineError(rc, msg)
 
     def set_launch_shape(self, precision: int, n_factors: int, lockstep_width: int, path_dependent: bool,
-                         block: int, unroll: int, grid_factor: int = 4):
+                         block: int, unroll: int, grid_factor: int = 4, four_wave: bool = False):
         """Fix one path kernel's launch shape (myql_debug_set_launch_shape)."""
         self._check(self.lib.myql_debug_set_launch_shape(self.ctx, precision, n_factors, lockstep_width,
-                                                         int(path_dependent), block, unroll, grid_factor))
+                                                         int(path_dependent), block, unroll, grid_factor,
+                                                         int(four_wave)))
 
     def tuning_report(self) -> str:
         """Launch shapes chosen under FLAG_AUTOTUNE on this context (one line

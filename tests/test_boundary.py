@@ -97,7 +97,7 @@ def test_null_context_is_rejected_without_device_work():
     assert lib.myql_fourier_calculate_batch(None, None, 0) == capi.E_INVALID_ARG
     buf = C.create_string_buffer(64)
     assert lib.myql_tuning_report(None, buf, len(buf)) == capi.E_INVALID_ARG
-    assert lib.myql_debug_set_launch_shape(None, capi.FP32, 1, 5, 0, 256, 4, 4) == capi.E_INVALID_ARG
+    assert lib.myql_debug_set_launch_shape(None, capi.FP32, 1, 5, 0, 256, 4, 4, 0) == capi.E_INVALID_ARG
 
 
 def test_cpp_dropin_headers_compile_and_link(tmp_path):

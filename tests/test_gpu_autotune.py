@@ -45,11 +45,14 @@ def test_every_launch_shape_matches_default(engine, prec):
     tol = 1e-12 if prec == capi.FP64 else 1e-9
     eng = Engine(0)
     try:
-        for blk, unr, gf in [(256, 1, 4), (128, 4, 4), (128, 1, 1), (256, 4, 1), (64, 4, 4), (64, 1, 2)]:
-            eng.set_launch_shape(prec, 1, 5, False, blk, unr, gf)
+        for blk, unr, gf, w4 in [(256, 1, 4, 0), (128, 4, 4, 0), (128, 1, 1, 0), (256, 4, 1, 0), (64, 4, 4, 0),
+                                 (64, 1, 2, 0), (64, 4, 4, 1), (64, 1, 1, 1)]:
+            eng.set_launch_shape(prec, 1, 5, False, blk, unr, gf, bool(w4))
             close(eng.mc_batch(strip_jobs(prec)), ref, tol)
         with pytest.raises(Exception):
             eng.set_launch_shape(prec, 1, 5, False, 32, 4, 4)   # not compiled
+        with pytest.raises(Exception):
+            eng.set_launch_shape(prec, 1, 5, False, 128, 4, 4, True)   # four-wave variant: 64 only
     finally:
         eng.close()
 
@@ -62,7 +65,7 @@ def test_autotuned_strip_matches_default_and_repeats(engine):
         a = eng.mc_batch(strip_jobs(capi.FP64, capi.FLAG_AUTOTUNE))
         rep = eng.tuning_report()
         assert rep.startswith("fp64 nf=1 nv=5 european: block"), rep
-        assert rep.count(" ms") == 12, rep   # 6 shapes x 2 grid factors
+        assert rep.count(" ms") == 16, rep   # 8 shapes x 2 grid factors
         b = eng.mc_batch(strip_jobs(capi.FP64, capi.FLAG_AUTOTUNE))
         assert eng.tuning_report() == rep   # tuned once per context
         close(a, ref)
