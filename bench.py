@@ -289,6 +289,28 @@ def batched_leg(eng, n_sets: int = 1024, steps: int = 252, flags: int = 0):
             "price_range": [float(min(r["price"][0] for r in res)), float(max(r["price"][0] for r in res))]}
 
 
+def fp32_vs_fp64(r32, r64):
+    """Error of the FP32-path strip against the FP64-path strip on identical
+    draws (same seed and streams; sums are FP64 in both), per result field:
+    max |fp32 - fp64| / max(|fp64|, the field's standard error), over options
+    with price >= 0.1 -- the FP32 error in units of the result or of its
+    Monte Carlo noise, whichever is larger."""
+    out = {}
+    for f in ("price", "delta", "gamma", "vega", "theta", "rho", "vanna", "charm"):
+        worst = 0.0
+        for a, b in zip(r32, r64):
+            ok = b["price"] >= 0.1
+            x, y = np.asarray(a[f], float), np.asarray(b[f], float)
+            se = np.asarray(b[f + "_std_err"], float)
+            if x.ndim == 2:
+                x, y, se = x[0], y[0], se[0]
+            d = np.abs(x - y)[ok] / np.maximum(np.abs(y), se)[ok]
+            if d.size:
+                worst = max(worst, float(d.max()))
+        out[f] = worst
+    return out
+
+
 def fourier_check(eng, mc_results):
     """The strip's 512 options with Full Greeks by the GPU Gil-Pelaez pricer
     (one launch), and the FP32 Monte Carlo strip against it: z = (MC - Fourier)
@@ -420,6 +442,7 @@ def main():
     fourier_leg = None if args.no_fourier else fourier_check(eng, res)
 
     ms64 = None
+    fp32_err = None
     if not args.no_fp64:
         J64 = jobs(paths, capi.FP64, flags=flags)
         eng.mc_batch(J64)
@@ -428,6 +451,8 @@ def main():
         w64 = 1e3 * (time.perf_counter() - t0)
         ms64, worst64 = ms_to_target(r64, w64)
         print(f"# fp64 strip: {w64:.1f} ms, worst rel se {worst64:.2e}", file=sys.stderr, flush=True)
+        fp32_err = fp32_vs_fp64(res, r64)
+        print(f"# fp32 paths vs fp64 paths on the same draws: {fp32_err}", file=sys.stderr, flush=True)
 
     # price-only strip (GreekMode None): plain FP32 / FP64 and the learned CV
     cv = {}
@@ -477,6 +502,8 @@ def main():
         },
         # price-only runs of the strip's 512 options (GreekMode None); the CV
         # time includes its training; "full_greeks_*" are the Full-mode runs
+        # FP32-path error against FP64 paths on the same draws, per field
+        "fp32_vs_fp64_same_draws_max_rel": fp32_err,
         "ms_to_1e-4_rel_se": {
             "plain_fp32": po32, "plain_fp64": po64,
             "learned_cv_fp32": cv["bates_strip"]["learned_cv_fp32_ms_to_1e-4"] if cv else None,
