@@ -3,7 +3,7 @@
 Workload (one "step"): one pricing of the whole strip through
 myql_mc_calculate_batch -- 64 strikes uniform in [70, 130] x 8 maturities
 {0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0}, each maturity a separate
-MonteCarloPricer::calculate call (2^24 paths, 100 steps per year, seed 42,
+MonteCarloPricer::calculate call (2^24 paths on 262,144 xoshiro streams, 100 steps per year, seed 42,
 GreekMode::Full with common random numbers) under Bates with SchemeNCI:
 kappa 2, theta 0.04, sigma 1.0 (Feller violated), rho -0.7, v0 0.04, Merton
 jumps lambda 0.5, mu -0.1, delta 0.15; S0 100, r 0.02, q 0. Paths are FP32,
@@ -71,7 +71,14 @@ def model():
     return make_model(1, [Heston(2.0, 0.04, 1.0, -0.7, 0.04)], jump_kind=jk, jump=jp)
 
 
-def jobs(paths: int, precision: int, layout: int = 0, streams: int = 0, mode: int = None,
+# xoshiro streams of every strip call: 64 samples per stream at 2^24 paths,
+# 4,096 work units per maturity, so the persistent queue balances to within a
+# fraction of a unit (65,536 streams leave ~2 units per wavefront slot and a
+# 480-550 ms spread from run to run; 262,144 run at a steady ~450 ms)
+STRIP_STREAMS = 1 << 18
+
+
+def jobs(paths: int, precision: int, layout: int = 0, streams: int = STRIP_STREAMS, mode: int = None,
          seed: int = 42, bates_kou: bool = False, flags: int = 0):
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import InstrumentSpec, mc_config
@@ -493,8 +500,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "bates_strip: Bates SchemeNCI (Feller violated) + Merton, 64 strikes x 8 "
-                        f"maturities, 2^{args.paths_log2} paths per maturity, 100 steps/yr, "
-                        "GreekMode::Full CRN, FP32 paths, FP64 sums, seed 42",
+                        f"maturities, 2^{args.paths_log2} paths per maturity on {STRIP_STREAMS} streams, "
+                        "100 steps/yr, GreekMode::Full CRN, FP32 paths, FP64 sums, seed 42",
             "paths_per_maturity": paths,
             "path_steps_per_step": total,
             "launch_shape": ("autotuned: " + eng.tuning_report().strip()) if flags else
