@@ -404,8 +404,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md but not launched"}))
+        print(json.dumps({"error": "this engine runs on one GPU: sharding the strip's streams "
+                                   "across GPUs is designed but not launched; use --gpus 1"}))
         sys.exit(2)
 
     from myquantlib_amd import capi
