@@ -549,6 +549,13 @@ void fill_variant(VarParams<R> &V, const myql_model &m, const int32_t *schemes, 
     V.jmean = 0.0;
     V.jexpm = 1.0;
   }
+  // The reference draws jump counts with Boost's poisson_distribution
+  // (JumpPolicies.hpp:38,48,86,98), which inverts one uniform for means
+  // below 10 (restated here) and switches to transformed rejection (PTRD,
+  // a variable number of draws) from 10 on -- not restated, so refused
+  // rather than priced from different draws.
+  if (V.jmean >= 10.0)
+    throw Fail(MYQL_E_UNSUPPORTED, "jump intensity x span >= 10 (Boost PTRD Poisson sampler) is not supported");
 }
 
 struct JobSpec {

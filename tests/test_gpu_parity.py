@@ -303,3 +303,20 @@ def test_fused_asian_barrier_equal_separate_calls(engine, prec, mode, nf):
     for f_res, s_res in zip(fused, sep):
         for f in fields:
             assert np.array_equal(np.asarray(f_res[f]), np.asarray(s_res[f])), f
+
+
+def test_jump_means_from_ten_are_refused(engine):
+    """Boost's poisson_distribution (the reference's jump counts) inverts one
+    uniform below mean 10 and switches to its PTRD rejection sampler from 10
+    on; that branch is not restated, so the engine refuses such calls instead
+    of pricing them from different draws."""
+    from myquantlib_amd.pricing import EngineError
+    jk, jp = merton(6.0, -0.05, 0.1)
+    m = make_model(1, [H1], jump_kind=jk, jump=jp)
+    inst = InstrumentSpec(strikes=[100.0], maturity=2.0)          # European: mean = 6 x 2
+    with pytest.raises(EngineError) as e:
+        engine.mc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE, mc_config(1 << 12, 50), 100.0, 0.02, 0.0)
+    assert e.value.code == capi.E_UNSUPPORTED and "PTRD" in str(e.value)
+    ok = engine.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=[100.0], maturity=1.5), capi.GREEK_NONE,
+                   mc_config(1 << 12, 50), 100.0, 0.02, 0.0)      # mean 9: inversion, priced
+    assert np.isfinite(ok["price"][0])
