@@ -114,6 +114,26 @@ def test_cpp_dropin_headers_compile_and_link(tmp_path):
     assert exe.exists()
 
 
+REF_INCLUDE = "/root/reference/include"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_INCLUDE, "myql", "dispatcher")),
+                    reason="needs the reference's dispatcher headers")
+def test_reference_dispatcher_needs_only_dropin_headers():
+    """The reference's runtime dispatcher (myql/dispatcher/*.hpp), preprocessed
+    with include/ first and the reference's include/ second, takes every
+    other myql/ header it includes from the drop-in: the reference supplies
+    only the dispatcher itself (oracle/_ref/dispatch_check, run by
+    test_gpu_dispatcher.py, is that build)."""
+    src = os.path.join(ROOT, "tests", "cpp", "dispatch_check.cpp")
+    out = subprocess.run(["g++", "-std=c++20", "-M", "-I", os.path.join(ROOT, "include"), "-I", REF_INCLUDE, src],
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    deps = [d for d in out.stdout.replace("\\\n", " ").split() if d.startswith(REF_INCLUDE)]
+    assert deps, "the dispatcher headers were not found"
+    assert all(os.path.dirname(d) == os.path.join(REF_INCLUDE, "myql", "dispatcher") for d in deps), deps
+
+
 @pytest.mark.parametrize("snippet,needle", [
     # a stepper whose tracker differs from the instrument's
     ("MonteCarloPricer<HestonModel, HestonStepper<SchemeNV>, UpAndOutCall<>> p(m, c);",
