@@ -429,8 +429,9 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   return ni;
 }
 
-// Table fields of a job factor; the tail rows are used unless the config
-// asks for the reference's table semantics.
+// Table fields of a job factor; the tail rows and the exact inversion of the
+// first grid interval are used unless the config asks for the reference's
+// table semantics.
 template <typename R>
 void set_nci(Job<R> &J, int f, const NciInfo &ni, bool reference) {
   J.nci[f] = ni.tab;
@@ -440,6 +441,7 @@ void set_nci(Job<R> &J, int f, const NciInfo &ni, bool reference) {
   J.nci_dx = ni.dx;
   J.nci_inv_dx = ni.inv_dx;
   J.nci_tail[f] = reference ? nullptr : ni.tail;
+  J.nci_u_lo = reference ? 0.0 : ni.dx;
   J.tail_pts = NCI_TAIL;
   J.tail_w0 = NCI_TAIL_W0;
   J.tail_inv_hw = (NCI_TAIL - 1) / (NCI_TAIL_W1 - NCI_TAIL_W0);

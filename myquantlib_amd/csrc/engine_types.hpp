@@ -70,6 +70,9 @@ struct Job {
   int tail_pts;
   int nci_grid;
   double nci_dx, nci_inv_dx;
+  // u below this (the first grid interval, or 0 = never): rows N >= 2 are
+  // inverted exactly instead of interpolated from the table's u = 0 node
+  double nci_u_lo;
   const double *strikes;
   int nk, nkp;                     // strikes in this job, padded to a power of 2 (<= 64)
   double *partial;                 // per unit: 2*NSTAT x nk sums

@@ -483,7 +483,12 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     // the table (chi2 quantiles of d + 2N degrees of freedom) and u2 are
     // common to all slots, so slots sharing slot 0's count share its
     // interpolated quantile and differ only in their scale factor
-    const bool main0 = Ns[0] <= P[0]->nmax[f] && u2 < 0.999;
+    // Rows N >= 2 have quantile 0 at the table's u = 0 node but rise
+    // steeply just above it (like u^(2/(d+2N))), so interpolating the first
+    // grid interval biases the draw low (by up to ~3e-5 of the row mean per
+    // step for large N); below nci_u_lo those rows are inverted exactly
+    const bool lo = u2 < J->nci_u_lo;
+    const bool main0 = Ns[0] <= P[0]->nmax[f] && u2 < 0.999 && !(lo && Ns[0] >= 2);
     R y0 = R(0);
     if (main0) {
       const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
@@ -492,12 +497,13 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const int N = Ns[k];
+      const bool in_tab = N <= P[k]->nmax[f] && !(lo && N >= 2);
       if (main0 && N == Ns[0] && N <= P[k]->nmax[f]) {
         vn[k][f] = y0 * P[k]->Kr[f][2];
-      } else if (N <= P[k]->nmax[f] && u2 < 0.999) {
+      } else if (in_tab && u2 < 0.999) {
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
-      } else if (N <= P[k]->nmax[f] && J->nci_tail[f] != nullptr) {
+      } else if (in_tab && u2 >= 0.999 && J->nci_tail[f] != nullptr) {
         // u >= 0.999: cubic Hermite in w = -log(1 - u) with exact slopes
         double tp = (-log1p(-u2) - J->tail_w0) * J->tail_inv_hw;
         if (tp < 0.0) tp = 0.0;

@@ -64,6 +64,9 @@ class Oracle:
         lib.oracle_cir_terminal.restype = C.c_int
         lib.oracle_cir_terminal.argtypes = [C.c_int32, C.POINTER(capi.HestonParams), C.c_double,
                                             C.c_int64, C.c_int64, C.c_uint64, dp, C.c_int]
+        lib.oracle_nci_lookup.restype = C.c_int
+        lib.oracle_nci_lookup.argtypes = [C.POINTER(capi.HestonParams), C.c_double, C.c_int, C.c_int32, dp,
+                                          C.c_int64, dp]
         lib.oracle_mc_calculate.restype = C.c_int
         lib.oracle_mc_calculate.argtypes = [C.POINTER(capi.Model), C.POINTER(C.c_int32),
                                             C.POINTER(capi.Instrument), C.c_int32,
@@ -158,6 +161,18 @@ class Oracle:
     def poisson_samples(self, mean: float, n: int, seed: int) -> np.ndarray:
         out = np.zeros(n, dtype=np.int32)
         self.lib.oracle_poisson_samples(mean, n, seed, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def nci_lookup(self, h, dt: float, N: int, u, reference_rows: bool = False) -> np.ndarray:
+        """The NCI table's chi2_{d+2N} quantile (before scaling) at uniforms u."""
+        p = capi.HestonParams(h.kappa, h.theta, h.sigma, h.rho, h.v0)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(u.size)
+        rc = self.lib.oracle_nci_lookup(C.byref(p), dt, int(reference_rows), N,
+                                        u.ctypes.data_as(C.POINTER(C.c_double)), u.size,
+                                        out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise EngineError(rc, self.lib.oracle_last_error().decode())
         return out
 
     def cir_terminal(self, scheme: int, h, dt: float, n_steps: int, n: int, seed: int = 1,

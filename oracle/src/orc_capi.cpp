@@ -239,6 +239,26 @@ int oracle_cir_terminal(int32_t scheme, const myql_heston_params *p, double dt, 
   }
 }
 
+// The NCI scheme's table value for row N (the chi2_{d+2N} quantile before
+// the scale factor) at the uniforms u[0..n), from the table a pricing call
+// with this factor and step builds (reference_rows: the reference's rows and
+// exact-inversion rules, else the extended rows), NCI table check only.
+int oracle_nci_lookup(const myql_heston_params *p, double dt, int reference_rows, int32_t N, const double *u,
+                      int64_t n, double *out) {
+  try {
+    TableCache tc;
+    tc.reference_rows = reference_rows != 0;
+    Factor f;
+    prepare_factor(f, MYQL_SCHEME_NCI, *p, 0.0, dt, tc.get(*p, dt));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = nci_lookup(f, N, u[i]) / f.K[2];
+    return MYQL_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  }
+}
+
 int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const myql_instrument *instr,
                         int32_t mode, const myql_mc_config *cfg, double S0, double r, double q,
                         myql_mc_result *res, int nthreads) {

@@ -237,9 +237,15 @@ inline std::atomic<uint64_t> &nci_fallbacks() {
   return n;
 }
 
+// Outside the reference's table semantics (tail rows present), rows N >= 2
+// are inverted exactly in the first grid interval U < dx: their quantile is 0
+// at the u = 0 node and rises like u^(2/(d+2N)) just above it, so the
+// interpolated first interval biases E[v'] low (by up to ~3e-5 of the row
+// mean at large N), an error each step adds to the mean-reverting variance.
 static inline double nci_lookup(const Factor &f, int N, double U) {
   const NciTable &t = *f.nci;
-  if (static_cast<size_t>(N) <= t.n_max && U < 0.999) {
+  const bool lo_exact = !t.ty.empty() && N >= 2 && U < t.dx;
+  if (static_cast<size_t>(N) <= t.n_max && U < 0.999 && !lo_exact) {
     const double pos = U * t.inv_dx;
     size_t i = static_cast<size_t>(pos);
     if (i >= t.grid - 1) i = t.grid - 2;
@@ -252,7 +258,7 @@ static inline double nci_lookup(const Factor &f, int N, double U) {
             (t3 - t2) * t.dx * m1) *
            f.K[2];
   }
-  if (static_cast<size_t>(N) <= t.n_max && !t.ty.empty()) {
+  if (static_cast<size_t>(N) <= t.n_max && !t.ty.empty() && U >= 0.999) {
     // upper tail: cubic Hermite in w = -log(1 - U) with exact slopes
     const size_t P = NciTable::kTail;
     double pos = (-std::log1p(-U) - t.tw0) * t.inv_thw;

@@ -79,6 +79,13 @@ def main():
     p_t[23:43] = 1e-6
     p_t[43:63] = 1.0 - 1e-9
     g["chi2_table_range"] = {"dof": dof_t.tolist(), "p": p_t.tolist(), "q": stats.chi2.ppf(p_t, dof_t).tolist()}
+    # the first NCI grid interval, inverted exactly for rows N >= 2: p from
+    # the smallest nonzero uniform (2^-64) up to the grid step 0.999 / 9999
+    rl = np.random.default_rng(20261204)
+    dof_l = 0.32 + 2.0 * np.concatenate([[2, 5, 1023], rl.integers(2, 1024, 197)])
+    p_l = np.exp(rl.uniform(np.log(2.0 ** -64), np.log(0.999 / 9999), dof_l.size))
+    p_l[:3] = [2.0 ** -64, 0.999 / 9999, 2.0 ** -64]
+    g["chi2_low_tail"] = {"dof": dof_l.tolist(), "p": p_l.tolist(), "q": stats.chi2.ppf(p_l, dof_l).tolist()}
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json")
     with open(out, "w") as f:
         json.dump(g, f, indent=1)

@@ -97,6 +97,9 @@ def test_chi2_quantile_device(engine, oracle):
     p[:20] = 0.999
     p[20:40] = 1 - 1e-9
     p[40:60] = 1e-6
+    # the first NCI grid interval (inverted exactly for rows N >= 2)
+    p[500:700] = np.exp(rng.uniform(np.log(2.0 ** -64), np.log(1e-4), 200))
+    p[1000:1200] = np.exp(rng.uniform(np.log(2.0 ** -64), np.log(1e-4), 200))
     out = engine.chi2_quantile(dof, p)
     ref = np.array([oracle.chi2_quantile(a, b) for a, b in zip(dof, p)])
     assert rel_err(out, ref, 1e-300) < 1e-12

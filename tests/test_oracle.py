@@ -146,6 +146,48 @@ def test_chi2_quantile_golden_table_range(oracle):
     assert np.max(np.abs(got - ref) / ref) < 1e-12
 
 
+def test_chi2_quantile_golden_low_tail(oracle):
+    """The first NCI grid interval, where rows N >= 2 are inverted exactly:
+    p from 2^-64 (the smallest nonzero uniform) to the grid step, degrees of
+    freedom d + 2N up to N = 1023."""
+    g = GOLDEN["chi2_low_tail"]
+    got = np.array([oracle.chi2_quantile(d, p) for d, p in zip(g["dof"], g["p"])])
+    ref = np.array(g["q"])
+    assert np.max(np.abs(got - ref) / ref) < 1e-12
+
+
+def test_nci_table_row_means(oracle):
+    """Each NCI table row must average to its chi-squared mean d + 2N over
+    the uniform (a stratified 2^21-point average of the table lookup).
+    Under the reference's table semantics the first grid interval is
+    interpolated from the u = 0 node (quantile 0) across a steep rise
+    ~u^(2/(d+2N)), so the row mean comes out low -- by 2.5e-6 at N = 5 and
+    ~2e-5 at N = 40 on the strip's factor at dt = 0.01 -- a bias each step
+    adds to the mean-reverting variance (~bias/(kappa dt) on its level). The
+    engine's default semantics invert that interval exactly for N >= 2."""
+    h = Heston(2.0, 0.04, 1.0, -0.7, 0.04)
+    d = 4 * h.kappa * h.theta / h.sigma ** 2
+    # midpoints on [0, 0.999) plus the upper tail in w = -log(1 - u) up to
+    # w = 36 (the mass beyond is below 1e-14)
+    n = 1 << 21
+    u = 0.999 * (np.arange(n) + 0.5) / n
+    m = 1 << 14
+    hw = (36.0 - math.log(1000.0)) / m
+    w = math.log(1000.0) + hw * (np.arange(m) + 0.5)
+
+    def row_mean(N, ref):
+        main = 0.999 * oracle.nci_lookup(h, 0.01, N, u, ref).mean()
+        tail = np.sum(oracle.nci_lookup(h, 0.01, N, -np.expm1(-w), ref) * np.exp(-w)) * hw
+        return (main + tail) / (d + 2 * N) - 1
+
+    for N in (0, 1, 2, 5, 20, 40):
+        rel = row_mean(N, False)
+        assert abs(rel) < 4e-7, (N, rel)
+    for N, lo, hi in ((5, -3e-6, -2e-6), (20, -1.5e-5, -1.1e-5), (40, -2.2e-5, -1.6e-5)):
+        rel = row_mean(N, True)
+        assert lo < rel < hi, (N, rel)
+
+
 def test_incomplete_gamma_golden(oracle):
     g = GOLDEN["igamma"]
     P = np.array([oracle.gamma_p(a, x) for a, x in zip(g["a"], g["x"])])
