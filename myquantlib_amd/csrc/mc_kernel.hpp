@@ -494,6 +494,20 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
       y0 = h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3];
     }
+    // likewise the exact quantile when slot 0's row is inverted exactly:
+    // computed once, taken by the slots that invert the same row (FP32
+    // paths; the FP64 kernel runs faster without the extra live state)
+    bool ex0 = false;
+    double q0 = 0.0;
+    if constexpr (NV > 1 && sizeof(R) == 4) {
+      const bool in0 = Ns[0] <= P[0]->nmax[f] && !(lo && Ns[0] >= 2);
+      ex0 = !(in0 && (u2 < 0.999 || J->nci_tail[f] != nullptr));
+      if (ex0) {
+        const double dof = P[0]->Kd[f][1] + 2.0 * Ns[0];
+        const double lg = (Ns[0] < J->nci_lg_n) ? J->nci_lgam[f][Ns[0]] : lgamma(0.5 * dof);
+        q0 = chi2_quantile_lg(dof, u2, lg);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const int N = Ns[k];
@@ -515,9 +529,15 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
                     (R(-2) * tt3 + R(3) * tt2) * row[2] + (tt3 - tt2) * row[3]) *
                    P[k]->Kr[f][2];
       } else {
-        const double dof = P[k]->Kd[f][1] + 2.0 * N;
-        const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
-        vn[k][f] = static_cast<R>(chi2_quantile_lg(dof, u2, lg) * P[k]->Kd[f][2]);
+        double q;
+        if (ex0 && N == Ns[0] && P[k]->Kd[f][1] == P[0]->Kd[f][1]) {
+          q = q0;
+        } else {
+          const double dof = P[k]->Kd[f][1] + 2.0 * N;
+          const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
+          q = chi2_quantile_lg(dof, u2, lg);
+        }
+        vn[k][f] = static_cast<R>(q * P[k]->Kd[f][2]);
       }
     }
   } else {  // Exact (Poisson-gamma mixture); variable draws, so never lockstep
