@@ -416,6 +416,20 @@ __device__ __forceinline__ int poisson_count_packed(double lam, double u) {
   return Ns[0];
 }
 
+// NCI upper tail (u >= 0.999), row N: cubic Hermite in w = -log(1 - u) with
+// exact slopes, before the slot's scale factor
+template <typename R>
+__device__ __forceinline__ R nci_tail_value(const Job<R> *__restrict__ J, int f, int N, double u2) {
+  double tp = (-log1p(-u2) - J->tail_w0) * J->tail_inv_hw;
+  if (tp < 0.0) tp = 0.0;
+  int j = static_cast<int>(tp);
+  if (j >= J->tail_pts - 1) j = J->tail_pts - 2;
+  const R tt = static_cast<R>(tp - static_cast<double>(j)), tt2 = tt * tt, tt3 = tt2 * tt;
+  const R *row = static_cast<const R *>(J->nci_tail[f]) + 2 * (static_cast<size_t>(N) * J->tail_pts + j);
+  return (R(2) * tt3 - R(3) * tt2 + R(1)) * row[0] + (tt3 - R(2) * tt2 + tt) * row[1] +
+         (R(-2) * tt3 + R(3) * tt2) * row[2] + (tt3 - tt2) * row[3];
+}
+
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
 template <typename R, int NV, int NFX>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
@@ -494,14 +508,18 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
       y0 = h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3];
     }
-    // likewise the exact quantile when slot 0's row is inverted exactly:
-    // computed once, taken by the slots that invert the same row (FP32
-    // paths; the FP64 kernel runs faster without the extra live state)
-    bool ex0 = false;
+    // likewise slot 0's upper-tail value, or its exact quantile when its
+    // row is inverted exactly: computed once, taken by the slots on the same
+    // row and path (FP32 paths; the FP64 kernel runs faster without the
+    // extra live state)
+    bool ex0 = false, tl0 = false;
     double q0 = 0.0;
+    R yt0 = R(0);
     if constexpr (NV > 1 && sizeof(R) == 4) {
       const bool in0 = Ns[0] <= P[0]->nmax[f] && !(lo && Ns[0] >= 2);
+      tl0 = in0 && u2 >= 0.999 && J->nci_tail[f] != nullptr;
       ex0 = !(in0 && (u2 < 0.999 || J->nci_tail[f] != nullptr));
+      if (tl0) yt0 = nci_tail_value<R>(J, f, Ns[0], u2);
       if (ex0) {
         const double dof = P[0]->Kd[f][1] + 2.0 * Ns[0];
         const double lg = (Ns[0] < J->nci_lg_n) ? J->nci_lgam[f][Ns[0]] : lgamma(0.5 * dof);
@@ -518,16 +536,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
         vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
       } else if (in_tab && u2 >= 0.999 && J->nci_tail[f] != nullptr) {
-        // u >= 0.999: cubic Hermite in w = -log(1 - u) with exact slopes
-        double tp = (-log1p(-u2) - J->tail_w0) * J->tail_inv_hw;
-        if (tp < 0.0) tp = 0.0;
-        int j = static_cast<int>(tp);
-        if (j >= J->tail_pts - 1) j = J->tail_pts - 2;
-        const R tt = static_cast<R>(tp - static_cast<double>(j)), tt2 = tt * tt, tt3 = tt2 * tt;
-        const R *row = static_cast<const R *>(J->nci_tail[f]) + 2 * (static_cast<size_t>(N) * J->tail_pts + j);
-        vn[k][f] = ((R(2) * tt3 - R(3) * tt2 + R(1)) * row[0] + (tt3 - R(2) * tt2 + tt) * row[1] +
-                    (R(-2) * tt3 + R(3) * tt2) * row[2] + (tt3 - tt2) * row[3]) *
-                   P[k]->Kr[f][2];
+        vn[k][f] = ((tl0 && N == Ns[0]) ? yt0 : nci_tail_value<R>(J, f, N, u2)) * P[k]->Kr[f][2];
       } else {
         double q;
         if (ex0 && N == Ns[0] && P[k]->Kd[f][1] == P[0]->Kd[f][1]) {
