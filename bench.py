@@ -62,7 +62,7 @@ EXECUTED_VARIANTS = 5
 # HBM bytes per path-kernel launch (FETCH_SIZE + WRITE_SIZE from rocprofv3,
 # scaled by the same counters on a plain copy of known size), by
 # --paths-log2 of the measured run; None where not measured
-TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 2.340e11}
+TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 1.846e11}
 
 
 def model():
@@ -296,6 +296,40 @@ def batched_leg(eng, n_sets: int = 1024, steps: int = 252, flags: int = 0):
             "price_range": [float(min(r["price"][0] for r in res)), float(max(r["price"][0] for r in res))]}
 
 
+def batched_by_scheme(eng, n_sets: int = 1024, steps: int = 252):
+    """The batched workload's sets split by how many factors take
+    SchemeExact (0, 1, 2), each class priced as one batch at a fixed launch
+    shape (64-stream units, strike loop unrolled by 4, grid 4x): device time
+    per 1e9 path-steps of each class."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec, make_model, mc_config
+    eng.set_launch_shape(capi.FP32, 2, 1, True, 64, 4, 4)
+    sets = double_heston_sets(n_sets)
+    asian = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=1.0)
+    dao = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0,
+                         barrier_dir=capi.BARRIER_DOWN, barrier_action=capi.BARRIER_OUT)
+
+    def jobs(sel):
+        J = []
+        for hs, schemes, paths in sel:
+            m = make_model(2, hs)
+            cfg = mc_config(paths, steps, seed=42, precision=capi.FP32, num_streams=1 << 14)
+            J += [(m, schemes, asian, capi.GREEK_NONE, cfg, S0, R, Q), (m, schemes, dao, capi.GREEK_NONE, cfg, S0, R, Q)]
+        return J
+
+    eng.mc_batch(jobs(sets[:64]))
+    out = {}
+    for n_exact in (0, 1, 2):
+        sel = [x for x in sets if x[1].count(capi.SCHEME_EXACT) == n_exact]
+        res = eng.mc_batch(jobs(sel))
+        ps = sum(p * steps for _h, _s, p in sel)   # simulated once per set
+        out[f"exact_factors_{n_exact}"] = {"sets": len(sel), "path_steps": ps, "device_ms": res[0]["device_ms"],
+                                           "ms_per_1e9_path_steps": res[0]["device_ms"] / (ps / 1e9)}
+        print(f"# batched, sets with {n_exact} SchemeExact factor(s): {len(sel)} sets, {ps:.3e} path-steps, "
+              f"{res[0]['device_ms']:.1f} ms device", file=sys.stderr, flush=True)
+    return out
+
+
 def fp32_vs_fp64(r32, r64):
     """Error of the FP32-path strip against the FP64-path strip on identical
     draws (same seed and streams; sums are FP64 in both), per result field:
@@ -399,6 +433,9 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="BLOCK,UNROLL,GRID[,FOUR_WAVE]: fix the strip kernel's launch shape (no "
                          "autotuning), e.g. to profile the shape an autotuned run chose")
+    ap.add_argument("--batched-by-scheme", action="store_true",
+                    help="only time the batched Double-Heston sets split by their SchemeExact factor "
+                         "count, print one JSON line and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -414,6 +451,9 @@ def main():
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import Engine
     eng = Engine(0)
+    if args.batched_by_scheme:
+        print(json.dumps({"batched_by_scheme": batched_by_scheme(eng)}))
+        return
     paths = 1 << args.paths_log2
     flags = 0 if (args.no_autotune or args.shape) else capi.FLAG_AUTOTUNE
     if args.shape:
