@@ -95,13 +95,18 @@ enum { MYQL_FP64 = 0, MYQL_FP32 = 1 };
  * OpenMP static schedule over S threads would give thread s. */
 enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
 /* Config flags.
- *  REFERENCE_NCI_ROWS  size the SchemeNCI quantile table exactly as the
- *      reference does (n_max from v_max = max(v0, 3 theta), VolSchemes.hpp
- *      :245-255) and invert the chi-squared law exactly above it. Without the
- *      flag the table gets rows up to n_ext = ceil(l + 6 sqrt(l)), l the
- *      Poisson intensity at v = max(v0, 25 theta) (at most 1023 rows), so the
- *      exact inversion -- thousands of cycles, run by the whole wavefront --
- *      is left for u >= 0.999 and essentially never for the row count.
+ *  REFERENCE_NCI_ROWS  use the reference's SchemeNCI table semantics
+ *      (VolSchemes.hpp:245-255, 342-360): n_max from v_max = max(v0, 3 theta),
+ *      every u < 0.999 interpolated, the chi-squared law inverted exactly for
+ *      u >= 0.999 and above the last row. Without the flag: rows up to
+ *      n_ext = ceil(l + 6 sqrt(l)), l the Poisson intensity at
+ *      v = max(v0, 25 theta) (at most 1023 rows), so the exact inversion --
+ *      thousands of cycles, run by the whole wavefront -- is essentially never
+ *      needed for the row count; u >= 0.999 from a tail table on
+ *      w = -log(1 - u); and rows N >= 2 inverted exactly in the first grid
+ *      interval u < 0.999/9999, where interpolating from the quantile-0 node
+ *      at u = 0 lowers the row's mean by up to 3e-5 -- a per-step bias of
+ *      the variance that grows as the step shrinks.
  *  AUTOTUNE  the first pricing call of a context that meets a path kernel
  *      (per path precision, factor count, lockstep width, path dependence)
  *      times its launch shapes -- 256, 128 or 64 streams per work unit
