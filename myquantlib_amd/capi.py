@@ -59,12 +59,18 @@ class MCConfig(C.Structure):
 _PD = C.POINTER(C.c_double)
 
 
+# The result arrays (double *) are declared as plain addresses so that a
+# batch of thousands of results can be pointed at numpy buffers with integer
+# arithmetic instead of one ctypes pointer object per field.
+_VP = C.c_void_p
+
+
 class MCResult(C.Structure):
-    _fields_ = [("price", _PD), ("price_std_err", _PD), ("delta", _PD), ("delta_std_err", _PD),
-                ("gamma", _PD), ("gamma_std_err", _PD), ("vega", _PD * 2),
-                ("vega_std_err", _PD * 2), ("theta", _PD), ("theta_std_err", _PD), ("rho", _PD),
-                ("rho_std_err", _PD), ("vanna", _PD * 2), ("vanna_std_err", _PD * 2),
-                ("charm", _PD), ("charm_std_err", _PD), ("total_steps", C.c_uint64),
+    _fields_ = [("price", _VP), ("price_std_err", _VP), ("delta", _VP), ("delta_std_err", _VP),
+                ("gamma", _VP), ("gamma_std_err", _VP), ("vega", _VP * 2),
+                ("vega_std_err", _VP * 2), ("theta", _VP), ("theta_std_err", _VP), ("rho", _VP),
+                ("rho_std_err", _VP), ("vanna", _VP * 2), ("vanna_std_err", _VP * 2),
+                ("charm", _VP), ("charm_std_err", _VP), ("total_steps", C.c_uint64),
                 ("device_ms", C.c_double)]
 
 

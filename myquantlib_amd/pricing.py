@@ -125,25 +125,31 @@ def mlmc_config(epsilon: float = 0.005, L_min: int = 2, L_max: int = 15, N0: int
 
 
 class ResultBuffers:
-    """numpy arrays that an MCResult points into."""
+    """numpy arrays that an MCResult points into: one NaN-filled block, one
+    row per result array (two for the per-factor fields)."""
+
+    ROWS = 2 * len(RESULT_FIELDS) + 4 * len(PAIR_FIELDS)
 
     def __init__(self, n: int):
         self.n = n
         self.arr = {}
+        block = np.full((self.ROWS, n), np.nan)
+        self._block = block
+        base, stride = block.ctypes.data, block.strides[0]
         r = MCResult()
-        pd = C.POINTER(C.c_double)
+        row = 0
         for f in RESULT_FIELDS:
             for suffix in ("", "_std_err"):
-                a = np.full(n, np.nan)
-                self.arr[f + suffix] = a
-                setattr(r, f + suffix, a.ctypes.data_as(pd))
+                self.arr[f + suffix] = block[row]
+                setattr(r, f + suffix, base + row * stride)
+                row += 1
         for f in PAIR_FIELDS:
             for suffix in ("", "_std_err"):
-                a = np.full((2, n), np.nan)
-                self.arr[f + suffix] = a
+                self.arr[f + suffix] = block[row:row + 2]
                 arrp = getattr(r, f + suffix)
-                arrp[0] = a[0].ctypes.data_as(pd)
-                arrp[1] = a[1].ctypes.data_as(pd)
+                arrp[0] = base + row * stride
+                arrp[1] = base + (row + 1) * stride
+                row += 2
         self.struct = r
 
     def as_dict(self) -> dict:
@@ -237,8 +243,11 @@ class Engine:
     @staticmethod
     def _job_array(jobs: list):
         keep, arr, bufs = [], (capi.MCJob * len(jobs))(), []
+        structs = {}  # one C struct per instrument spec object of the batch
         for i, (model, schemes, instr, mode, cfg, S0, r, q) in enumerate(jobs):
-            s = instr.struct()
+            s = structs.get(id(instr))
+            if s is None:
+                s = structs[id(instr)] = instr.struct()
             b = ResultBuffers(s.n_strikes)
             keep.append((model, s, cfg, instr))
             bufs.append(b)
