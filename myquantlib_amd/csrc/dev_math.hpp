@@ -452,7 +452,10 @@ __device__ static __noinline__ double igamc_cf(double a, double x, double lga) {
 // branch (p > 1/2) is solved on Q(a, x) = q with q passed separately, so a
 // tail probability far below 2^-53 keeps its precision. The device lgamma
 // costs thousands of cycles, so callers pass it (the path kernels read it
-// from a per-table list of lgamma(d/2 + N)).
+// from a per-table list of lgamma(d/2 + N)). Accurate to ~1e-14 for
+// p >= 2^-64, the smallest nonzero uniform the engine draws; far below that
+// (p < ~1e-60 at a > 1) the starting point falls outside the bracket logic's
+// reach and the result is not accurate.
 __device__ static __noinline__ double gamma_inv(double a, double p, double q, double lga) {
   if (p <= 0.0) return 0.0;
   if (q <= 0.0) return __builtin_inf();

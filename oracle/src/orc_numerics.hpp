@@ -301,7 +301,9 @@ static inline double gamma_q(double a, double x) {
 
 // x with P(a, x) = p, q = 1 - p passed separately. The lower tail is solved
 // on P, the upper tail (p > 0.5) on Q = q so that a tail probability far
-// below 2^-53 keeps its precision.
+// below 2^-53 keeps its precision. Pinned against scipy for p >= 2^-64 (the
+// smallest nonzero uniform, tests/golden chi2_low_tail); far below that
+// (p < ~1e-60 at a > 1) it is not accurate.
 static inline double gamma_inv(double a, double p, double q) {
   if (p <= 0.0) return 0.0;
   if (q <= 0.0) return std::numeric_limits<double>::infinity();
