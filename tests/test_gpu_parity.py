@@ -323,3 +323,27 @@ def test_jump_means_from_ten_are_refused(engine):
     ok = engine.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=[100.0], maturity=1.5), capi.GREEK_NONE,
                    mc_config(1 << 12, 50), 100.0, 0.02, 0.0)      # mean 9: inversion, priced
     assert np.isfinite(ok["price"][0])
+
+
+def test_nci_fine_steps_agree_with_fourier(engine):
+    """SchemeNCI at 1,600 steps per year against the Gil-Pelaez prices (Bates,
+    sigma 1, T 0.5, 64 strikes, 2^23 paths). Interpolating the table's first
+    grid interval -- the reference's rule -- lowers every step's variance
+    draw slightly, and the mean-reverting variance adds that up to a drift
+    that grows as the step shrinks. At this size the reference's table
+    semantics (MYQL_FLAG_REFERENCE_NCI_ROWS) put the strikes at mean z
+    +3.2..+3.8 above the Fourier prices (max |z| 6.7, seeds 7 and 42); the
+    engine's default inverts that interval exactly and gives -0.02..+0.57
+    (max |z| 1.4). The strikes share paths, so their mean z is tested, with
+    every |z| < 4."""
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
+    strikes = list(np.linspace(70.0, 130.0, 64))
+    inst = InstrumentSpec(strikes=strikes, maturity=0.5)
+    print("nci fine steps: 2^23 paths x 800 steps", flush=True)
+    mc = engine.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_NONE,
+                   mc_config(1 << 23, 800, seed=42, precision=capi.FP32, num_streams=1 << 18), 100.0, 0.02, 0.0)
+    fo = engine.fourier(m, inst, capi.GREEK_NONE, 100.0, 0.02, 0.0)
+    ok = fo["price"] >= 0.1
+    z = ((mc["price"] - fo["price"]) / mc["price_std_err"])[ok]
+    assert abs(z.mean()) < 2.0 and np.abs(z).max() < 4.0, (z.mean(), np.abs(z).max())
