@@ -163,3 +163,20 @@ def test_bad_arguments_raise(engine):
     with pytest.raises(EngineError):
         engine.fourier(m, InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0),
                        capi.GREEK_NONE, S0, R, Q)
+
+
+@pytest.mark.parametrize("layout,seed", [(capi.STREAMS_JUMP, 42), (capi.STREAMS_SEEDED, 42), (capi.STREAMS_JUMP, 7)])
+def test_heston_call_mc_within_3se_of_fourier(engine, layout, seed):
+    """The plain Heston European call (kappa 2, theta 0.04, sigma 0.3, rho -0.7,
+    v0 0.04, S0 = K = 100, T 1, r 0.02, q 0; SchemeNV, FP64, 2^20 paths x 50
+    steps, GreekMode::Essential): price, delta and gamma of independent
+    stream sets within 3 standard errors of the Gil-Pelaez values."""
+    from myquantlib_amd.pricing import mc_config
+    m = make_model(1, [H1])
+    inst = InstrumentSpec(strikes=[100.0], maturity=1.0)
+    mc = engine.mc(m, [capi.SCHEME_NV], inst, capi.GREEK_ESSENTIAL,
+                   mc_config(1 << 20, 50, seed=seed, stream_layout=layout), 100.0, 0.02, 0.0)
+    fo = engine.fourier(m, inst, capi.GREEK_FULL, 100.0, 0.02, 0.0)
+    for f in ("price", "delta", "gamma"):
+        z = (mc[f][0] - fo[f][0]) / mc[f + "_std_err"][0]
+        assert abs(z) < 3.0, (f, mc[f][0], fo[f][0], z)
