@@ -733,6 +733,14 @@ template <typename R, int BLK, int UNR>
 __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
                                            const int *__restrict__ valid, int tid, double *__restrict__ accg,
                                            bool first) {
+  // An opaque zero on the lane index keeps the loop-invariant parts of this
+  // function (strike, accumulator slot addresses) from being hoisted out of
+  // the sample loop, where they would stay live across the whole path
+  // simulation: 155 -> 130 VGPRs for the Bates strip kernel, and the
+  // four-wave build's spills drop from 26 to 3.
+  int zero = 0;
+  asm volatile("" : "+s"(zero));
+  tid += zero;
   const int nkp = J->nkp;
   const int j = tid & (nkp - 1);
   const int grp = tid / nkp;
