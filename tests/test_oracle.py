@@ -188,6 +188,22 @@ def test_nci_table_row_means(oracle):
         assert lo < rel < hi, (N, rel)
 
 
+def test_acklam_and_giles_bit_exact_with_reference(oracle):
+    """The oracle's Acklam inverse normal and Giles inverse Poisson CDF equal
+    the reference's own Numerics::normcdfinv_as241 / Numerics::poissinvcdf
+    bit for bit (tests/golden/numerics_ref.json, from math/Numerics.hpp
+    compiled with the same IEEE flags by tests/golden/gen_numerics_ref.py):
+    465 probabilities over both branches, their boundary and tails to 1e-295,
+    2,920 (lambda, U) pairs over the summation, asymptotic and tail branches
+    (lambda 1e-3 .. 1e5, U from 1e-300 to 1 - 1e-12)."""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "numerics_ref.json")) as f:
+        d = json.load(f)
+    z = np.array([oracle.normcdfinv(x) for x in d["p"]])
+    assert np.array_equal(z, np.array(d["z"]))
+    n = np.array([oracle.poissinvcdf(a, b) for a, b in zip(d["lam"], d["u"])])
+    assert np.array_equal(n, np.array(d["n"]))
+
+
 def test_fourier_oracle_matches_reference_pricer():
     """The oracle's Gil-Pelaez prices (QUADPACK integrals of the restated
     characteristic functions) against the reference's own FourierPricer,
