@@ -64,6 +64,9 @@ class Oracle:
         lib.oracle_cir_terminal.restype = C.c_int
         lib.oracle_cir_terminal.argtypes = [C.c_int32, C.POINTER(capi.HestonParams), C.c_double,
                                             C.c_int64, C.c_int64, C.c_uint64, dp, C.c_int]
+        lib.oracle_path_payoffs.restype = C.c_int
+        lib.oracle_path_payoffs.argtypes = [C.POINTER(capi.Instrument), C.c_int32, C.c_double, C.c_double, dp,
+                                            C.c_int64, C.c_double, dp, dp, dp]
         lib.oracle_nci_lookup.restype = C.c_int
         lib.oracle_nci_lookup.argtypes = [C.POINTER(capi.HestonParams), C.c_double, C.c_int, C.c_int32, dp,
                                           C.c_int64, dp]
@@ -162,6 +165,20 @@ class Oracle:
         out = np.zeros(n, dtype=np.int32)
         self.lib.oracle_poisson_samples(mean, n, seed, out.ctypes.data_as(C.POINTER(C.c_int32)))
         return out
+
+    def path_payoffs(self, instr, mode: int, S0: float, fd_bump: float, logS, dt: float):
+        """Trackers and payoffs of `instr` (an InstrumentSpec) on the log-price
+        path logS[0..n]: (base, up, dn) arrays over the strikes."""
+        st = instr.struct()
+        logS = np.ascontiguousarray(logS, dtype=np.float64)
+        n = st.n_strikes
+        base, up, dn = np.zeros(n), np.zeros(n), np.zeros(n)
+        pd = C.POINTER(C.c_double)
+        rc = self.lib.oracle_path_payoffs(C.byref(st), mode, S0, fd_bump, logS.ctypes.data_as(pd), logS.size - 1, dt,
+                                          base.ctypes.data_as(pd), up.ctypes.data_as(pd), dn.ctypes.data_as(pd))
+        if rc != 0:
+            raise EngineError(rc, self.lib.oracle_last_error().decode())
+        return base, up, dn
 
     def nci_lookup(self, h, dt: float, N: int, u, reference_rows: bool = False) -> np.ndarray:
         """The NCI table's chi2_{d+2N} quantile (before scaling) at uniforms u."""

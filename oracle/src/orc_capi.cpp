@@ -239,6 +239,39 @@ int oracle_cir_terminal(int32_t scheme, const myql_heston_params *p, double dt, 
   }
 }
 
+// Trackers and Instrument::calculate_to_buffer on a given log-price path
+// logS[0..n_steps] (ASVJStepper::start_path / multi_step / finalize order,
+// total time n_steps * dt): base / up / dn payoffs per strike, spot bump
+// h = S0 * fd_bump, the instrument's own payoff_eps as the smoothing width.
+// Tracker and payoff check only.
+int oracle_path_payoffs(const myql_instrument *instr, int32_t greek_mode, double S0, double fd_bump,
+                        const double *logS, int64_t n_steps, double dt, double *base, double *up, double *dn) {
+  try {
+    if (!instr || !logS || !base || n_steps < 1) throw Error(MYQL_E_INVALID_ARG, "bad arguments");
+    Instr I;
+    I.in = *instr;
+    I.K.assign(instr->strikes, instr->strikes + instr->n_strikes);
+    I.eps = instr->payoff_eps;
+    const double h = S0 * fd_bump;
+    const TrackerCfg c = tracker_config(I, greek_mode, S0, h);
+    PathState s;
+    s.logS = logS[0];
+    trk_init(s, c, s.logS, dt);
+    const bool pd = path_dependent(c.kind);
+    for (int64_t k = 1; k <= n_steps; ++k) {
+      s.logS = logS[k];
+      if (pd) trk_update(s, c, dt);
+    }
+    trk_finalize(s, c, dt, static_cast<double>(n_steps) * dt);
+    std::vector<double> u(I.K.size()), d(I.K.size());
+    to_buffer(I, greek_mode, s, S0, h, base, up ? up : u.data(), dn ? dn : d.data());
+    return MYQL_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  }
+}
+
 // The NCI scheme's table value for row N (the chi2_{d+2N} quantile before
 // the scale factor) at the uniforms u[0..n), from the table a pricing call
 // with this factor and step builds (reference_rows: the reference's rows and

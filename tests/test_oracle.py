@@ -204,6 +204,43 @@ def test_acklam_and_giles_bit_exact_with_reference(oracle):
     assert np.array_equal(n, np.array(d["n"]))
 
 
+def _fixture_instrument(d):
+    kind = {"european": capi.INSTR_EUROPEAN, "geo_asian": capi.INSTR_GEO_ASIAN,
+            "arith_asian": capi.INSTR_ARITH_ASIAN, "lookback": capi.INSTR_LOOKBACK, "barrier": capi.INSTR_BARRIER}
+    pay = {"vanilla": capi.PAYOFF_VANILLA, "cash": capi.PAYOFF_CASH, "asset": capi.PAYOFF_ASSET}
+    reb = {"none": capi.REBATE_NONE, "at_hit": capi.REBATE_AT_HIT, "at_maturity": capi.REBATE_AT_MATURITY}
+    return InstrumentSpec(
+        kind=kind[d["kind"]], payoff=pay[d.get("payoff", "vanilla")],
+        option_type=capi.CALL if d["type"] == "call" else capi.PUT, strikes=d["strikes"], maturity=d["maturity"],
+        is_strip=len(d["strikes"]) > 1, fixed_strike=d.get("fixed_strike", True),
+        barrier_dir=capi.BARRIER_UP if d.get("dir", "up") == "up" else capi.BARRIER_DOWN,
+        barrier_action=capi.BARRIER_OUT if d.get("action", "out") == "out" else capi.BARRIER_IN,
+        rebate_timing=reb[d.get("rebate_timing", "none")], barrier=d.get("barrier", 0.0),
+        rebate=d.get("rebate", 0.0), rebate_rate=d.get("rebate_rate", 0.0), payoff_eps=d.get("payoff_eps", -1.0),
+        past_avg_log=d.get("past_avg_log", 0.0), past_avg_S=d.get("past_avg_S", 0.0),
+        t_elapsed=d.get("t_elapsed", 0.0), hist_min_S=d.get("hist_min_S", 0.0), hist_max_S=d.get("hist_max_S", 0.0))
+
+
+def test_trackers_and_payoffs_bit_exact_with_reference(oracle):
+    """The oracle's trackers and Instrument::calculate_to_buffer equal the
+    reference's own (PathTrackers.hpp, Payoffs.hpp and the four option
+    headers, compiled from its headers; tests/golden/instruments_ref.json by
+    tests/golden/gen_instruments_ref.py) bit for bit: base, spot-up and
+    spot-down payoffs of 21 instruments -- European vanilla and smoothed
+    digital strips, geometric and arithmetic Asians fixed and floating,
+    lookbacks fixed and floating, the four barrier kinds with and without
+    rebates, in-flight Asians and lookbacks -- on 8 fixed 50-step paths,
+    two of which cross both barriers."""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "instruments_ref.json")) as f:
+        fx = json.load(f)
+    dt = fx["T"] / fx["n_steps"]
+    for c in fx["cases"]:
+        b, u, d = oracle.path_payoffs(_fixture_instrument(c["spec"]), capi.GREEK_ESSENTIAL, fx["S0"], fx["fd_bump"],
+                                      c["logS"], dt)
+        for got, ref in ((b, c["base"]), (u, c["up"]), (d, c["dn"])):
+            assert np.array_equal(got, np.asarray(ref)), (c["name"], c["path"], got, ref)
+
+
 def test_fourier_oracle_matches_reference_pricer():
     """The oracle's Gil-Pelaez prices (QUADPACK integrals of the restated
     characteristic functions) against the reference's own FourierPricer,
