@@ -231,3 +231,20 @@ def test_matches_reference_fourier_pricer_fixture(engine):
     print("max |engine - reference| / max(1, |reference|):", worst)
     for k, w in worst.items():
         assert w < 3e-6, (k, w)
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
+def test_mlmc_european_within_3se_of_fourier(engine, prec):
+    """Multilevel Monte Carlo of the Heston European call (SchemeNV, levels
+    2..6 with 4 * 2^l fine steps, epsilon 2e-3) against the Gil-Pelaez price:
+    within 3 standard errors plus 2e-3 for the finest level's bias (the
+    reference's mlmc_verifier.cpp:27-45 check)."""
+    from myquantlib_amd.pricing import mlmc_config
+    m = make_model(1, [H1])
+    inst = InstrumentSpec(strikes=[100.0], maturity=1.0)
+    res = engine.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE,
+                      mlmc_config(epsilon=2e-3, L_min=2, L_max=6, N0=20000, base_steps=4, precision=prec,
+                                  num_streams=1 << 16), 100.0, 0.02, 0.0)
+    fo = engine.fourier(m, inst, capi.GREEK_NONE, 100.0, 0.02, 0.0)
+    assert abs(res["price"] - fo["price"][0]) < 3.0 * res["price_std_err"] + 2e-3, (res["price"], fo["price"][0],
+                                                                                    res["price_std_err"])
