@@ -383,9 +383,9 @@ def mlmc_leg(eng):
     sigma 0.3, rho -0.7, v0 0.04; SchemeNV), K 100, B 130, T 1: levels 0..6
     with 4 * 2^l fine steps, epsilon = 1e-4 x the price of a plain pilot run,
     Giles' sample sizes, FP32 paths, FP64 level sums, seed 42; each level a
-    replayed HIP graph. 196,608 streams: one lane each, three wavefronts per
-    SIMD (the level kernel's 133 VGPRs allow three), so a level launch fills
-    the chip; the library default of 65,536 leaves it at one."""
+    replayed HIP graph. 262,144 streams: one lane each, four wavefronts per
+    SIMD (the level kernel is compiled for four), so a level launch fills the
+    chip; the library default of 65,536 leaves it at one."""
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import Heston, InstrumentSpec, make_model, mc_config, mlmc_config
     m = make_model(1, [Heston(2.0, 0.04, 0.3, -0.7, 0.04)])
@@ -394,7 +394,7 @@ def mlmc_leg(eng):
     pilot = eng.mc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE,
                    mc_config(1 << 20, 256, seed=1, precision=capi.FP32), S0, R, Q)
     eps = 1e-4 * float(pilot["price"][0])
-    streams = 3 * 65536
+    streams = 4 * 65536
     cfg = mlmc_config(epsilon=eps, L_min=6, L_max=6, N0=10000, base_steps=4, seed=42,
                       precision=capi.FP32, num_streams=streams)
     eng.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE,
@@ -407,7 +407,7 @@ def mlmc_leg(eng):
           f"{res['price_std_err']:.2e}, {res['total_steps']:.3e} path-steps", file=sys.stderr,
           flush=True)
     return {"workload": "Heston NV up-and-out call K 100 B 130 T 1, levels 0..6, base 4 steps, "
-                        "eps 1e-4 x pilot price, FP32 paths, FP64 sums, seed 42, 196,608 streams",
+                        "eps 1e-4 x pilot price, FP32 paths, FP64 sums, seed 42, 262,144 streams",
             "epsilon": eps, "price": res["price"], "price_std_err": res["price_std_err"],
             "ms": wall, "device_ms": res["device_ms"], "total_path_steps": res["total_steps"],
             "path_steps_per_s": res["total_steps"] / (wall / 1e3),
