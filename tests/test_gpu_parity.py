@@ -364,3 +364,30 @@ def test_giles_and_acklam_against_reference_numerics(engine):
     z64, _ = engine.normcdfinv(np.array(d["p"]))
     ref = np.array(d["z"])
     assert np.max(np.abs(z64 - ref) / np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["geo_in_flight", "arith_in_flight", "lookback_float_in_flight",
+                                  "barrier_di_rebate_at_maturity"])
+def test_in_flight_and_rebate_variants_fp64(engine, oracle, kind):
+    """In-flight Asians (observed averages over an elapsed period), an
+    in-flight floating lookback (observed extremes) and a down-and-in put with
+    a rebate paid at maturity: FP64 engine against the oracle on identical
+    draws (the oracle's trackers and payoffs equal the reference's own on
+    fixed paths, test_oracle.py)."""
+    m = make_model(1, [H1])
+    spec = dict(strikes=[95.0, 105.0], maturity=1.0)
+    if kind == "geo_in_flight":
+        inst = InstrumentSpec(kind=capi.INSTR_GEO_ASIAN, past_avg_log=float(np.log(95.0)), t_elapsed=0.25, **spec)
+    elif kind == "arith_in_flight":
+        inst = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, past_avg_S=97.0, past_avg_log=float(np.log(97.0)),
+                              t_elapsed=0.5, **spec)
+    elif kind == "lookback_float_in_flight":
+        inst = InstrumentSpec(kind=capi.INSTR_LOOKBACK, fixed_strike=False, hist_min_S=90.0, hist_max_S=115.0,
+                              **spec)
+    else:
+        inst = InstrumentSpec(kind=capi.INSTR_BARRIER, barrier=90.0, barrier_dir=capi.BARRIER_DOWN,
+                              barrier_action=capi.BARRIER_IN, option_type=capi.PUT,
+                              rebate_timing=capi.REBATE_AT_MATURITY, rebate=1.5, rebate_rate=0.03, **spec)
+    cfg = mc_config(1 << 12, 24, seed=11, num_streams=256)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NV], inst, capi.GREEK_FULL, cfg)
+    compare(g, c, FULL, 1e-9)
