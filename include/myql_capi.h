@@ -346,6 +346,17 @@ int myql_debug_poissinv(myql_ctx *ctx, const double *lambda, const double *u,
 /* Device chi-squared quantile (inverse regularised incomplete gamma). */
 int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
                              int64_t n, double *out);
+/* Device trackers and payoffs on given log-price paths: logS holds n_paths
+ * rows of n_steps + 1 values (log S at t = 0, dt, ..., T; dt = the
+ * instrument's maturity / n_steps); per path and strike the base, spot-up
+ * and spot-down payoffs of Instrument::calculate_to_buffer (rows of
+ * n_strikes in base / up / dn) after the tracker's init, per-step update
+ * and finalize (PathTrackers.hpp and the option headers; FP64). */
+int myql_debug_path_payoffs(myql_ctx *ctx, const myql_instrument *instrument,
+                            int32_t greek_mode, double S0, double fd_bump,
+                            const double *logS, int64_t n_paths,
+                            int64_t n_steps, double *base, double *up,
+                            double *dn);
 /* Fix the launch shape of one path kernel on this context, as if the
  * autotuner had chosen it: precision MYQL_FP32/FP64, factor count, lockstep
  * width (1 for replays; 5 / 7 for 0-1 / 2-factor lockstep Europeans; 7 / 9

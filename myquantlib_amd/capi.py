@@ -132,7 +132,7 @@ EXPORTED = [
     "myql_create", "myql_destroy", "myql_last_error", "myql_tuning_report", "myql_mc_calculate",
     "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_fourier_calculate_batch", "myql_cv_train",
     "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
-    "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_set_launch_shape",
+    "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
     "myql_debug_cv_input_grad",
     "myql_debug_cv_param_grad",
 ]
@@ -178,6 +178,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_poissinv.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
     lib.myql_debug_chi2_quantile.restype = C.c_int
     lib.myql_debug_chi2_quantile.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
+    lib.myql_debug_path_payoffs.restype = C.c_int
+    lib.myql_debug_path_payoffs.argtypes = [vp, C.POINTER(Instrument), C.c_int32, C.c_double, C.c_double, _PD,
+                                            C.c_int64, C.c_int64, _PD, _PD, _PD]
     lib.myql_debug_set_launch_shape.restype = C.c_int
     lib.myql_debug_set_launch_shape.argtypes = [vp] + [C.c_int32] * 8
     pf = C.POINTER(C.c_float)

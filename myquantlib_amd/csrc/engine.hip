@@ -1980,6 +1980,96 @@ static int debug_map(myql_ctx *c, int which, const double *a, const double *b, i
   }
 }
 
+// The device trackers and calculate_to_buffer on given log-price paths: one
+// lane per path runs trk_init, trk_update after every step and
+// finalize_stage as the FP64 path kernel does, then eval3 per strike.
+constexpr int DBG_PATH_BLK = 64;
+__global__ void __launch_bounds__(DBG_PATH_BLK)
+    debug_path_payoffs_kernel(const Job<double> *__restrict__ J, const double *__restrict__ logS, int n_paths,
+                              int n_steps, double *__restrict__ base, double *__restrict__ up,
+                              double *__restrict__ dn) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double *stage = reinterpret_cast<double *>(smem);
+  const int tid = threadIdx.x;
+  const int path = blockIdx.x * DBG_PATH_BLK + tid;
+  if (path >= n_paths) return;  // each lane stages and reads only its own slots
+  const VarParams<double> *P = &J->var[V_BASE];
+  const double dt = P->dt;
+  const double *x = logS + static_cast<size_t>(path) * (n_steps + 1);
+  Trk<double> tr;
+  trk_init(J, tr, x[0], dt);
+  for (int k = 1; k <= n_steps; ++k) trk_update(J, tr, x[k], dt);
+  finalize_stage<double, DBG_PATH_BLK>(J, tr, x[n_steps], dt, P->total_time, stage, V_BASE, tid);
+  const bool bumps = J->mode != 0;
+  for (int j = 0; j < J->nk; ++j) {
+    double p = 0.0, pu = 0.0, pd = 0.0;
+    eval3<double, DBG_PATH_BLK>(J, stage, V_BASE, tid, J->strikes[j], bumps, p, pu, pd);
+    const size_t o = static_cast<size_t>(path) * J->nk + j;
+    base[o] = p;
+    up[o] = pu;
+    dn[o] = pd;
+  }
+}
+
+int myql_debug_path_payoffs(myql_ctx *c, const myql_instrument *in, int32_t greek_mode, double S0, double fd_bump,
+                            const double *logS, int64_t n_paths, int64_t n_steps, double *base, double *up,
+                            double *dn) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!in || !logS || !base || !up || !dn || n_paths <= 0 || n_steps <= 0 || n_paths > (1 << 20) ||
+        n_steps > (1 << 20))
+      throw Fail(MYQL_E_INVALID_ARG, "debug_path_payoffs: bad arguments");
+    // the paths' own dt is maturity / n_steps, as in a pricing call; a
+    // zero-factor model only carries the instrument through build_job
+    myql_model m{};
+    m.n_factors = 0;
+    m.vol = 0.2;
+    m.jump_kind = MYQL_JUMP_NONE;
+    const int32_t schemes[2] = {MYQL_SCHEME_NULL, MYQL_SCHEME_NULL};
+    myql_mc_config cfg{};
+    cfg.num_paths = static_cast<uint64_t>(n_paths);
+    cfg.time_steps = static_cast<uint64_t>(n_steps);
+    cfg.seed = 1;
+    cfg.fd_bump = fd_bump;
+    cfg.vol_bump = cfg.t_bump = cfg.r_bump = 1e-4;
+    cfg.precision = MYQL_FP64;
+    cfg.stream_layout = MYQL_STREAMS_JUMP;
+    cfg.num_streams = 1;
+    validate(&m, schemes, in, &cfg);
+    JobSpec js{&m, schemes, in, greek_mode, &cfg, S0, 0.0, 0.0, nullptr, 0, static_cast<int>(in->n_strikes)};
+    Job<double> J;
+    std::vector<double> ks;
+    build_job<double>(c, js, J, ks);
+    const size_t nk = ks.size(), nx = static_cast<size_t>(n_paths) * (n_steps + 1),
+                 no = static_cast<size_t>(n_paths) * nk;
+    DevBuf dj, dk, dx, dout;
+    dj.reserve(sizeof(Job<double>));
+    dk.reserve(nk * 8);
+    dx.reserve(nx * 8);
+    dout.reserve(3 * no * 8);
+    J.strikes = dk.as<double>();
+    HIPCHECK(hipMemcpyAsync(dk.p, ks.data(), nk * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(dj.p, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(dx.p, logS, nx * 8, hipMemcpyHostToDevice, c->stream));
+    const size_t smem = static_cast<size_t>(J.n_fields) * MAXV * DBG_PATH_BLK * sizeof(double);
+    if (smem > c->lds_max) throw Fail(MYQL_E_UNSUPPORTED, "staging area exceeds the device's LDS per block");
+    const unsigned nb = static_cast<unsigned>((n_paths + DBG_PATH_BLK - 1) / DBG_PATH_BLK);
+    double *o = dout.as<double>();
+    hipLaunchKernelGGL(debug_path_payoffs_kernel, dim3(nb), dim3(DBG_PATH_BLK), smem, c->stream, dj.as<Job<double>>(),
+                       dx.as<double>(), static_cast<int>(n_paths), static_cast<int>(n_steps), o, o + no, o + 2 * no);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(base, o, no * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(up, o + no, no * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(dn, o + 2 * no, no * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
 int myql_debug_normcdfinv(myql_ctx *c, const double *p, int64_t n, double *out64, float *out32) {
   if (!p || !out64 || !out32) return MYQL_E_INVALID_ARG;
   return debug_map(c, 0, p, nullptr, n, out64, out32);

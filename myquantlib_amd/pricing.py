@@ -384,6 +384,19 @@ class Engine:
         self._check(rc)
         return out
 
+    def path_payoffs(self, instr: "InstrumentSpec", mode: int, S0: float, fd_bump: float, logS) -> tuple:
+        """Device trackers and payoffs on log-price paths logS (n_paths x
+        (n_steps + 1)): (base, up, dn), each n_paths x n_strikes."""
+        logS = np.ascontiguousarray(np.atleast_2d(logS), dtype=np.float64)
+        st = instr.struct()
+        n_paths, n_steps = logS.shape[0], logS.shape[1] - 1
+        out = [np.zeros((n_paths, st.n_strikes)) for _ in range(3)]
+        pd = C.POINTER(C.c_double)
+        rc = self.lib.myql_debug_path_payoffs(self.ctx, C.byref(st), mode, S0, fd_bump, logS.ctypes.data_as(pd),
+                                              n_paths, n_steps, *[o.ctypes.data_as(pd) for o in out])
+        self._check(rc)
+        return tuple(out)
+
     def chi2_quantile(self, dof: np.ndarray, p: np.ndarray) -> np.ndarray:
         dof = np.ascontiguousarray(dof, dtype=np.float64)
         p = np.ascontiguousarray(p, dtype=np.float64)

@@ -391,3 +391,41 @@ def test_in_flight_and_rebate_variants_fp64(engine, oracle, kind):
     cfg = mc_config(1 << 12, 24, seed=11, num_streams=256)
     g, c = _run_both(engine, oracle, m, [capi.SCHEME_NV], inst, capi.GREEK_FULL, cfg)
     compare(g, c, FULL, 1e-9)
+
+
+def test_device_trackers_and_payoffs_match_reference_fixture(engine):
+    """The device's trackers and calculate_to_buffer (FP64) on the fixed
+    log-price paths of tests/golden/instruments_ref.json, the reference's own
+    trackers, payoffs and option headers compiled from its headers: base,
+    spot-up and spot-down payoffs of 21 instruments on 8 paths within 1e-13
+    of max(1, |value|) (FMA contraction on the device; the oracle equals the
+    fixture bit for bit)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "instruments_ref.json")) as f:
+        fx = json.load(f)
+    kind = {"european": capi.INSTR_EUROPEAN, "geo_asian": capi.INSTR_GEO_ASIAN,
+            "arith_asian": capi.INSTR_ARITH_ASIAN, "lookback": capi.INSTR_LOOKBACK, "barrier": capi.INSTR_BARRIER}
+    pay = {"vanilla": capi.PAYOFF_VANILLA, "cash": capi.PAYOFF_CASH, "asset": capi.PAYOFF_ASSET}
+    reb = {"none": capi.REBATE_NONE, "at_hit": capi.REBATE_AT_HIT, "at_maturity": capi.REBATE_AT_MATURITY}
+    worst = 0.0
+    for c in fx["cases"]:
+        d = c["spec"]
+        inst = InstrumentSpec(
+            kind=kind[d["kind"]], payoff=pay[d.get("payoff", "vanilla")],
+            option_type=capi.CALL if d["type"] == "call" else capi.PUT, strikes=d["strikes"],
+            maturity=d["maturity"], is_strip=len(d["strikes"]) > 1, fixed_strike=d.get("fixed_strike", True),
+            barrier_dir=capi.BARRIER_UP if d.get("dir", "up") == "up" else capi.BARRIER_DOWN,
+            barrier_action=capi.BARRIER_OUT if d.get("action", "out") == "out" else capi.BARRIER_IN,
+            rebate_timing=reb[d.get("rebate_timing", "none")], barrier=d.get("barrier", 0.0),
+            rebate=d.get("rebate", 0.0), rebate_rate=d.get("rebate_rate", 0.0), payoff_eps=d.get("payoff_eps", -1.0),
+            past_avg_log=d.get("past_avg_log", 0.0), past_avg_S=d.get("past_avg_S", 0.0),
+            t_elapsed=d.get("t_elapsed", 0.0), hist_min_S=d.get("hist_min_S", 0.0),
+            hist_max_S=d.get("hist_max_S", 0.0))
+        b, u, dn = engine.path_payoffs(inst, capi.GREEK_ESSENTIAL, fx["S0"], fx["fd_bump"], [c["logS"]])
+        for got, ref in ((b[0], c["base"]), (u[0], c["up"]), (dn[0], c["dn"])):
+            ref = np.asarray(ref)
+            err = float(np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))))
+            worst = max(worst, err)
+            assert err < 1e-13, (c["name"], c["path"], got, ref)
+    print("worst relative difference", worst)
