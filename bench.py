@@ -51,7 +51,13 @@ VARIANTS = 7
 # and chi-squared inversion, jumps, payoffs, Greek estimators and sums; NCI
 # table set-up excluded), counted by the oracle's op-counting build:
 #   python oracle/count_flops.py --paths-log2 16   -> 103.67
-FLOP_PER_PATH_STEP = 103.67
+FLOP_PER_PATH_STEP = 103.69
+# The same count under the reference's NCI rows and exact chi-squared
+# fallback above them (MYQL_FLAG_REFERENCE_NCI_ROWS; VolSchemes.hpp:245-255,
+# 334-370): the fallback fires on ~4 % of steps and its Halley iterations
+# raise the count:
+#   python oracle/count_flops.py --paths-log2 16 --reference-nci-rows -> 143.61
+FLOP_PER_PATH_STEP_REF_ROWS = 143.61
 # MI355X published FP32 vector peak (packed dual-issue FP32; scalar FP32 and
 # FP64 vector issue peak at 78.6 TFLOP/s)
 FP32_PEAK_TFLOPS = 157.3
@@ -150,15 +156,40 @@ def ms_to_target(results, t_run_ms: float, target: float = 1e-4):
     return t_run_ms * (worst / target) ** 2, worst
 
 
-def cpu_baseline(target_s: float = 12.0):
+def host_cpu():
+    """(nproc, logical CPUs, CPU model name) of this host. `nproc` counts the
+    cores this process may use and honours OMP_NUM_THREADS, as an OpenMP run
+    with the default team size would."""
+    import subprocess
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip())
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        nproc = len(os.sched_getaffinity(0))
+    name = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return nproc, os.cpu_count() or nproc, name
+
+
+def cpu_baseline(target_s: float = 8.0):
     """The oracle compiled with the reference's flags, OpenMP over host cores
     in the reference's structure (per-thread stream seeded seed + tid*999983,
     static partition). Sample: the T = 1 maturity of the strip (100 steps,
-    Full CRN Greeks, 64 strikes), path count sized to ~target_s seconds."""
+    Full CRN Greeks, 64 strikes), path count sized to ~target_s seconds; one
+    warm-up run of that size, then the median of 3 timed runs. Threads =
+    `nproc` (OMP_NUM_THREADS=$(nproc), OMP_PROC_BIND=close)."""
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import InstrumentSpec, mc_config
     from oracle.oracle import Oracle
-    cores = int(os.environ.get("MYQL_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    nproc, logical, cpu_name = host_cpu()
+    cores = int(os.environ.get("MYQL_CPU_THREADS", nproc))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     o = Oracle(fast=True)
     m = model()
     inst = InstrumentSpec(strikes=STRIKES, maturity=1.0)
@@ -177,11 +208,18 @@ def cpu_baseline(target_s: float = 12.0):
     per_path = max((t2 - t1) / (p2 - p1), 1e-9)
     fixed = max(t1 - p1 * per_path, 0.0)
     paths = int(max(p2, min(1 << 23, (target_s - fixed) / per_path)))
-    t = run(paths)
+    run(paths)  # warm-up at the timed size
+    times = sorted(run(paths) for _ in range(3))
+    t = times[1]
     rate = paths * 100 * VARIANTS / t
+    print(f"# cpu baseline: {cores} threads on {cpu_name}, runs {[round(x, 2) for x in times]} s, "
+          f"median {rate:.3e} path-steps/s", file=sys.stderr, flush=True)
     return {"value": rate, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "nproc": nproc, "logical_cpus": logical, "cpu_model": cpu_name,
+            "runs_s": times, "statistic": "median of 3 after 1 warm-up",
             "sample": f"T=1 maturity of the strip: {paths} paths x 100 steps x 7 CRN variants, "
-                      f"64 strikes, Full Greeks, FP64, {t:.1f} s on {platform.processor() or 'host'}"}
+                      f"64 strikes, Full Greeks, FP64, median {t:.1f} s of 3 on {cores} threads "
+                      f"({cpu_name})"}
 
 
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
@@ -229,7 +267,7 @@ def _xoshiro_uniforms(seed: int):
             yield u
 
 
-def double_heston_sets(n_sets: int = 1024, seed: int = 2024, log2_paths=(14, 20)):
+def double_heston_sets(n_sets: int = 1024, seed: int = 42, log2_paths=(14, 20)):
     """Parameter sets of the batched workload, drawn in this order per set:
     for factor 1 then factor 2: kappa ~ U[0.5, 5], theta ~ U[0.01, 0.09],
     sigma ~ U[0.1, 1], rho ~ U[-0.9, 0], v0 ~ U[0.01, 0.09]; then the path
@@ -330,6 +368,41 @@ def batched_by_scheme(eng, n_sets: int = 1024, steps: int = 252):
     return out
 
 
+def reference_rows_leg(eng, paths: int, flags: int, steps: int = 3, fp64: bool = True):
+    """The headline strip under the reference's NCI semantics
+    (MYQL_FLAG_REFERENCE_NCI_ROWS: the reference's row count from
+    v_max = max(v0, 3 theta), interpolation of every u < 0.999 including the
+    first grid interval, exact chi-squared inversion otherwise;
+    VolSchemes.hpp:245-255, 334-370), FP32 and FP64 paths, same seed and
+    streams: wall ms per strip, path-steps/s and the path kernel's roofline
+    at the reference algorithm's op count."""
+    from myquantlib_amd import capi
+    out = {}
+    total = path_steps(paths)
+    for name, prec, n in (("fp32", capi.FP32, steps), ("fp64", capi.FP64, 1 if fp64 else 0)):
+        if n == 0:
+            continue
+        J = jobs(paths, prec, flags=flags | capi.FLAG_REFERENCE_NCI_ROWS)
+        eng.mc_batch(J)  # warm-up (launch shape, tables)
+        dev = []
+        t0 = time.perf_counter()
+        for _ in range(n):
+            res = eng.mc_batch(J)
+            dev.append(res[0]["device_ms"])
+        ms = 1e3 * (time.perf_counter() - t0) / n
+        kern_s = float(np.mean(dev)) / 1e3
+        ach = total * FLOP_PER_PATH_STEP_REF_ROWS / kern_s / 1e12
+        _, worst = ms_to_target(res, ms)
+        print(f"# reference NCI rows, {name} strip: {ms:.1f} ms/step, device {np.mean(dev):.1f} ms",
+              file=sys.stderr, flush=True)
+        out[name] = {"ms_per_step": ms, "device_ms": float(np.mean(dev)), "path_steps_per_s": total / (ms / 1e3),
+                     "worst_rel_se": worst,
+                     "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": ach / FP32_PEAK_TFLOPS,
+                                  "flop_per_path_step": FLOP_PER_PATH_STEP_REF_ROWS}}
+    return out
+
+
 def fp32_vs_fp64(r32, r64):
     """Error of the FP32-path strip against the FP64-path strip on identical
     draws (same seed and streams; sums are FP64 in both), per result field:
@@ -427,6 +500,8 @@ def main():
     ap.add_argument("--no-mlmc", action="store_true", help="skip the multilevel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Double-Heston leg")
     ap.add_argument("--no-fourier", action="store_true", help="skip the Gil-Pelaez leg")
+    ap.add_argument("--no-ref-rows", action="store_true",
+                    help="skip the strip timed under the reference's NCI semantics")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the strip kernel at its default launch shape (256-stream units, "
                          "strike loop unrolled by 4, grid 4x) instead of autotuning it")
@@ -490,6 +565,7 @@ def main():
           f"worst rel se {worst32:.2e}", file=sys.stderr, flush=True)
 
     fourier_leg = None if args.no_fourier else fourier_check(eng, res)
+    ref_rows = None if args.no_ref_rows else reference_rows_leg(eng, paths, flags, fp64=not args.no_fp64)
 
     ms64 = None
     fp32_err = None
@@ -545,6 +621,10 @@ def main():
             "workload": "bates_strip: Bates SchemeNCI (Feller violated) + Merton, 64 strikes x 8 "
                         f"maturities, 2^{args.paths_log2} paths per maturity on {STRIP_STREAMS} streams, "
                         "100 steps/yr, GreekMode::Full CRN, FP32 paths, FP64 sums, seed 42",
+            "nci_semantics": "engine default: NCI rows to lambda(max(v0, 25 theta)) + 6 sqrt(lambda), upper-tail "
+                             "table for u >= 0.999, first grid interval inverted exactly for rows N >= 2; the "
+                             "reference's rows and fallback (VolSchemes.hpp:245-255,334-370) are timed under "
+                             "reference_nci_rows",
             "paths_per_maturity": paths,
             "path_steps_per_step": total,
             "launch_shape": ("autotuned: " + eng.tuning_report().strip()) if flags else
@@ -562,6 +642,7 @@ def main():
         "mlmc": mlmc,
         "batched_double_heston": batched,
         "fourier": fourier_leg,
+        "reference_nci_rows": ref_rows,
         "sample_steps_per_s": value / VARIANTS,
         "executed_variant_fraction": EXECUTED_VARIANTS / VARIANTS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,

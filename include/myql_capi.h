@@ -351,7 +351,10 @@ int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
  * instrument's maturity / n_steps); per path and strike the base, spot-up
  * and spot-down payoffs of Instrument::calculate_to_buffer (rows of
  * n_strikes in base / up / dn) after the tracker's init, per-step update
- * and finalize (PathTrackers.hpp and the option headers; FP64). */
+ * and finalize (PathTrackers.hpp and the option headers; FP64). There is
+ * no model, so no automatic smoothing bandwidth: a cash/asset digital under
+ * MYQL_GREEK_ESSENTIAL or MYQL_GREEK_FULL needs payoff_eps > 0
+ * (MYQL_E_INVALID_ARG otherwise). */
 int myql_debug_path_payoffs(myql_ctx *ctx, const myql_instrument *instrument,
                             int32_t greek_mode, double S0, double fd_bump,
                             const double *logS, int64_t n_paths,

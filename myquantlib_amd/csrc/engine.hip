@@ -49,14 +49,22 @@ struct DevBuf {
   size_t n = 0;
   void reserve(size_t bytes) {
     if (bytes <= n) return;
-    if (p) (void)hipFree(p);
+    // a queued kernel may still read the old buffer: wait for the device
+    // before freeing it (growth only, so this is rare)
+    if (p) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
     HIPCHECK(hipMalloc(&p, bytes));
     n = bytes;
   }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipDeviceSynchronize();  // an error path may leave work queued
+      (void)hipFree(p);
+    }
   }
   template <typename T>
   T *as() const { return static_cast<T *>(p); }
@@ -1100,6 +1108,7 @@ struct LevelGraph {
   hipGraph_t g = nullptr;
   hipGraphExec_t exec = nullptr;
   ~LevelGraph() {
+    if (exec) (void)hipDeviceSynchronize();  // no replay may be in flight
     if (exec) (void)hipGraphExecDestroy(exec);
     if (g) (void)hipGraphDestroy(g);
   }
@@ -1509,9 +1518,14 @@ void cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_con
   }
   hipError_t ge = hipSuccess;
   for (uint32_t it = 0; it < cfg->steps && ge == hipSuccess; ++it) ge = hipGraphLaunch(exec, c->stream);
+  // the replays must have finished before their executable graph is
+  // destroyed: the runtime (and a profiler's tracing thread) still reads a
+  // queued launch's dispatch data from it
+  const hipError_t se = hipStreamSynchronize(c->stream);
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
   HIPCHECK(ge);
+  HIPCHECK(se);
   HIPCHECK(hipEventRecord(c->ev1, c->stream));
   std::vector<float> h(cfg->steps);
   HIPCHECK(hipMemcpyAsync(net->params, prm.p, CV_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -2037,6 +2051,10 @@ int myql_debug_path_payoffs(myql_ctx *c, const myql_instrument *in, int32_t gree
     cfg.stream_layout = MYQL_STREAMS_JUMP;
     cfg.num_streams = 1;
     validate(&m, schemes, in, &cfg);
+    // no model here, so no automatic bandwidth: a smoothed digital needs
+    // its bandwidth given
+    if (greek_mode != MYQL_GREEK_NONE && in->payoff != MYQL_PAYOFF_VANILLA && !(in->payoff_eps > 0.0))
+      throw Fail(MYQL_E_INVALID_ARG, "debug_path_payoffs: a digital payoff in a Greek mode needs payoff_eps > 0");
     JobSpec js{&m, schemes, in, greek_mode, &cfg, S0, 0.0, 0.0, nullptr, 0, static_cast<int>(in->n_strikes)};
     Job<double> J;
     std::vector<double> ks;
