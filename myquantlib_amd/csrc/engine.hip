@@ -437,9 +437,22 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   return ni;
 }
 
-// Table fields of a job factor; the tail rows and the exact inversion of the
-// first grid interval are used unless the config asks for the reference's
-// table semantics.
+// Under the reference's table semantics the rows above its row count and the
+// upper tail u >= 0.999 are exact chi2 quantiles (VolSchemes.hpp:362-369).
+// The engine evaluates them from the extended rows (exact slopes) for u in
+// [NCI_U_ACC grid steps, NCI_U_ACC_HI), and from the tail rows for
+// u >= 0.999, whose interpolation agrees with the exact inversion to ~1e-11
+// and ~3e-11 relative there (the iterative inversion costs a whole
+// wavefront several thousand cycles and fired on ~4 % of the Bates strip's
+// steps); nearer u = 0 and in [NCI_U_ACC_HI, 0.999), where the uniform grid
+// is coarse against the quantile's curvature, the rows above the
+// reference's count are still inverted exactly.
+constexpr int NCI_U_ACC = 100;
+constexpr double NCI_U_ACC_HI = 0.99;
+
+// Table fields of a job factor; the exact inversion of the first grid
+// interval is used unless the config asks for the reference's table
+// semantics.
 template <typename R>
 void set_nci(Job<R> &J, int f, const NciInfo &ni, bool reference) {
   J.nci[f] = ni.tab;
@@ -448,8 +461,11 @@ void set_nci(Job<R> &J, int f, const NciInfo &ni, bool reference) {
   J.nci_grid = ni.grid;
   J.nci_dx = ni.dx;
   J.nci_inv_dx = ni.inv_dx;
-  J.nci_tail[f] = reference ? nullptr : ni.tail;
+  J.nci_tail[f] = ni.tail;
   J.nci_u_lo = reference ? 0.0 : ni.dx;
+  J.nci_rows_acc[f] = reference ? ni.rows - 1 : -1;
+  J.nci_u_acc = NCI_U_ACC * ni.dx;
+  J.nci_u_acc_hi = NCI_U_ACC_HI;
   J.tail_pts = NCI_TAIL;
   J.tail_w0 = NCI_TAIL_W0;
   J.tail_inv_hw = (NCI_TAIL - 1) / (NCI_TAIL_W1 - NCI_TAIL_W0);
@@ -503,6 +519,8 @@ void fill_variant(VarParams<R> &V, const myql_model &m, const int32_t *schemes, 
     V.gvol = static_cast<R>(gv);
     V.gdrift_n = static_cast<R>(gd * static_cast<double>(steps));
     V.gvol_n = static_cast<R>(gv * std::sqrt(static_cast<double>(steps)));
+    V.gdrift_nd = gd * static_cast<double>(steps);
+    V.gvol_nd = gv * std::sqrt(static_cast<double>(steps));
   } else {
     const double r_part = (r - q) / static_cast<double>(m.n_factors);
     for (int f = 0; f < m.n_factors; ++f) {
@@ -514,7 +532,12 @@ void fill_variant(VarParams<R> &V, const myql_model &m, const int32_t *schemes, 
       V.C[f][1] = static_cast<R>(term - p.rho / p.sigma);
       V.C[f][2] = static_cast<R>(term + p.rho / p.sigma);
       V.C[f][3] = static_cast<R>(h * (1.0 - p.rho * p.rho) * 0.5);
+      V.Cd[f][0] = C0;
+      V.Cd[f][1] = term - p.rho / p.sigma;
+      V.Cd[f][2] = term + p.rho / p.sigma;
+      V.Cd[f][3] = h * (1.0 - p.rho * p.rho) * 0.5;
       V.v0[f] = static_cast<R>(p.v0);
+      V.v0d[f] = p.v0;
       double K[3] = {0, 0, 0};
       if (schemes[f] == MYQL_SCHEME_NV) {
         if (p.sigma * p.sigma > 4.0 * p.kappa * p.theta)
@@ -548,13 +571,15 @@ void fill_variant(VarParams<R> &V, const myql_model &m, const int32_t *schemes, 
     const double lam = m.jump[0], mu = m.jump[1], dl = m.jump[2];
     V.jmean = lam * span;
     V.jexpm = std::exp(-V.jmean);
-    V.jdrift = static_cast<R>(-lam * (std::exp(mu + 0.5 * dl * dl) - 1.0) * span);
+    V.jdrift_d = -lam * (std::exp(mu + 0.5 * dl * dl) - 1.0) * span;
+    V.jdrift = static_cast<R>(V.jdrift_d);
   } else if (m.jump_kind == MYQL_JUMP_KOU) {
     const double lam = m.jump[0], p = m.jump[1], e1 = m.jump[2], e2 = m.jump[3];
     const double k = (p * e1) / (e1 - 1.0) + ((1.0 - p) * e2) / (e2 + 1.0) - 1.0;
     V.jmean = lam * span;
     V.jexpm = std::exp(-V.jmean);
-    V.jdrift = static_cast<R>(-lam * k * span);
+    V.jdrift_d = -lam * k * span;
+    V.jdrift = static_cast<R>(V.jdrift_d);
   } else {
     V.jmean = 0.0;
     V.jexpm = 1.0;
@@ -739,22 +764,25 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
   }
   J.derive_r = full && !pd;
 
-  // NCI tables: one per factor, rows for the largest workspace among variants
+  // NCI tables: one per factor, rows for the largest workspace among
+  // variants (under the reference's semantics the variants' own row counts
+  // are the reference's, and the table still holds the extended rows that
+  // serve its exact fallback)
+  const bool ref_rows = cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS;
   int nmax_max[2] = {0, 0};
   std::vector<std::array<int, 2>> nmaxv(vd.size(), {0, 0});
   for (size_t i = 0; i < vd.size(); ++i)
     for (int f = 0; f < nf; ++f)
       if (js.schemes[f] == MYQL_SCHEME_NCI) {
-        nmaxv[i][f] = nci_nmax(vd[i].m.heston[f], vd[i].dt, cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS);
-        nmax_max[f] = std::max(nmax_max[f], nmaxv[i][f]);
+        nmaxv[i][f] = nci_nmax(vd[i].m.heston[f], vd[i].dt, ref_rows);
+        nmax_max[f] = std::max(nmax_max[f], nci_nmax(vd[i].m.heston[f], vd[i].dt, false));
       }
   for (int f = 0; f < nf; ++f)
     if (js.schemes[f] == MYQL_SCHEME_NCI) {
       const myql_heston_params &p = M0.heston[f];
       const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
-      const bool ref = cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS;
       NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision, nci_nmax(p, dt, true));
-      set_nci(J, f, ni, ref);
+      set_nci(J, f, ni, ref_rows);
     }
   for (size_t i = 0; i < vd.size(); ++i)
     fill_variant(J.var[vd[i].id], vd[i].m, js.schemes, vd[i].dt, vd[i].r, q, vd[i].T, steps, pd,
@@ -1161,8 +1189,8 @@ struct MlmcRun {
         const myql_heston_params &p = m.heston[f];
         nmax[f] = nci_nmax(p, dtf, cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS);
         const bool ref = cfg.flags & MYQL_FLAG_REFERENCE_NCI_ROWS;
-        NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nmax[f], cfg.precision,
-                               nci_nmax(p, dtf, true));
+        NciInfo ni = nci_table(c, 4.0 * p.kappa * p.theta / (p.sigma * p.sigma), nci_nmax(p, dtf, false),
+                               cfg.precision, nci_nmax(p, dtf, true));
         set_nci(J, f, ni, ref);
       }
     fill_variant(J.var[0], m, schemes, dtf, r, q, T, coupled ? fs : steps, pd, nmax);

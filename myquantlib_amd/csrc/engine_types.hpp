@@ -35,6 +35,11 @@ struct VarParams {
   R dt, total_time;
   double jmean, jexpm;  // Poisson mean of the jump span and exp(-mean)
   R jdrift;             // drift compensation over the span
+  // FP64 copies of the log-S coefficients and drifts, for the maturity-bumped
+  // variants that FP32 kernels step in FP64
+  double Cd[2][4];
+  double v0d[2];
+  double jdrift_d, gdrift_nd, gvol_nd;
 };
 
 template <typename R>
@@ -73,6 +78,13 @@ struct Job {
   // u below this (the first grid interval, or 0 = never): rows N >= 2 are
   // inverted exactly instead of interpolated from the table's u = 0 node
   double nci_u_lo;
+  // Rows above a variant's nmax (and u >= 0.999) that the reference inverts
+  // exactly: rows N <= nci_rows_acc[f] at u in [nci_u_acc, nci_u_acc_hi) or
+  // u >= 0.999 are read from the extended rows and the tail rows instead,
+  // where the table agrees with the exact quantile to ~1e-11 relative (-1:
+  // never; the extended-rows default already serves every row it holds)
+  int nci_rows_acc[2];
+  double nci_u_acc, nci_u_acc_hi;
   const double *strikes;
   int nk, nkp;                     // strikes in this job, padded to a power of 2 (<= 64)
   double *partial;                 // per unit: 2*NSTAT x nk sums

@@ -43,6 +43,29 @@ __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
 template <>
 __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
 
+// sqrt in FP64 from the FP32 reciprocal square root and one Newton step
+// (relative error ~5e-15; the library sqrt's scaling and rounding steps cost
+// three times as many instructions)
+__device__ __forceinline__ double sqrt_nr(double a) {
+  if (!(a > 1e-30)) return a > 0.0 ? sqrt(a) : 0.0;
+  const double y = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(a)));
+  return a * (y * (1.5 - 0.5 * a * y * y));
+}
+
+// Lockstep groups of Full-mode Europeans in FP32 kernels keep the two
+// maturity-bumped variants (T+ in slot 1 + 2 max(NF, 1), T- next; the
+// reference's T +- t_bump replays, MonteCarloPricer.hpp:374-395) in FP64:
+// theta and charm difference those two paths and divide by 2 t_bump
+// (2e-4), which FP32 rounding of the log-price would not survive. Their
+// terminal spots keep FP64 precision through the FP32 stage: T+ as a
+// hi + lo pair of floats (lo in the r+ variant's slot, unused by Europeans,
+// whose rate-bumped paths are derived from the base path), T- relative to
+// it as e^(x_T- - x_T+) - 1.
+template <typename R, int NV, int NFX, bool PD>
+constexpr bool t_slots_fp64() { return sizeof(R) == 4 && !PD && NV == 3 + 2 * NFX; }
+template <int NFX>
+constexpr int t_slot0() { return 1 + 2 * NFX; }
+
 template <typename R>
 struct Trk {
   R sumlog, sumS, mn, mx, t;
@@ -238,9 +261,10 @@ __device__ __forceinline__ R log_jump(const Job<R> *__restrict__ J, const VarPar
 // European end-of-path jump for NV lockstep slots. All slots share the
 // Poisson uniform; slots needing more draws read them from a copy of the
 // stream, and the stream advances by what the base path (slot 0) consumed.
-template <typename R, int NV>
+template <typename R, int NV, bool TD = false, int KT = 0>
 __device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
-                                          const VarParams<R> *__restrict__ const *P, R *x, Rng &g) {
+                                          const VarParams<R> *__restrict__ const *P, R *x, Rng &g,
+                                          double *xt = nullptr) {
   if (J->jump_kind == 0) return;
   const double ua = uniform(g);
   int n[NV];
@@ -275,7 +299,15 @@ __device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
     }
   }
 #pragma unroll
-  for (int k = 0; k < NV; ++k) x[k] += P[k]->jdrift + tot[k];
+  for (int k = 0; k < NV; ++k) {
+    if constexpr (TD) {
+      if (k == KT || k == KT + 1) {
+        xt[k - KT] += P[k]->jdrift_d + static_cast<double>(tot[k]);
+        continue;
+      }
+    }
+    x[k] += P[k]->jdrift + tot[k];
+  }
 }
 
 // ---- rare Giles branches, packed across the wavefront ----------------------
@@ -431,15 +463,34 @@ __device__ __forceinline__ R nci_tail_value(const Job<R> *__restrict__ J, int f,
 }
 
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
-template <typename R, int NV, int NFX>
+// TD: slots KT, KT + 1 are stepped in FP64 from vt into vnt (their entries
+// of v / vn are not used).
+template <typename R, int NV, int NFX, bool TD = false, int KT = 0>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
                                               const VarParams<R> *__restrict__ const *P, int f,
-                                              R (&v)[NV][NFX], R (&vn)[NV][NFX], Rng &g) {
+                                              R (&v)[NV][NFX], R (&vn)[NV][NFX], Rng &g,
+                                              double (*vt)[NFX] = nullptr, double (*vnt)[NFX] = nullptr) {
   const int sch = J->scheme[f];
+  auto is_t = [](int k) { return TD && (k == KT || k == KT + 1); };
+  // a slot's new variance from its (unscaled) table value or exact quantile
+  auto put = [&](int k, R y) {
+    if (is_t(k)) vnt[k - KT][f] = static_cast<double>(y) * P[k]->Kd[f][2];
+    else vn[k][f] = y * P[k]->Kr[f][2];
+  };
+  auto putd = [&](int k, double q) {
+    if (is_t(k)) vnt[k - KT][f] = q * P[k]->Kd[f][2];
+    else vn[k][f] = static_cast<R>(q * P[k]->Kd[f][2]);
+  };
   if (sch == 1) {  // Ninomiya-Victoir
     const R Z = normcdfinv<R>(uniform(g));
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
+      if (is_t(k)) {
+        const double *K = P[k]->Kd[f];
+        const double root = sqrt_nr(K[0] * vt[k - KT][f] + K[1]) + K[2] * static_cast<double>(Z);
+        vnt[k - KT][f] = K[0] * (root * root) + K[1];
+        continue;
+      }
       const R root = dsqrt<R>(P[k]->Kr[f][0] * v[k][f] + P[k]->Kr[f][1]) + P[k]->Kr[f][2] * Z;
       vn[k][f] = P[k]->Kr[f][0] * (root * root) + P[k]->Kr[f][1];
     }
@@ -473,7 +524,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       bool big = false;
 #pragma unroll
       for (int k = 0; k < NV; ++k) {
-        lams[k] = P[k]->Kd[f][0] * static_cast<double>(v[k][f]);
+        lams[k] = P[k]->Kd[f][0] * (is_t(k) ? vt[k - KT][f] : static_cast<double>(v[k][f]));
         big = big || lams[k] > 4.0;
         Ns[k] = 0;
       }
@@ -502,7 +553,12 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     // grid interval biases the draw low (by up to ~3e-5 of the row mean per
     // step for large N); below nci_u_lo those rows are inverted exactly
     const bool lo = u2 < J->nci_u_lo;
-    const bool main0 = Ns[0] <= P[0]->nmax[f] && u2 < 0.999 && !(lo && Ns[0] >= 2);
+    const bool acc = u2 >= J->nci_u_acc && (u2 < J->nci_u_acc_hi || u2 >= 0.999);
+    // row N is read from the table for slot k (else inverted exactly)
+    auto in_tab = [&](int N, int k) {
+      return (N <= P[k]->nmax[f] && !(lo && N >= 2)) || (N <= J->nci_rows_acc[f] && acc);
+    };
+    const bool main0 = in_tab(Ns[0], 0) && u2 < 0.999;
     R y0 = R(0);
     if (main0) {
       const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
@@ -516,7 +572,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     double q0 = 0.0;
     R yt0 = R(0);
     if constexpr (NV > 1 && sizeof(R) == 4) {
-      const bool in0 = Ns[0] <= P[0]->nmax[f] && !(lo && Ns[0] >= 2);
+      const bool in0 = in_tab(Ns[0], 0);
       tl0 = in0 && u2 >= 0.999 && J->nci_tail[f] != nullptr;
       ex0 = !(in0 && (u2 < 0.999 || J->nci_tail[f] != nullptr));
       if (tl0) yt0 = nci_tail_value<R>(J, f, Ns[0], u2);
@@ -529,14 +585,14 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const int N = Ns[k];
-      const bool in_tab = N <= P[k]->nmax[f] && !(lo && N >= 2);
-      if (main0 && N == Ns[0] && N <= P[k]->nmax[f]) {
-        vn[k][f] = y0 * P[k]->Kr[f][2];
-      } else if (in_tab && u2 < 0.999) {
+      const bool tab_k = in_tab(N, k);
+      if (main0 && N == Ns[0] && tab_k) {
+        put(k, y0);
+      } else if (tab_k && u2 < 0.999) {
         const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
-        vn[k][f] = (h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]) * P[k]->Kr[f][2];
-      } else if (in_tab && u2 >= 0.999 && J->nci_tail[f] != nullptr) {
-        vn[k][f] = ((tl0 && N == Ns[0]) ? yt0 : nci_tail_value<R>(J, f, N, u2)) * P[k]->Kr[f][2];
+        put(k, h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]);
+      } else if (tab_k && u2 >= 0.999 && J->nci_tail[f] != nullptr) {
+        put(k, (tl0 && N == Ns[0]) ? yt0 : nci_tail_value<R>(J, f, N, u2));
       } else {
         double q;
         if (ex0 && N == Ns[0] && P[k]->Kd[f][1] == P[0]->Kd[f][1]) {
@@ -546,7 +602,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
           const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
           q = chi2_quantile_lg(dof, u2, lg);
         }
-        vn[k][f] = static_cast<R>(q * P[k]->Kd[f][2]);
+        putd(k, q);
       }
     }
   } else {  // Exact (Poisson-gamma mixture); variable draws, so never lockstep
@@ -575,6 +631,10 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
   R x[NV], v[NV][NFX];
   Trk<R> tr[PD ? NV : 1];
   const R x0 = static_cast<R>(J->logS0);
+  // FP64 state of the maturity-bumped slots (t_slots_fp64)
+  constexpr bool TD = t_slots_fp64<R, NV, NFX, PD>();
+  constexpr int KT = t_slot0<NFX>();
+  double xt[2], vt[2][NFX], vnt[2][NFX];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     x[k] = x0;
@@ -582,13 +642,27 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
     for (int f = 0; f < NFX; ++f) v[k][f] = P[k]->v0[f];
     if constexpr (PD) trk_init(J, tr[k], x0, P[k]->dt);
   }
+  if constexpr (TD) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      xt[i] = static_cast<double>(x0);
+#pragma unroll
+      for (int f = 0; f < NFX; ++f) vt[i][f] = P[KT + i]->v0d[f];
+    }
+  }
   const unsigned steps = static_cast<unsigned>(J->steps);
   if (NF == 0 && !PD) {
     // zero-factor European: one step to maturity (ASVJstepper.hpp:145-161)
     const R G = normcdfinv<R>(uniform(g));
 #pragma unroll
-    for (int k = 0; k < NV; ++k) x[k] += P[k]->gdrift_n + P[k]->gvol_n * G;
-    euro_jump<R, NV>(J, P, x, g);
+    for (int k = 0; k < NV; ++k) {
+      if (TD && (k == KT || k == KT + 1)) {
+        xt[k - KT] += P[k]->gdrift_nd + P[k]->gvol_nd * static_cast<double>(G);
+        continue;
+      }
+      x[k] += P[k]->gdrift_n + P[k]->gvol_n * G;
+    }
+    euro_jump<R, NV, TD, KT>(J, P, x, g, xt);
   } else {
     for (unsigned step = 0; step < steps; ++step) {
       const R G = normcdfinv<R>(uniform(g));
@@ -599,10 +673,24 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
         R vn[NV][NFX];
         // factors in draw order; written out so the factor index is a
         // compile-time constant and the slot arrays stay in registers
-        evolve_factor<R, NV, NFX>(J, P, 0, v, vn, g);
-        if constexpr (NF > 1) evolve_factor<R, NV, NFX>(J, P, 1, v, vn, g);
+        evolve_factor<R, NV, NFX, TD, KT>(J, P, 0, v, vn, g, vt, vnt);
+        if constexpr (NF > 1) evolve_factor<R, NV, NFX, TD, KT>(J, P, 1, v, vn, g, vt, vnt);
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
+          if (TD && (k == KT || k == KT + 1)) {
+            const int i = k - KT;
+            double a = 0, d = 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+              const double *C = P[k]->Cd[f];
+              a += C[3] * (vt[i][f] + vnt[i][f]);
+              d += C[0] + C[1] * vt[i][f] + C[2] * vnt[i][f];
+            }
+            xt[i] += d + sqrt_nr(a) * static_cast<double>(G);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) vt[i][f] = vnt[i][f];
+            continue;
+          }
           R a = 0, d = 0;
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
@@ -622,10 +710,21 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
         for (int k = 0; k < NV; ++k) trk_update(J, tr[k], x[k], P[k]->dt);
       }
     }
-    if constexpr (!PD && NF > 0) euro_jump<R, NV>(J, P, x, g);
+    if constexpr (!PD && NF > 0) euro_jump<R, NV, TD, KT>(J, P, x, g, xt);
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
+    if (TD && k == KT) {
+      const double S = exp(xt[0]);
+      const R hi = static_cast<R>(S);
+      stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = hi;
+      stage[(F_SPOT * MAXV + V_RUP) * BLK + tid] = static_cast<R>(S - static_cast<double>(hi));
+      continue;
+    }
+    if (TD && k == KT + 1) {
+      stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = static_cast<R>(expm1f(static_cast<float>(xt[1] - xt[0])));
+      continue;
+    }
     if constexpr (PD) {
       finalize_stage<R, BLK>(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
     } else {
@@ -729,7 +828,18 @@ __device__ __forceinline__ void flush(double *__restrict__ accg, int a, int tid,
   *slot = first ? x : *slot + x;
 }
 
-template <typename R, int BLK, int UNR>
+// base / up / dn payoffs of a European at terminal spot S (FP64)
+template <typename R>
+__device__ __forceinline__ void eval3_spot(const Job<R> *__restrict__ J, double S, double K, double &p, double &pu,
+                                           double &pd) {
+  const bool smooth = J->mode != 0;
+  p = payoff(J, smooth, S, K);
+  pu = payoff(J, smooth, S * J->mult_up, K);
+  pd = payoff(J, smooth, S * J->mult_dn, K);
+}
+
+// REL: the maturity-bumped variants are staged in FP64 form (t_slots_fp64).
+template <typename R, int BLK, int UNR, bool REL = false>
 __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
                                            const int *__restrict__ valid, int tid, double *__restrict__ accg,
                                            bool first) {
@@ -815,8 +925,16 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
       for (int s = s0; s < s1; ++s) {
         if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
-        eval3<R, BLK>(J, stage, V_TUP, s, K, true, ap, au, ad);
-        eval3<R, BLK>(J, stage, V_TDN, s, K, true, bp, bu, bd);
+        if constexpr (REL) {
+          const double Su = static_cast<double>(stage[(F_SPOT * MAXV + V_TUP) * BLK + s]) +
+                            static_cast<double>(stage[(F_SPOT * MAXV + V_RUP) * BLK + s]);
+          const double Sd = Su + Su * static_cast<double>(stage[(F_SPOT * MAXV + V_TDN) * BLK + s]);
+          eval3_spot(J, Su, K, ap, au, ad);
+          eval3_spot(J, Sd, K, bp, bu, bd);
+        } else {
+          eval3<R, BLK>(J, stage, V_TUP, s, K, true, ap, au, ad);
+          eval3<R, BLK>(J, stage, V_TDN, s, K, true, bp, bu, bd);
+        }
         const double th = (ap * J->dfTu - bp * J->dfTd) * J->inv2dT;
         const double ch = ((au - ad) * J->dfTu - (bu - bd) * J->dfTd) * (J->inv2dT * J->inv2h);
         st += th;
@@ -941,7 +1059,8 @@ __device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, c
       }
       s_valid[tid] = valid ? 1 : 0;
       __syncthreads();
-      accumulate<R, BLK, UNR>(J, stage, s_valid, tid, accg, it == 0);
+      accumulate<R, BLK, UNR, t_slots_fp64<R, NV, (NF > 0 ? NF : 1), PD>()>(J, stage, s_valid, tid, accg,
+                                                                          it == 0);
       if (J->view)
         accumulate<R, BLK, UNR>(jobs + (J->view - 1), stage, s_valid, tid,
                                 J->view_accg + static_cast<size_t>(u) * (2 * NSTAT * BLK), it == 0);

@@ -75,6 +75,16 @@ class Oracle:
                                             C.POINTER(capi.Instrument), C.c_int32,
                                             C.POINTER(capi.MCConfig), C.c_double, C.c_double,
                                             C.c_double, C.POINTER(capi.MCResult), C.c_int]
+        for fn in (lib.oracle_debug_plain_path, lib.oracle_debug_coupled_path):
+            fn.restype = C.c_int
+        lib.oracle_debug_plain_path.argtypes = [C.POINTER(capi.Model), C.POINTER(C.c_int32),
+                                                C.POINTER(capi.Instrument), C.c_uint64, C.c_uint64, C.c_double,
+                                                C.c_double, C.c_double, dp, dp, dp, C.c_int64,
+                                                C.POINTER(C.c_int64)]
+        lib.oracle_debug_coupled_path.argtypes = [C.POINTER(capi.Model), C.POINTER(C.c_int32),
+                                                  C.POINTER(capi.Instrument), C.c_uint64, C.c_uint64, C.c_double,
+                                                  C.c_double, C.c_double, dp, dp, dp, dp, C.c_int64,
+                                                  C.POINTER(C.c_int64)]
         lib.oracle_mlmc_calculate.restype = C.c_int
         lib.oracle_mlmc_calculate.argtypes = [C.POINTER(capi.Model), C.POINTER(C.c_int32),
                                               C.POINTER(capi.Instrument), C.c_int32,
@@ -223,3 +233,47 @@ class Oracle:
         if rc != 0:
             raise EngineError(rc, self.lib.oracle_last_error().decode())
         return buf.as_dict()
+
+    PLAIN_REC, COUPLED_REC, FIN = 5, 16, 10
+
+    def debug_plain_path(self, model, schemes, instr: InstrumentSpec, steps: int, seed: int, S0: float, r: float,
+                         q: float):
+        """One plain-stepper path from seed_state(seed): per-step record
+        (first-draw index, log S, V0, V1, step jump; plus the European's
+        terminal jump row), the final tracker state and every uniform drawn."""
+        rec = np.zeros((steps + 1) * self.PLAIN_REC)
+        fin = np.zeros(self.FIN)
+        draws = np.zeros(64 * steps + 1024)
+        n = C.c_int64(0)
+        s = instr.struct()
+        rc = self.lib.oracle_debug_plain_path(C.byref(model), schemes_array(schemes), C.byref(s), steps, seed, S0,
+                                              r, q, _dp(rec), _dp(fin), _dp(draws), draws.size, C.byref(n))
+        if rc != 0:
+            raise EngineError(rc, self.lib.oracle_last_error().decode())
+        assert n.value <= draws.size
+        return rec.reshape(steps + 1, self.PLAIN_REC), fin, draws[:n.value]
+
+    def debug_coupled_path(self, model, schemes, instr: InstrumentSpec, coarse_steps: int, seed: int, S0: float,
+                           r: float, q: float):
+        """One MLMC coupled path (fine step T / (2 coarse_steps)) from
+        seed_state(seed): per-coarse-step record (first-draw index, G1, G2,
+        Gc, a1, a2, j1, j2, xf, vf0, vf1, xc, vc0, vc1, mid0, mid1; plus the
+        European's shared jump row), the final fine and coarse tracker
+        states and every uniform drawn."""
+        nc = coarse_steps
+        rec = np.zeros((nc + 1) * self.COUPLED_REC)
+        ff, fc = np.zeros(self.FIN), np.zeros(self.FIN)
+        draws = np.zeros(128 * nc + 1024)
+        n = C.c_int64(0)
+        s = instr.struct()
+        rc = self.lib.oracle_debug_coupled_path(C.byref(model), schemes_array(schemes), C.byref(s), nc, seed, S0,
+                                                r, q, _dp(rec), _dp(ff), _dp(fc), _dp(draws), draws.size,
+                                                C.byref(n))
+        if rc != 0:
+            raise EngineError(rc, self.lib.oracle_last_error().decode())
+        assert n.value <= draws.size
+        return rec.reshape(nc + 1, self.COUPLED_REC), ff, fc, draws[:n.value]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))

@@ -465,6 +465,100 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
   }
 }
 
+// ---------------------------------------------------------------------------
+// Debug exports (tests only): one path of the plain stepper or of the MLMC
+// coupled stepper from the state seed_state(seed), with its per-step record
+// (multi_step / multi_step_coupled `rec`), the final tracker states and the
+// log of every uniform drawn, so that a test can replay the path's draws
+// through the step equations.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int FIN = 10;  // S (finalized), V0, V1, sumLogS, sumS, min, max, hit0..2
+
+void put_final(const PathState &s, double *o) {
+  const double v[FIN] = {s.logS, s.V[0], s.V[1], s.sumLogS, s.sumS, s.minLogS, s.maxLogS,
+                         s.hit[0] ? 1.0 : 0.0, s.hit[1] ? 1.0 : 0.0, s.hit[2] ? 1.0 : 0.0};
+  for (int i = 0; i < FIN; ++i) o[i] = v[i];
+}
+
+struct DrawLog {
+  std::vector<double> log;
+#ifndef ORC_NO_DRAW_LOG
+  DrawLog() { g_draw_log = &log; }
+  ~DrawLog() { g_draw_log = nullptr; }
+#else
+  DrawLog() { throw Error(MYQL_E_UNSUPPORTED, "this oracle build has no draw log"); }
+#endif
+  void copy(double *draws, int64_t max_draws, int64_t *n_draws) const {
+    *n_draws = static_cast<int64_t>(log.size());
+    const size_t n = std::min<size_t>(log.size(), static_cast<size_t>(std::max<int64_t>(max_draws, 0)));
+    std::copy(log.begin(), log.begin() + n, draws);
+  }
+};
+}  // namespace
+
+int oracle_debug_plain_path(const myql_model *model, const int32_t *schemes, const myql_instrument *instr,
+                            uint64_t steps, uint64_t seed, double S0, double r, double q, double *rec,
+                            double *fin, double *draws, int64_t max_draws, int64_t *n_draws) {
+  try {
+    validate(model, schemes, instr);
+    if (!rec || !fin || !draws || !n_draws || steps < 1) throw Error(MYQL_E_INVALID_ARG, "bad arguments");
+    Instr I;
+    I.in = *instr;
+    I.K.assign(instr->strikes, instr->strikes + 1);
+    const double T = I.in.maturity, dt = T / static_cast<double>(steps);
+    const bool pd = path_dependent(I.in.kind);
+    const TrackerCfg tc = tracker_config(I, MYQL_GREEK_NONE, S0, S0 * 1e-4);
+    TableCache cache;
+    Tables tb = tables_for(cache, *model, schemes, dt);
+    const Stepper st = make_stepper(*model, schemes, dt, r, q, T, pd, tb.t[0], tb.t[1]);
+    Xoshiro g = seed_state(seed);
+    DrawLog dl;
+    PathState s;
+    start_path(st, s, tc, S0);
+    multi_step(st, s, tc, g, static_cast<unsigned>(steps), rec);
+    put_final(s, fin);
+    dl.copy(draws, max_draws, n_draws);
+    return MYQL_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  }
+}
+
+int oracle_debug_coupled_path(const myql_model *model, const int32_t *schemes, const myql_instrument *instr,
+                              uint64_t coarse_steps, uint64_t seed, double S0, double r, double q, double *rec,
+                              double *fin_f, double *fin_c, double *draws, int64_t max_draws, int64_t *n_draws) {
+  try {
+    validate(model, schemes, instr);
+    if (!rec || !fin_f || !fin_c || !draws || !n_draws || coarse_steps < 1)
+      throw Error(MYQL_E_INVALID_ARG, "bad arguments");
+    Instr I;
+    I.in = *instr;
+    I.K.assign(instr->strikes, instr->strikes + 1);
+    const double T = I.in.maturity, dtf = T / static_cast<double>(2 * coarse_steps);
+    const bool pd = path_dependent(I.in.kind);
+    const TrackerCfg tc = tracker_config(I, MYQL_GREEK_NONE, S0, S0 * 1e-4);
+    TableCache cache;
+    Tables tb = tables_for(cache, *model, schemes, dtf);
+    const Stepper F = make_stepper(*model, schemes, dtf, r, q, T, pd, tb.t[0], tb.t[1]);
+    const Stepper C = make_stepper(*model, schemes, 2.0 * dtf, r, q, T, pd, tb.t[0], tb.t[1]);
+    Xoshiro g = seed_state(seed);
+    DrawLog dl;
+    PathState sf, sc;
+    start_path(F, sf, tc, S0);
+    start_path(C, sc, tc, S0);
+    multi_step_coupled(F, C, sf, sc, tc, g, static_cast<unsigned>(coarse_steps), rec);
+    put_final(sf, fin_f);
+    put_final(sc, fin_c);
+    dl.copy(draws, max_draws, n_draws);
+    return MYQL_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  }
+}
+
 int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const myql_instrument *instr,
                           int32_t mode, const myql_mlmc_config *cfg, double S0, double r, double q,
                           myql_mlmc_result *res, int nthreads) {

@@ -520,9 +520,13 @@ static inline void start_path(const Stepper &st, PathState &s, const TrackerCfg 
   trk_init(s, c, s.logS, st.dt);
 }
 
-// ASVJStepper::multi_step
+// ASVJStepper::multi_step. rec (debug exports only): per step, the index of
+// its first uniform in the draw log, then log S, V0, V1 and the step's log
+// jump after the step (PLAIN_REC doubles); the European's terminal jump in
+// one more row.
+constexpr int PLAIN_REC = 5;
 static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg &c, Xoshiro &g,
-                              unsigned n) {
+                              unsigned n, double *rec = nullptr) {
   double v0 = s.V[0], v1 = s.V[1];
   double x = s.logS;
   const double total_time = n * st.dt;
@@ -536,6 +540,9 @@ static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg 
     return;
   }
   for (unsigned k = 0; k < n; ++k) {
+    double *rk = rec ? rec + static_cast<size_t>(k) * PLAIN_REC : nullptr;
+    if (rk) rk[0] = draw_index();
+    double jk = 0.0;
     if (st.nf == 0) {
       const double G = normcdfinv(uniform(g));
       x += st.gbm_drift + st.gbm_vol * G;
@@ -560,12 +567,23 @@ static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg 
       v1 = vn1;
     }
     if (st.pathdep) {
-      x += log_jump(st.jump, g);
+      jk = log_jump(st.jump, g);
+      x += jk;
       s.logS = x;
       trk_update(s, c, st.dt);
     }
+    if (rk) {
+      rk[1] = x;
+      rk[2] = v0;
+      rk[3] = v1;
+      rk[4] = jk;
+    }
   }
-  if (!st.pathdep && st.nf > 0) x += log_jump(st.jump, g);
+  if (!st.pathdep && st.nf > 0) {
+    const double J = log_jump(st.jump, g);
+    x += J;
+    if (rec) rec[static_cast<size_t>(n) * PLAIN_REC] = J;
+  }
   s.logS = x;
   s.V[0] = v0;
   s.V[1] = v1;
@@ -573,9 +591,14 @@ static inline void multi_step(const Stepper &st, PathState &s, const TrackerCfg 
 }
 
 // CoupledASVJStepper::multi_step_coupled. `fine` is prepared with dt_f,
-// `coarse` with 2 dt_f; their jump workspaces are the fine one's.
+// `coarse` with 2 dt_f; their jump workspaces are the fine one's. rec
+// (debug exports only): per coarse step, after it, COUPLED_REC doubles:
+// draw-log index of its first uniform, G1, G2, Gc, a1, a2, j1, j2, xf, vf0,
+// vf1, xc, vc0, vc1, mid0, mid1 (mid = fine variance after the first fine
+// step); the European's shared jump in one more row.
+constexpr int COUPLED_REC = 16;
 static inline void multi_step_coupled(const Stepper &F, const Stepper &Cs, PathState &sf, PathState &sc,
-                                      const TrackerCfg &c, Xoshiro &g, unsigned nc) {
+                                      const TrackerCfg &c, Xoshiro &g, unsigned nc, double *rec = nullptr) {
   double vf[2] = {sf.V[0], sf.V[1]}, vc[2] = {sc.V[0], sc.V[1]};
   double xf = sf.logS, xc = sc.logS;
   const double total_time = nc * Cs.dt;
@@ -598,12 +621,16 @@ static inline void multi_step_coupled(const Stepper &F, const Stepper &Cs, PathS
     const double J = log_jump(F.jump, g);
     xf += J;
     xc += J;
+    if (rec) rec[static_cast<size_t>(nc) * COUPLED_REC] = J;
   }
   for (unsigned k = 0; k < nc; ++k) {
+    double *rk = rec ? rec + static_cast<size_t>(k) * COUPLED_REC : nullptr;
+    if (rk) rk[0] = draw_index();
     const double G1 = normcdfinv(uniform(g));
     const double G2 = normcdfinv(uniform(g));
     double Gc = (G1 + G2) / SQ2;
     double j1 = 0.0, j2 = 0.0;
+    double ra1 = 0.0, ra2 = 0.0, rmid[2] = {0.0, 0.0};
     if (F.nf == 0) {
       xc += Cs.gbm_drift + Cs.gbm_vol * Gc;
       xf += F.gbm_drift + F.gbm_vol * G1;
@@ -697,6 +724,15 @@ static inline void multi_step_coupled(const Stepper &F, const Stepper &Cs, PathS
         sc.logS = xc;
         trk_update(sc, c, Cs.dt);
       }
+      ra1 = a1;
+      ra2 = a2;
+      rmid[0] = mid[0];
+      rmid[1] = nf > 1 ? mid[1] : 0.0;
+    }
+    if (rk) {
+      const double r[COUPLED_REC - 1] = {G1, G2, Gc, ra1, ra2, j1, j2, xf, vf[0], vf[1], xc, vc[0], vc[1],
+                                          rmid[0], rmid[1]};
+      for (int i = 0; i < COUPLED_REC - 1; ++i) rk[1 + i] = r[i];
     }
   }
   sf.logS = xf;

@@ -82,12 +82,32 @@ static inline Xoshiro seed_state(uint64_t seed) {
 // uniform_real_distribution<double>(0,1) / uniform_01<double> over a 64-bit
 // integer engine: u = double(x) / 2^64 (round to nearest), redrawn while
 // u == 1.0 (x >= 2^64 - 2^10).
+#ifndef ORC_NO_DRAW_LOG
+// Debug exports only (oracle_debug_*_path): when set, every uniform this
+// thread draws is appended, so a test can replay the draws of a path. The
+// CPU-baseline and op-counting builds compile it out (-DORC_NO_DRAW_LOG).
+inline thread_local std::vector<double> *g_draw_log = nullptr;
+#endif
+
 static inline double uniform(Xoshiro &g) {
   for (;;) {
     const double u = static_cast<double>(xnext(g)) * 0x1p-64;
     ORC_OPS(1);
-    if (u < 1.0) return u;
+    if (u < 1.0) {
+#ifndef ORC_NO_DRAW_LOG
+      if (g_draw_log) g_draw_log->push_back(u);
+#endif
+      return u;
+    }
   }
+}
+
+static inline double draw_index() {
+#ifndef ORC_NO_DRAW_LOG
+  return g_draw_log ? static_cast<double>(g_draw_log->size()) : -1.0;
+#else
+  return -1.0;
+#endif
 }
 
 // ---------------------------------------------------------------------------

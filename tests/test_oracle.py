@@ -188,6 +188,44 @@ def test_nci_table_row_means(oracle):
         assert lo < rel < hi, (N, rel)
 
 
+def _nci_rows(h, dt, reference):
+    v_max = max(h.v0, (3.0 if reference else 25.0) * h.theta)
+    ex = math.exp(-h.kappa * dt)
+    lam = 2.0 * v_max * ex / (h.sigma ** 2 * (1.0 - ex) / h.kappa)
+    return int(min(1023, max(23, math.ceil(lam + (4.0 if reference else 6.0) * math.sqrt(lam)))))
+
+
+@pytest.mark.parametrize("h,dt", [(Heston(2.0, 0.04, 1.0, -0.7, 0.04), 0.01),
+                                  (Heston(5.0, 0.02, 0.8, -0.3, 0.01), 1.0 / 252)])
+def test_extended_rows_serve_the_exact_fallback(oracle, h, dt):
+    """Under the reference's table semantics the chi2 quantile is inverted
+    exactly above its row count and for u >= 0.999 (VolSchemes.hpp:362-369).
+    The engine serves those cases from the extended rows (exact slopes) at
+    u in [100 grid steps, 0.99) and from the upper-tail rows at u >= 0.999:
+    there the table agrees with the exact inversion to 5e-11 relative, so the
+    reference semantics lose nothing measurable by it (the device tables are
+    built by the same rule). Between 0.99 and 0.999 the uniform grid errs by
+    up to ~5e-8, so those stay exact."""
+    d = 4 * h.kappa * h.theta / h.sigma ** 2
+    n_ref, n_ext = _nci_rows(h, dt, True), _nci_rows(h, dt, False)
+    assert n_ext > n_ref
+    dx = 0.999 / 9999
+    rng = np.random.default_rng(5)
+    rows = np.unique(np.concatenate([[n_ref + 1, n_ext], rng.integers(n_ref + 1, n_ext + 1, 4)]))
+    worst_main = worst_tail = 0.0
+    for N in rows:
+        u = rng.uniform(100 * dx, 0.99, 30)
+        t = oracle.nci_lookup(h, dt, int(N), u, False)
+        ex = np.array([oracle.chi2_quantile(d + 2 * N, x) for x in u])
+        worst_main = max(worst_main, float(np.max(np.abs(t - ex) / ex)))
+    for N in np.unique(np.concatenate([[0, 1, 2, n_ref, n_ext], rows])):
+        u = 1.0 - 10.0 ** rng.uniform(-12, -3, 20)
+        t = oracle.nci_lookup(h, dt, int(N), u, False)
+        ex = np.array([oracle.chi2_quantile(d + 2 * N, x) for x in u])
+        worst_tail = max(worst_tail, float(np.max(np.abs(t - ex) / ex)))
+    assert worst_main < 5e-11 and worst_tail < 5e-11, (worst_main, worst_tail)
+
+
 def test_acklam_and_giles_bit_exact_with_reference(oracle):
     """The oracle's Acklam inverse normal and Giles inverse Poisson CDF equal
     the reference's own Numerics::normcdfinv_as241 / Numerics::poissinvcdf
