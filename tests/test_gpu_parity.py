@@ -139,6 +139,34 @@ def test_bates_nci_full_strip(engine, oracle, prec, tol, flags):
         compare(g, c, ["price", "delta"], tol)
 
 
+# FP32 paths against the FP64 oracle on identical draws, every Full field:
+# max over options (price >= 0.1) of |fp32 - fp64| / max(|fp64|, its SE).
+# The maturity-bumped variants behind theta and charm are stepped in FP64
+# inside the FP32 kernel; what remains is paths that part ways where FP32
+# rounding of a table value moves a Poisson count across a CDF boundary
+# (noise, largest in the most amplified estimators: gamma, vanna, charm).
+FP32_FULL_BOUNDS = {"price": 1e-4, "delta": 1e-4, "rho": 1e-4, "vega": 1e-3, "theta": 0.05, "gamma": 0.05,
+                    "vanna": 0.05, "charm": 0.2}
+
+
+def test_bates_nci_full_strip_fp32_every_field(engine, oracle):
+    jk, jp = merton(0.5, -0.1, 0.15)
+    m = make_model(1, [H_HI], jump_kind=jk, jump=jp)
+    inst = InstrumentSpec(strikes=np.linspace(70, 130, 16), maturity=1.0)
+    print("fp32 full strip vs fp64 oracle: 2^16 paths, 100 steps", flush=True)
+    g = engine.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL,
+                  mc_config(1 << 16, 100, seed=42, precision=capi.FP32, num_streams=1024), 100.0, 0.02, 0.0)
+    c = oracle.mc(m, [capi.SCHEME_NCI], inst, capi.GREEK_FULL,
+                  mc_config(1 << 16, 100, seed=42, precision=capi.FP64, num_streams=1024), 100.0, 0.02, 0.0)
+    ok = c["price"] >= 0.1
+    errs = {}
+    for f, bound in FP32_FULL_BOUNDS.items():
+        x, y, se = (np.asarray(a, float).reshape(-1)[:16] for a in (g[f], c[f], c[f + "_std_err"]))
+        errs[f] = float(np.max(np.abs(x - y)[ok] / np.maximum(np.abs(y), se)[ok]))
+    bad = {f: e for f, e in errs.items() if not e <= FP32_FULL_BOUNDS[f]}
+    assert not bad, (bad, errs)
+
+
 @pytest.mark.parametrize("bump", [1e-3, 3e-2])
 def test_bates_nci_bumped_slot_counts(engine, oracle, bump):
     """The lockstep kernel gives each bumped Greek slot (v0 +-, T +-) its
