@@ -241,6 +241,24 @@ int myql_mc_calculate(myql_ctx *ctx, const myql_model *model,
 int myql_mc_calculate_batch(myql_ctx *ctx, const myql_mc_job *jobs,
                             int64_t n_jobs);
 
+/* myql_mc_calculate without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results as myql_mc_calculate,
+ * but the call's device memory (stream states, job blocks, work units,
+ * queue counters, per-unit accumulators, partial sums) is carved from a
+ * file-scope device array of 64 MiB that each GPU holds until the process
+ * ends; it calls no hipMalloc or hipFree. MYQL_E_INVALID_ARG when the call
+ * needs more than that (about 65,536 streams x 64 strikes in Full mode;
+ * fewer streams or strikes fit), MYQL_E_UNSUPPORTED for SchemeNCI (its
+ * chi-squared table is allocated). MYQL_FLAG_AUTOTUNE is ignored (the
+ * context's tuned or default launch shape is used). All such calls on one
+ * GPU share the array: calls that overlap on one GPU race. */
+int myql_mc_calculate_noalloc(myql_ctx *ctx, const myql_model *model,
+                              const int32_t schemes[2], const myql_instrument *instr,
+                              int32_t greek_mode, const myql_mc_config *cfg,
+                              double S0, double r, double q, myql_mc_result *res);
+
 int myql_mlmc_calculate(myql_ctx *ctx, const myql_model *model,
                         const int32_t schemes[2], const myql_instrument *instr,
                         int32_t greek_mode, const myql_mlmc_config *cfg,

@@ -130,7 +130,7 @@ class CVResult(C.Structure):
 # Every extern "C" symbol declared in include/myql_capi.h.
 EXPORTED = [
     "myql_create", "myql_destroy", "myql_last_error", "myql_tuning_report", "myql_mc_calculate",
-    "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_fourier_calculate_batch", "myql_cv_train",
+    "myql_mc_calculate_noalloc", "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_fourier_calculate_batch", "myql_cv_train",
     "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
     "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
     "myql_debug_cv_input_grad",
@@ -162,6 +162,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_mc_calculate.argtypes = [vp, C.POINTER(Model), C.POINTER(C.c_int32),
                                       C.POINTER(Instrument), C.c_int32, C.POINTER(MCConfig),
                                       C.c_double, C.c_double, C.c_double, C.POINTER(MCResult)]
+    lib.myql_mc_calculate_noalloc.restype = C.c_int
+    lib.myql_mc_calculate_noalloc.argtypes = lib.myql_mc_calculate.argtypes
     lib.myql_mc_calculate_batch.restype = C.c_int
     lib.myql_mc_calculate_batch.argtypes = [vp, C.POINTER(MCJob), C.c_int64]
     lib.myql_mlmc_calculate.restype = C.c_int

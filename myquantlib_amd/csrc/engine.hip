@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -156,6 +157,11 @@ std::vector<uint64_t> host_jump_polys(int nk) {
 // small kernels
 // ---------------------------------------------------------------------------
 namespace myql {
+
+// n 32-bit words set to zero (queue counters)
+__global__ void zero_words_kernel(unsigned *__restrict__ p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
 
 __device__ void apply_jump(Rng &g, const uint64_t *__restrict__ J) {
   Rng a = {0, 0, 0, 0};
@@ -322,7 +328,21 @@ struct myql_ctx {
   DevBuf tjobs, tunits, tcounter;
   DevBuf vaccg;  // accumulators of fused (view) jobs
   DevBuf fopts, fout;  // Fourier pricer options and integrals
+  // myql_mc_calculate_noalloc: this device's copy of the file-scope scratch
+  // array (its device address, from hipGetSymbolAddress at myql_create), and
+  // while such a call runs, the bump offset into it
+  void *scratch = nullptr;
+  bool noalloc = false;
+  size_t scratch_off = 0;
 };
+
+// Device memory of myql_mc_calculate_noalloc: stream states, job blocks,
+// work units, queue counters, per-unit accumulators and partial sums of one
+// call are carved from this array instead of being allocated. Each GPU the
+// library runs on holds its own copy until the process ends; every such call
+// on one GPU shares it, so calls that overlap on one GPU race.
+constexpr size_t NOALLOC_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 namespace {
 
@@ -335,8 +355,36 @@ void set_device(myql_ctx *c) {
   if (c->nci.size() > 32) c->nci.clear();
 }
 
-// S stream states for (seed, layout), cached per context.
+// Device memory of `bytes` for buffer b: b grown as needed, or, inside
+// myql_mc_calculate_noalloc, the next 256-byte aligned piece of the scratch
+// array (never an allocation there; too large a call is refused).
+void *dev_mem(myql_ctx *c, DevBuf &b, size_t bytes) {
+  if (!c->noalloc) {
+    b.reserve(bytes);
+    return b.p;
+  }
+  const size_t off = (c->scratch_off + 255) / 256 * 256;
+  if (bytes > NOALLOC_BYTES || off > NOALLOC_BYTES - bytes)
+    throw Fail(MYQL_E_INVALID_ARG, "myql_mc_calculate_noalloc: the call needs more device memory than the " +
+                                       std::to_string(NOALLOC_BYTES >> 20) +
+                                       " MiB scratch (fewer streams or strikes, or use myql_mc_calculate)");
+  c->scratch_off = off + bytes;
+  return static_cast<char *>(c->scratch) + off;
+}
+
+// S stream states for (seed, layout), cached per context (computed into the
+// scratch, uncached, inside myql_mc_calculate_noalloc).
 const uint64_t *stream_states(myql_ctx *c, uint64_t seed, int layout, uint64_t S) {
+  if (c->noalloc) {
+    DevBuf none;
+    uint64_t *p = static_cast<uint64_t *>(dev_mem(c, none, S * 32));
+    const int tb = 256;
+    const unsigned nb = static_cast<unsigned>((S + tb - 1) / tb);
+    hipLaunchKernelGGL(stream_states_kernel, dim3(nb), dim3(tb), 0, c->stream, seed, layout, S,
+                       c->d_polys.as<uint64_t>(), p);
+    HIPCHECK(hipGetLastError());
+    return p;
+  }
   auto key = std::make_tuple(seed, layout, S);
   auto it = c->streams.find(key);
   if (it != c->streams.end()) return it->second->as<uint64_t>();
@@ -779,6 +827,9 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
       }
   for (int f = 0; f < nf; ++f)
     if (js.schemes[f] == MYQL_SCHEME_NCI) {
+      if (c->noalloc)
+        throw Fail(MYQL_E_UNSUPPORTED, "myql_mc_calculate_noalloc: SchemeNCI needs its chi-squared table "
+                                       "allocated (use myql_mc_calculate)");
       const myql_heston_params &p = M0.heston[f];
       const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
       NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision, nci_nmax(p, dt, true));
@@ -879,6 +930,7 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   const int prec = sizeof(R) == 4 ? MYQL_FP32 : MYQL_FP64;
   bool autotune = false;
   for (const BuiltJob &b : built) autotune = autotune || (b.spec.cfg->flags & MYQL_FLAG_AUTOTUNE);
+  if (c->noalloc) autotune = false;  // trial launches would need buffers of their own
   auto shape_of = [&](const GKey &k) -> std::pair<McShape, int> {
     auto it = c->tuned.find(std::make_tuple(prec, std::get<0>(k), std::get<1>(k), std::get<2>(k)));
     if (it != c->tuned.end()) return {it->second.shape, it->second.gridf};
@@ -907,31 +959,42 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     partial_elems += static_cast<size_t>(slots_for(i, min_blk)) * 2 * NSTAT * nks[i];
     stream_total += jobs[i].S + BLOCK;
   }
-  c->partial.reserve(std::max<size_t>(partial_elems, 1) * sizeof(double));
-  c->sums.reserve(std::max(soff, 1) * sizeof(double));
+  double *d_partial = static_cast<double *>(dev_mem(c, c->partial, std::max<size_t>(partial_elems, 1) * sizeof(double)));
+  double *d_sums = static_cast<double *>(dev_mem(c, c->sums, std::max(soff, 1) * sizeof(double)));
   const size_t strike_bytes = strikes.size() * sizeof(double);
   const size_t jobs_bytes = jobs.size() * sizeof(Job<R>);
-  c->jobs.reserve(jobs_bytes + strike_bytes + 256);
-  double *d_strikes = reinterpret_cast<double *>(static_cast<char *>(c->jobs.p) + ((jobs_bytes + 255) / 256) * 256);
+  char *d_jobs_raw = static_cast<char *>(dev_mem(c, c->jobs, jobs_bytes + strike_bytes + 256));
+  double *d_strikes = reinterpret_cast<double *>(d_jobs_raw + ((jobs_bytes + 255) / 256) * 256);
+  double *d_vaccg = nullptr;
   if (std::find(is_view.begin(), is_view.end(), 1) != is_view.end())
-    c->vaccg.reserve(static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double));
+    d_vaccg = static_cast<double *>(dev_mem(c, c->vaccg, static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double)));
   for (size_t i = 0; i < jobs.size(); ++i) {
-    if (jobs[i].view) jobs[i].view_accg = c->vaccg.as<double>();
+    if (jobs[i].view) jobs[i].view_accg = d_vaccg;
     jobs[i].strikes = d_strikes + reinterpret_cast<size_t>(jobs[i].strikes);
     // each job indexes its partials from its own base with first_slot = 0
-    jobs[i].partial = c->partial.as<double>() + pbase[i];
+    jobs[i].partial = d_partial + pbase[i];
     jobs[i].first_slot = 0;
   }
-  HIPCHECK(hipMemcpyAsync(c->jobs.p, jobs.data(), jobs_bytes, hipMemcpyHostToDevice, c->stream));
-  if (strike_bytes)
-    HIPCHECK(hipMemcpyAsync(d_strikes, strikes.data(), strike_bytes, hipMemcpyHostToDevice, c->stream));
-  const Job<R> *d_jobs = c->jobs.as<Job<R>>();
   // per-unit accumulators: 2*NSTAT*blk doubles per unit of blk streams
-  c->accg.reserve(static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double));
+  double *d_accg = static_cast<double *>(dev_mem(c, c->accg, static_cast<size_t>(stream_total) * 2 * NSTAT * sizeof(double)));
 
   std::map<GKey, std::vector<size_t>> members;  // jobs with work units (views have none)
   for (size_t i = 0; i < jobs.size(); ++i)
     if (!is_view[i]) members[gkey(i)].push_back(i);
+  // work units, queue counters and reduction meta; sized for the smallest
+  // block, so any launch shape fits (every buffer of the call is in place
+  // before the first copy)
+  size_t max_units = 0;
+  for (size_t i = 0; i < jobs.size(); ++i)
+    if (!is_view[i]) max_units += static_cast<size_t>(slots_for(i, min_blk));
+  const size_t nj = jobs.size();
+  Unit *d_units = static_cast<Unit *>(dev_mem(c, c->units, max_units * sizeof(Unit) + 64));
+  unsigned *d_counter = static_cast<unsigned *>(dev_mem(c, c->counter, members.size() * 64));
+  long long *d_base = static_cast<long long *>(dev_mem(c, c->meta, nj * (sizeof(long long) + 3 * sizeof(int)) + 64));
+  HIPCHECK(hipMemcpyAsync(d_jobs_raw, jobs.data(), jobs_bytes, hipMemcpyHostToDevice, c->stream));
+  if (strike_bytes)
+    HIPCHECK(hipMemcpyAsync(d_strikes, strikes.data(), strike_bytes, hipMemcpyHostToDevice, c->stream));
+  const Job<R> *d_jobs = reinterpret_cast<const Job<R> *>(d_jobs_raw);
   // units of one group for block size blk, most expensive job first
   auto units_for = [&](const std::vector<size_t> &idx, int blk) {
     std::vector<Unit> u;
@@ -988,7 +1051,7 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
           HIPCHECK(hipMemsetAsync(c->tcounter.p, 0, 64, c->stream));
           HIPCHECK(hipEventRecord(c->ev0, c->stream));
           HIPCHECK(launch_mc<R>(std::get<0>(k), std::get<1>(k), std::get<2>(k), sh, c->tjobs.as<Job<R>>(),
-                                c->tunits.as<Unit>(), n_units, c->tcounter.as<unsigned>(), c->accg.as<double>(),
+                                c->tunits.as<Unit>(), n_units, c->tcounter.as<unsigned>(), d_accg,
                                 smem, grid_for(k, sh, gridf, smem, n_units), c->stream));
           HIPCHECK(hipEventRecord(c->ev1, c->stream));
           HIPCHECK(hipEventSynchronize(c->ev1));
@@ -1027,22 +1090,22 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     all_units.insert(all_units.end(), u.begin(), u.end());
     order.push_back(kv.first);
   }
-  c->units.reserve(all_units.size() * sizeof(Unit) + 64);
-  c->counter.reserve(members.size() * 64);
-  HIPCHECK(hipMemsetAsync(c->counter.p, 0, members.size() * 64, c->stream));
-  HIPCHECK(hipMemcpyAsync(c->units.p, all_units.data(), all_units.size() * sizeof(Unit), hipMemcpyHostToDevice,
+  if (all_units.size() > max_units) throw Fail(MYQL_E_HIP, "work-unit count exceeds its buffer");
+  // queue counters start at zero (a fill kernel: callers may capture this
+  // sequence in a graph, where memset nodes are avoided)
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, c->stream, d_counter,
+                     static_cast<int>(members.size() * 16));
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(d_units, all_units.data(), all_units.size() * sizeof(Unit), hipMemcpyHostToDevice,
                           c->stream));
 
   // meta for the reduction: per job the element offset of its partials,
   // its unit count, strike count and output offset
-  const size_t nj = jobs.size();
-  c->meta.reserve(nj * (sizeof(long long) + 3 * sizeof(int)) + 64);
   std::vector<long long> mbase(pbase.begin(), pbase.end());
   std::vector<int> mint;
   mint.insert(mint.end(), n_slots.begin(), n_slots.end());
   mint.insert(mint.end(), nks.begin(), nks.end());
   mint.insert(mint.end(), out_off.begin(), out_off.end());
-  long long *d_base = c->meta.as<long long>();
   int *d_mint = reinterpret_cast<int *>(d_base + nj);
   HIPCHECK(hipMemcpyAsync(d_base, mbase.data(), nj * sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_mint, mint.data(), mint.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -1055,10 +1118,9 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     const McShape sh = sg.first;
     const size_t smem = staging(k, sh.blk);
     const int n_units = static_cast<int>(ranges[gi].second);
-    HIPCHECK(launch_mc<R>(std::get<0>(k), std::get<1>(k), std::get<2>(k), sh, d_jobs,
-                          c->units.as<Unit>() + ranges[gi].first, n_units, c->counter.as<unsigned>() + 16 * gi,
-                          c->accg.as<double>() + acc_off, smem, grid_for(k, sh, sg.second, smem, n_units),
-                          c->stream));
+    HIPCHECK(launch_mc<R>(std::get<0>(k), std::get<1>(k), std::get<2>(k), sh, d_jobs, d_units + ranges[gi].first,
+                          n_units, d_counter + 16 * gi, d_accg + acc_off, smem,
+                          grid_for(k, sh, sg.second, smem, n_units), c->stream));
     acc_off += static_cast<size_t>(n_units) * 2 * NSTAT * sh.blk;
   }
   {
@@ -1067,13 +1129,13 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     const int tb = 128;
     dim3 grid((maxw + tb - 1) / tb, static_cast<unsigned>(jobs.size()));
     const int nji = static_cast<int>(nj);
-    hipLaunchKernelGGL(reduce_partials_kernel, grid, dim3(tb), 0, c->stream, c->partial.as<double>(),
-                       c->sums.as<double>(), d_base, d_mint, d_mint + nj, d_mint + 2 * nj, nji);
+    hipLaunchKernelGGL(reduce_partials_kernel, grid, dim3(tb), 0, c->stream, d_partial, d_sums, d_base, d_mint,
+                       d_mint + nj, d_mint + 2 * nj, nji);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev1, c->stream));
   std::vector<double> sums(soff);
-  HIPCHECK(hipMemcpyAsync(sums.data(), c->sums.p, soff * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(sums.data(), d_sums, soff * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -1852,6 +1914,20 @@ myql_ctx *myql_create(int device) {
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));
     HIPCHECK(hipEventCreate(&c->ev1));
+    {
+      // this device's copy of the no-allocation scratch array: its address
+      // is looked up once per device, with the device current, and kept
+      static std::mutex mu;
+      static std::map<int, void *> addr;
+      std::lock_guard<std::mutex> lock(mu);
+      auto it = addr.find(device);
+      if (it == addr.end()) {
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        it = addr.emplace(device, p).first;
+      }
+      c->scratch = it->second;
+    }
     c->jump_polys = host_jump_polys(64);
     c->d_polys.reserve(c->jump_polys.size() * 8);
     HIPCHECK(hipMemcpy(c->d_polys.p, c->jump_polys.data(), c->jump_polys.size() * 8, hipMemcpyHostToDevice));
@@ -1906,6 +1982,32 @@ int myql_mc_calculate(myql_ctx *c, const myql_model *model, const int32_t scheme
   try {
     set_device(c);
     if (greek_mode < MYQL_GREEK_NONE || greek_mode > MYQL_GREEK_FULL) throw Fail(MYQL_E_INVALID_ARG, "bad greek mode");
+    JobSpec s{model, schemes, instr, greek_mode, cfg, S0, r, q, res, 0, 0};
+    run_specs(c, {s});
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_mc_calculate_noalloc(myql_ctx *c, const myql_model *model, const int32_t schemes[2],
+                              const myql_instrument *instr, int32_t greek_mode, const myql_mc_config *cfg, double S0,
+                              double r, double q, myql_mc_result *res) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    // no cache trimming here (that frees): only make the device current
+    HIPCHECK(hipSetDevice(c->device));
+    if (!c->scratch) throw Fail(MYQL_E_HIP, "myql_mc_calculate_noalloc: no scratch address for this device");
+    if (greek_mode < MYQL_GREEK_NONE || greek_mode > MYQL_GREEK_FULL) throw Fail(MYQL_E_INVALID_ARG, "bad greek mode");
+    struct Mode {
+      myql_ctx *c;
+      explicit Mode(myql_ctx *x) : c(x) {
+        c->noalloc = true;
+        c->scratch_off = 0;
+      }
+      ~Mode() { c->noalloc = false; }
+    } mode(c);
     JobSpec s{model, schemes, instr, greek_mode, cfg, S0, r, q, res, 0, 0};
     run_specs(c, {s});
     c->err.clear();

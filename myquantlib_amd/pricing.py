@@ -240,6 +240,18 @@ class Engine:
         self._check(rc)
         return buf.as_dict()
 
+    def mc_noalloc(self, model: Model, schemes, instr: InstrumentSpec, mode: int, cfg: MCConfig,
+                   S0: float, r: float, q: float) -> dict:
+        """mc() through myql_mc_calculate_noalloc: the call's device memory
+        comes from the library's per-GPU scratch array, never an allocation."""
+        s = instr.struct()
+        buf = ResultBuffers(s.n_strikes)
+        rc = self.lib.myql_mc_calculate_noalloc(self.ctx, C.byref(model), schemes_array(schemes),
+                                                C.byref(s), mode, C.byref(cfg), S0, r, q,
+                                                C.byref(buf.struct))
+        self._check(rc)
+        return buf.as_dict()
+
     @staticmethod
     def _job_array(jobs: list):
         keep, arr, bufs = [], (capi.MCJob * len(jobs))(), []
