@@ -85,7 +85,7 @@ STRIP_STREAMS = 1 << 18
 
 
 def jobs(paths: int, precision: int, layout: int = 0, streams: int = STRIP_STREAMS, mode: int = None,
-         seed: int = 42, bates_kou: bool = False, flags: int = 0):
+         seed: int = 42, bates_kou: bool = False, flags: int = 0, payoff: int = 0):
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import InstrumentSpec, mc_config
     m = model_bates_kou() if bates_kou else model()
@@ -93,7 +93,7 @@ def jobs(paths: int, precision: int, layout: int = 0, streams: int = STRIP_STREA
     mode = capi.GREEK_FULL if mode is None else mode
     out = []
     for T in MATURITIES:
-        inst = InstrumentSpec(strikes=STRIKES, maturity=T)
+        inst = InstrumentSpec(payoff=payoff, strikes=STRIKES, maturity=T)
         cfg = mc_config(paths, int(round(100 * T)), seed=seed, precision=precision,
                         stream_layout=layout, num_streams=streams, flags=flags)
         out.append((m, [scheme], inst, mode, cfg, S0, R, Q))
@@ -109,36 +109,69 @@ def model_bates_kou():
     return make_model(1, [Heston(2.0, 0.04, 0.5, -0.7, 0.04)], jump_kind=jk, jump=jp)
 
 
-def cv_leg(eng, eval_paths: int, bates_kou: bool):
-    """Price-only plain FP32 and learned-CV FP32 on the strip's 512 options:
-    ms to 1e-4 relative SE for each, the CV's time including its training
-    (2^20 training paths per maturity, seed 7, 2,000 Adam steps of 8,192
-    rows, lr 1e-3, 8 control points; evaluation seed 42)."""
+def cv_leg(eng, eval_paths: int, bates_kou: bool, payoff: int = 0, points: int = 8):
+    """Plain FP32, plain FP64 and learned-CV FP32 on the strip's 512 options
+    (vanilla calls, price only; or auto-smoothed cash digitals, which the
+    reference smooths in Essential mode, priced in that mode): ms to 1e-4
+    relative SE for each. The CV network is trained once (2^20 training
+    paths per maturity, seed 7, 2,000 Adam steps of 8,192 rows, lr 1e-3,
+    8 control points; the regressor coefficients are fitted on the same
+    training paths) and then evaluated on independent paths (seed 42), so
+    its time to the target is the training time plus the evaluation time
+    scaled to the target: t_train + t_eval (worst se / price / 1e-4)^2."""
     from myquantlib_amd import capi
     from myquantlib_amd.pricing import cv_config
-    J = jobs(eval_paths, capi.FP32, mode=capi.GREEK_NONE, bates_kou=bates_kou)
-    eng.mc_batch(J)
+    mode = capi.GREEK_NONE if payoff == capi.PAYOFF_VANILLA else capi.GREEK_ESSENTIAL
+    J = jobs(eval_paths, capi.FP32, mode=mode, bates_kou=bates_kou, payoff=payoff)
+    J64 = jobs(eval_paths, capi.FP64, mode=mode, bates_kou=bates_kou, payoff=payoff)
+    timed = {}
+    for name, jj in (("fp32", J), ("fp64", J64)):
+        eng.mc_batch(jj)
+        t0 = time.perf_counter()
+        r = eng.mc_batch(jj)
+        timed[name] = (r, 1e3 * (time.perf_counter() - t0))
+    plain, tp = timed["fp32"]
+    plain64, tp64 = timed["fp64"]
+    cfg = cv_config(train_paths=1 << 20, train_seed=7, steps=2000, batch=8192, lr=1e-3, points=points)
     t0 = time.perf_counter()
-    plain = eng.mc_batch(J)
-    tp = 1e3 * (time.perf_counter() - t0)
-    cfg = cv_config(train_paths=1 << 20, train_seed=7, steps=2000, batch=8192, lr=1e-3, points=8)
-    t0 = time.perf_counter()
-    net = eng.cv_train(jobs(1024, capi.FP32, mode=capi.GREEK_NONE, seed=7, bates_kou=bates_kou), cfg)
+    net = eng.cv_train(jobs(1024, capi.FP32, mode=mode, seed=7, bates_kou=bates_kou, payoff=payoff), cfg)
     tt = 1e3 * (time.perf_counter() - t0)
+    eng.mc_batch_cv(J, cfg, net)  # warm-up
     t0 = time.perf_counter()
     res = eng.mc_batch_cv(J, cfg, net)
     tc = 1e3 * (time.perf_counter() - t0)
     mp, wp = ms_to_target(plain, tp)
-    mc, wc = ms_to_target(res, tc + tt)
+    m64, w64 = ms_to_target(plain64, tp64)
+    _, wc = ms_to_target(res, tc)
+    mc = tt + tc * (wc / 1e-4) ** 2
     vr = np.concatenate([r["var_ratio"] for r in res])
     used = int(sum(int(np.sum(r["used_cv"])) for r in res))
-    print(f"# cv leg ({'bates-kou NV' if bates_kou else 'bates NCI strip'}): plain {tp:.0f} ms, "
-          f"cv {tc:.0f} ms + train {tt:.0f} ms, median variance ratio {np.median(vr):.2f}",
-          file=sys.stderr, flush=True)
-    return {"eval_paths_per_maturity": eval_paths, "plain_fp32_ms": tp, "plain_fp32_ms_to_1e-4": mp,
-            "plain_worst_rel_se": wp, "learned_cv_fp32_ms": tc, "train_ms": tt,
+    # the options that set the metric, plain and CV, and their variance ratios
+    def worst_option(rs):
+        best = (0.0, None)
+        for ti, r in enumerate(rs):
+            ok = r["price"] >= 0.1
+            rel = np.where(ok, r["price_std_err"] / np.maximum(r["price"], 1e-300), 0.0)
+            k = int(np.argmax(rel))
+            if rel[k] > best[0]:
+                best = (float(rel[k]), (MATURITIES[ti], float(STRIKES[k]), float(res[ti]["var_ratio"][k])))
+        return best[1]
+    label = ("bates-kou NV" if bates_kou else "bates NCI") + (" digitals" if payoff else "")
+    print(f"# cv leg ({label}): plain fp32 {tp:.0f} ms, fp64 {tp64:.0f} ms, cv {tc:.0f} ms + train {tt:.0f} ms, "
+          f"median variance ratio {np.median(vr):.2f}, ms to 1e-4: plain fp64 {m64:.3g}, cv {mc:.3g} "
+          f"({m64 / mc:.2f}x)", file=sys.stderr, flush=True)
+    return {"payoff": "cash digital, auto-smoothed (Essential)" if payoff else "vanilla call",
+            "eval_paths_per_maturity": eval_paths, "control_points": points,
+            "plain_fp32_ms": tp, "plain_fp32_ms_to_1e-4": mp, "plain_worst_rel_se": wp,
+            "plain_fp64_ms": tp64, "plain_fp64_ms_to_1e-4": m64,
+            "learned_cv_fp32_ms": tc, "train_ms": tt, "train_device_ms": net.train_ms,
             "learned_cv_fp32_ms_to_1e-4": mc, "cv_worst_rel_se": wc,
-            "variance_ratio_median": float(np.median(vr)), "variance_ratio_max": float(np.max(vr)),
+            "learned_cv_fp32_ms_to_1e-4_training_per_run": (tt + tc) * (wc / 1e-4) ** 2,
+            "speedup_vs_plain_fp64": m64 / mc,
+            "worst_option_plain": {"T_K_var_ratio": worst_option(plain)},
+            "worst_option_cv": {"T_K_var_ratio": worst_option(res)},
+            "variance_ratio_median": float(np.median(vr)), "variance_ratio_min": float(np.min(vr)),
+            "variance_ratio_max": float(np.max(vr)),
             "options_using_cv": used, "options": int(vr.size), "train_loss": net.final_loss}
 
 
@@ -598,6 +631,7 @@ def main():
         cv_paths = min(paths, 1 << 22)
         cv["bates_strip"] = cv_leg(eng, cv_paths, bates_kou=False)
         cv["bates_kou_nv_strip"] = cv_leg(eng, cv_paths, bates_kou=True)
+        cv["bates_kou_nv_smoothed_digitals"] = cv_leg(eng, cv_paths, bates_kou=True, payoff=capi.PAYOFF_CASH)
 
     mlmc = None if args.no_mlmc else mlmc_leg(eng)
     batched = None if args.no_batched else batched_leg(eng, flags=flags)
@@ -637,6 +671,7 @@ def main():
         "ms_to_1e-4_rel_se": {
             "plain_fp32": po32, "plain_fp64": po64,
             "learned_cv_fp32": cv["bates_strip"]["learned_cv_fp32_ms_to_1e-4"] if cv else None,
+            "learned_cv_speedup_vs_plain_fp64": cv["bates_strip"]["speedup_vs_plain_fp64"] if cv else None,
             "full_greeks_fp32": t32, "full_greeks_fp64": ms64},
         "learned_cv": cv or None,
         "mlmc": mlmc,

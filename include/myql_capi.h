@@ -298,16 +298,30 @@ int myql_fourier_calculate_batch(myql_ctx *ctx, const myql_fourier_job *jobs, in
 
 /* ---- learned martingale control variate ---------------------------------
  * No counterpart in the reference (its README.md:332 lists control variates
- * as future work). u(t, x) ~ E[e^{-r(T-t)} P | X_t = x] is an MLP
- * 6 -> 64 -> 64 -> 64 -> 1 (SiLU, 8,833 parameters) on the features
- * (t, log(S/K), v1, v2, T, log(K/S0)), trained by regression of discounted
- * payoffs on engine paths. Pricing then subtracts, per path and strike,
- *   CV = sum_j e^{r(T - t_j)} grad_x u(t_j, X_j) . sigma(X_j) dW_j
- * over `points` control points (integrand frozen on each block, dW the
- * stepper's own increments), which has zero mean for any weights. Europeans
- * with price only (GreekMode None); all jobs of one call share the model,
- * schemes, payoff, option type and (S0, r, q); at most 64 strikes per job. */
-enum { MYQL_CV_NPARAM = 8833, MYQL_CV_MAX_POINTS = 16 };
+ * as future work). A network u(t, log S, v1, v2, T) -> R^64 gives, for
+ * every strike K_k of the strip at once (a 64-output head over the strikes,
+ * shared by the call's maturities), u_k(t, X) ~ E[e^{-r(T-t)} P_k | X_t]:
+ * an MLP 5 -> 64 -> 64 -> 64 -> 64 with SiLU activations (12,864
+ * parameters), trained in FP32 on engine paths and evaluated on BF16 MFMA
+ * (FP32 accumulate) in forward mode. Per path and strike the estimator is
+ *   P_k - c_A M_A - c_V M_V - c_J M_J - c_D D
+ * with zero-mean terms
+ *   M_A = sum_j e^{r(T-t_j)} du_k/dx (t_j, X_j) A_j          (stock noise)
+ *   M_V = sum_j e^{r(T-t_j)} sum_f (du_k/dx C2_f + du_k/dv_f) V_f,j
+ *                                      (variance noise and its stock part)
+ *   M_J = e^{r(T-t_last)} du_k/dx (e^J - 1)      (the European's one jump)
+ *   D   = P_k - E[P_k | S_T-]  (Merton jumps, vanilla and sharp digitals:
+ *                               the conditional expectation in closed form)
+ * where A_j, V_f,j sum the stepper's own increments with exact zero
+ * conditional mean over `points` control-point blocks (x = log S, C2_f the
+ * log-step's loading on factor f's new variance). The coefficients c are
+ * fitted per (maturity, strike) by least squares on the training paths, so
+ * they are independent of the pricing paths and the estimator is unbiased.
+ * Europeans, GreekMode None or Essential (smoothed digitals; delta and gamma
+ * are the plain estimates of the same paths); all jobs of one call share
+ * the model, schemes, payoff, option type, strike strip (<= 64 strikes),
+ * Greek mode and (S0, r, q); at most 16 jobs; default NCI semantics. */
+enum { MYQL_CV_NPARAM = 12864, MYQL_CV_MAX_POINTS = 16, MYQL_CV_MAX_JOBS = 16, MYQL_CV_NREG = 4 };
 
 typedef struct {
   uint64_t train_paths;  /* paths per job in the training run */
@@ -320,11 +334,15 @@ typedef struct {
 } myql_cv_config;
 
 typedef struct {
-  /* W1[64][6], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4 */
+  /* W1[64][5], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], W4[64][64], b4[64] */
   float params[MYQL_CV_NPARAM];
   float norm[6];       /* feature scaling fixed by the training call */
-  double final_loss;   /* mean squared error over the last steps (scaled by S0^2) */
-  double train_ms;     /* GPU time of recording and training */
+  int32_t n_jobs, n_strikes;
+  double maturity[MYQL_CV_MAX_JOBS];
+  /* regressor coefficients per trained job (maturity): M_A, M_V, M_J, D */
+  double coef[MYQL_CV_MAX_JOBS][MYQL_CV_NREG][64];
+  double final_loss;   /* mean squared error of the normalised targets, last steps */
+  double train_ms;     /* GPU time of recording, training and fitting */
 } myql_cv_net;
 
 /* Per job of myql_mc_calculate_batch_cv; every pointer is an n_strikes
@@ -343,7 +361,11 @@ int myql_cv_train(myql_ctx *ctx, const myql_mc_job *jobs, int64_t n_jobs, const 
 
 /* Price each job (its own paths, seed, precision) with the control variate
  * of `net`: result->price / price_std_err hold the CV estimate per strike,
- * or the plain one where the CV does not lower the variance. */
+ * or the plain one where the CV does not lower the variance. The jobs' paths
+ * are recorded in one grid and evaluated in another (per precision, factor
+ * count and Greek mode), in chunks of at most 32 GiB of recordings;
+ * result->device_ms is the whole call's device time. The context keeps the
+ * recording buffers for later calls until myql_destroy. */
 int myql_mc_calculate_batch_cv(myql_ctx *ctx, const myql_mc_job *jobs, int64_t n_jobs,
                                const myql_cv_config *cfg, const myql_cv_net *net,
                                myql_cv_result *cv_results);
@@ -389,10 +411,12 @@ int myql_debug_set_launch_shape(myql_ctx *ctx, int32_t precision, int32_t n_fact
                                 int32_t lockstep_width, int32_t path_dependent,
                                 int32_t block, int32_t unroll, int32_t grid_factor,
                                 int32_t four_wave);
-/* Control-variate network on n caller feature rows (n x 6, scaled features):
- * the BF16 MFMA input gradient d net/d f_{1,2,3} used by the pricing kernel
- * (out: n x 3), and the FP32 training gradient of mean (net - target)^2 over
- * the rows (n a multiple of 32; grad: MYQL_CV_NPARAM; loss: the mean). */
+/* Control-variate network on n caller feature rows (n x 5, scaled
+ * features): the BF16 MFMA forward-mode tangents of the 64 outputs along
+ * f1, f2, f3 used by the pricing kernel (out: n x 64 x 3), and the FP32
+ * training gradient of the mean over rows and outputs of (net - target)^2
+ * (targets n x 64; n a multiple of 32; grad: MYQL_CV_NPARAM; loss: the
+ * mean). */
 int myql_debug_cv_input_grad(myql_ctx *ctx, const float *params, const float *feats,
                              int64_t n, float *out);
 int myql_debug_cv_param_grad(myql_ctx *ctx, const float *params, const float *feats,

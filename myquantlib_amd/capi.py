@@ -108,7 +108,7 @@ class MLMCResult(C.Structure):
                 ("beta_estimated", C.c_double), ("device_ms", C.c_double)]
 
 
-CV_NPARAM, CV_MAX_POINTS = 8833, 16
+CV_NPARAM, CV_MAX_POINTS, CV_MAX_JOBS, CV_NREG = 12864, 16, 16, 4
 
 
 class CVConfig(C.Structure):
@@ -118,8 +118,10 @@ class CVConfig(C.Structure):
 
 
 class CVNet(C.Structure):
-    _fields_ = [("params", C.c_float * CV_NPARAM), ("norm", C.c_float * 6),
-                ("final_loss", C.c_double), ("train_ms", C.c_double)]
+    _fields_ = [("params", C.c_float * CV_NPARAM), ("norm", C.c_float * 6), ("n_jobs", C.c_int32),
+                ("n_strikes", C.c_int32), ("maturity", C.c_double * CV_MAX_JOBS),
+                ("coef", C.c_double * (CV_MAX_JOBS * CV_NREG * 64)), ("final_loss", C.c_double),
+                ("train_ms", C.c_double)]
 
 
 class CVResult(C.Structure):

@@ -1,29 +1,46 @@
 // Learned martingale control variate (no counterpart in the reference, whose
 // README.md:332 lists control variates as future work).
 //
-// u(t, x) ~ E[e^{-r(T-t)} P(S_T) | X_t = x] is an MLP 6 -> 64 -> 64 -> 64 -> 1
-// with SiLU activations (8,833 parameters) on the features
-//   f = (t, log(S/K), v1, v2, T, log(K/S0)) (each scaled, see CvNorm),
-// trained by regression of the discounted payoff on the state at a few
-// control points of engine-generated paths. The control variate of one path
-// and strike is the discrete stochastic integral
-//   CV = sum_j e^{r(T - t_j)} grad_x u(t_j, X_j) . sigma(X_j) dW_j
-// with the integrand frozen at the control point t_j and dW_j the sum of the
-// stepper's own Brownian increments over [t_j, t_{j+1}) (sqrt(dt) G for the
-// stock's independent part, sqrt(dt) Z for each NV variance factor). Every
-// term has zero mean whatever the weights, so P - CV is an unbiased payoff
-// with lower variance when u is close to the price function.
+// A network u(t, X) -> R^64 gives, for every strike K_k of the strip at
+// once, u_k(t, X) ~ E[e^{-r(T-t)} P_k | X_t] on the features
+//   f = (t, log(S/S0), v1, v2, T) (each scaled, see CvNorm):
+// an MLP 5 -> 64 -> 64 -> 64 -> 64 with SiLU activations whose last layer is
+// a head over the strip's strikes. It is trained on the engine's own paths.
+// Per path and strike the estimator is
+//   Y_k = P_k - c_A M_A - c_V M_V - c_J M_J - c_D D
+// with four zero-mean terms:
+//   M_A = sum_j e^{r(T-t_j)} du_k/dx (t_j, X_j) A_j
+//   M_V = sum_j e^{r(T-t_j)} sum_f (du_k/dx C2_f + du_k/dv_f)(t_j, X_j) V_f,j
+//   M_J = e^{r(T-t_last)} du_k/dx (t_last, X_last) (e^J - 1)
+//   D   = P_k - E[P_k | S_T-]   (Merton; vanilla and sharp digital payoffs)
+// A_j and V_f,j are the block sums of the stepper's own increments whose
+// conditional mean is exactly zero: the stock's independent noise
+// sqrt(C3 (v + v')) G, and each factor's v' - E[v' | v] (closed form for NV
+// and Exact; for NCI, v' - C2 m_N + 2 C2 (N - lambda) with m_N the mean of
+// the table row's quantile map). The integrands are frozen at the control
+// point t_j, so every term has zero mean whatever the network learned; the
+// coefficients c are fitted per (maturity, strike) on the training paths,
+// independent of the pricing paths, so the estimator stays unbiased.
 //
 // Kernels:
-//   cv_paths_kernel   the engine's base path (same streams, partition and
-//                     draws as mc_paths_kernel) recording state and
-//                     increments at the control points and the terminal spot
-//   cv_eval_kernel    per (path, 32-strike tile, control point): MLP forward
-//                     and input gradient on BF16 MFMA (FP32 accumulate), then
-//                     per-strike FP64 sums of P - CV and P
-//   cv_train_kernel   FP32 forward/backward of 32 sampled rows per block,
-//                     per-block parameter gradients
-//   cv_adam_kernel    gradient reduction (fixed order) and Adam update
+//   cv_paths_kernel     the engine's base path (same streams, partition and
+//                       draws as mc_paths_kernel) recording state and the
+//                       block increments at the control points, the
+//                       terminal spot and the spot before the terminal jump
+//   nci_rowmean_kernel  means of the NCI table rows' quantile maps
+//   cv_rb_table_kernel  E[P_k | S_T-] of the n >= 1 jump counts on a log-spot
+//                       grid
+//   cv_moments_kernel   per-strike target scales of the training paths
+//   cv_eval_kernel      per 16 paths and control point: forward-mode
+//                       tangents of the hidden layers on BF16 MFMA (FP32
+//                       accumulate) summed against the block increments,
+//                       the heads once per path, then per (path, strike)
+//                       the estimator (pricing) or the regressors' cross
+//                       products (fitting), summed in FP64 by the strike's
+//                       lane
+//   cv_reduce_kernel    fixed-order sum of the waves' sums
+//   cv_train_kernel     FP32 forward/backward of 32 sampled rows per block
+//   cv_adam_kernel      gradient reduction (fixed order) and Adam update
 #pragma once
 #include "cv_types.hpp"
 #include "mlmc_kernel.hpp"
@@ -36,19 +53,107 @@ __device__ __forceinline__ short bf16_bits(float x) {
 }
 
 typedef short cv_v4s __attribute__((ext_vector_type(4)));
-typedef float cv_v16f __attribute__((ext_vector_type(16)));
+typedef float cv_v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float sigmoidf(float a) { return 1.0f / (1.0f + __expf(-a)); }
+// logistic function with the hardware reciprocal (1 ulp; the network runs
+// in BF16 anyway)
+__device__ __forceinline__ float sigmoidf(float a) { return __builtin_amdgcn_rcpf(1.0f + __expf(-a)); }
+
+__device__ __forceinline__ double norm_cdf(double x) { return 0.5 * erfc(-x * 0.70710678118654752440); }
+__device__ __forceinline__ double norm_pdf(double x) { return 0.39894228040143267794 * exp(-0.5 * x * x); }
+
+// Payoffs.hpp:14-69 (smoothed digitals in Greek modes, as the pricer)
+__device__ __forceinline__ double cv_payoff(const CvSet &s, double S, double K) {
+  const double diff = s.call ? S - K : K - S;
+  if (s.payoff == 0) return diff > 0.0 ? diff : 0.0;
+  const double w = s.smooth ? 0.5 * erfc(-diff * s.eps_scale) : (diff > 0.0 ? 1.0 : 0.0);
+  return s.payoff == 1 ? w : S * w;
+}
+
+// E[P(S e^{J_n})] for J_n ~ N(m, s^2), s > 0, and its derivative in log S
+// (vanilla, sharp cash and asset digitals)
+__device__ __forceinline__ void rb_term(const CvSet &c, double S, double K, double m, double s, double &g,
+                                        double &dg) {
+  const double F = S * exp(m + 0.5 * s * s);
+  const double d2 = (log(S / K) + m) / s, d1 = d2 + s;
+  const double sg = c.call ? 1.0 : -1.0;
+  if (c.payoff == 0) {
+    g = sg * (F * norm_cdf(sg * d1) - K * norm_cdf(sg * d2));
+    dg = sg * F * norm_cdf(sg * d1);
+  } else if (c.payoff == 1) {
+    g = norm_cdf(sg * d2);
+    dg = sg * norm_pdf(d2) / s;
+  } else {
+    g = F * norm_cdf(sg * d1);
+    dg = g + sg * F * norm_pdf(d1) / s;
+  }
+}
+
+// n >= 1 part of E[P_k | S_T- = S] and its derivative in log S, summed
+// directly
+__device__ __forceinline__ void rb_sum(const CvSet &c, double S, double K, double &g, double &dg) {
+  g = 0.0;
+  dg = 0.0;
+  for (int n = 0; n < c.rb_n; ++n) {
+    double a, b;
+    rb_term(c, S, K, c.rb_m[n], c.rb_s[n], a, b);
+    g += c.rb_p[n] * a;
+    dg += c.rb_p[n] * b;
+  }
+}
+
+// E[P_k | S_T- = Sm]: the no-jump term exactly, the rest from the table (or
+// directly off the grid); lSm = log Sm, Sm0 = Sm e^{m0}
+__device__ __forceinline__ double rb_expect(const CvSet &c, int k, double Sm, double lSm, double Sm0, double K) {
+  const double e0 = c.rb_p0 * cv_payoff(c, Sm0, K);
+  const double tp = (lSm - c.rb_y0) * c.rb_inv_h;
+  double g;
+  if (tp >= 0.0 && tp < static_cast<double>(CV_RB_PTS - 1)) {
+    const int i = static_cast<int>(tp);
+    const double t = tp - static_cast<double>(i), t2 = t * t, t3 = t2 * t;
+    const double *a = c.rb + 2 * (static_cast<size_t>(i) * c.nk + k), *b = a + 2 * c.nk;
+    g = (2.0 * t3 - 3.0 * t2 + 1.0) * a[0] + (t3 - 2.0 * t2 + t) * a[1] + (-2.0 * t3 + 3.0 * t2) * b[0] +
+        (t3 - t2) * b[1];
+  } else {
+    double dg;
+    rb_sum(c, Sm, K, g, dg);
+  }
+  return e0 + g;
+}
+
+// storage index t holds a path (the i-th of stream s, t = i S + s)
+__device__ __forceinline__ bool cv_valid(const CvSet &c, uint64_t t) {
+  const uint64_t i = t / c.S, s = t - i * c.S;
+  return i < c.M / c.S + (s < c.M % c.S ? 1 : 0);
+}
 
 // ---------------------------------------------------------------------------
 // recording path kernel
 // ---------------------------------------------------------------------------
+// task of this block in a batched launch
+template <typename T>
+__device__ __forceinline__ int cv_task_of(const T *__restrict__ tasks, int n) {
+  int t = 0;
+  while (t + 1 < n && blockIdx.x >= tasks[t + 1].first_block) ++t;
+  return __builtin_amdgcn_readfirstlane(t);
+}
+
+// One thread per stream of each task's job, the tasks' blocks in one grid.
 template <typename R, int NF>
-__global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restrict__ J, CvSet set) {
+__global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restrict__ jobs,
+                                                         const CvSet *__restrict__ sets,
+                                                         const CvPathTask *__restrict__ tasks, int n_tasks) {
   constexpr int NFX = NF > 0 ? NF : 1;
+  const int ti = cv_task_of(tasks, n_tasks);
+  const Job<R> *__restrict__ J = jobs + __builtin_amdgcn_readfirstlane(tasks[ti].job);
+  const CvSet &set = sets[__builtin_amdgcn_readfirstlane(tasks[ti].set)];
   const uint64_t S = J->S, M = J->M;
-  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x - tasks[ti].first_block) * BLOCK + threadIdx.x;
   if (s >= S) return;
+  float *__restrict__ rec_out = set.rec;
+  const int nj = set.nj;
+  const int *__restrict__ nci_rows = set.nci_rows;
+  const double *__restrict__ nci_mean[2] = {set.nci_mean[0], set.nci_mean[1]};
   Rng g;
   g.s0 = J->streams[4 * s + 0];
   g.s1 = J->streams[4 * s + 1];
@@ -56,58 +161,70 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
   g.s3 = J->streams[4 * s + 3];
   const uint64_t q = M / S, rem = M % S;
   const uint64_t count = q + (s < rem ? 1 : 0);
-  const uint64_t first = s * q + (s < rem ? s : rem);
+  const uint64_t Ms = set.Ms;
   const VarParams<R> *__restrict__ P = &J->var[V_BASE];
   const VarParams<R> *__restrict__ PP[1] = {P};
-  const R sqdt = static_cast<R>(set.sqdt);
   const unsigned steps = static_cast<unsigned>(J->steps);
+  // the i-th path of stream s (its i-th sample, the same draws as the
+  // plain pricer's) is stored at t = i S + s, so the lanes of a wave write
+  // consecutive entries
   for (uint64_t i = 0; i < count; ++i) {
-    const uint64_t p = first + i;
+    const uint64_t p = i * S + s;
     R x = static_cast<R>(J->logS0);
     R v[NFX];
 #pragma unroll
     for (int f = 0; f < NFX; ++f) v[f] = P->v0[f];
     int j = 0;
-    R dwp = 0, dwv[2] = {0, 0};
+    double A = 0.0, V[2] = {0.0, 0.0};
+    auto put = [&](int jj, int c, double val) {
+      rec_out[(static_cast<size_t>(jj) * CV_REC + c) * Ms + p] = static_cast<float>(val);
+    };
     if constexpr (NF == 0) {
       // zero-factor European: one step to maturity (ASVJstepper.hpp:145-161),
-      // one control point at t = 0 with dW = sqrt(T) G (set.sqdt = sqrt(T))
-      set.rec[0 * M + p] = static_cast<float>(x);
-      set.rec[1 * M + p] = 0.0f;
-      set.rec[2 * M + p] = 0.0f;
+      // one control point at t = 0
+      put(0, 0, x);
+      put(0, 1, 0.0);
+      put(0, 2, 0.0);
       const R G = normcdfinv<R>(uniform(g));
       x += P->gdrift_n + P->gvol_n * G;
-      dwp = sqdt * G;
+      A = static_cast<double>(P->gvol_n * G);
     } else {
-    for (unsigned step = 0; step < steps; ++step) {
-      if (j < set.nj && step == static_cast<unsigned>(set.step_of[j])) {
-        if (j > 0) {  // close the previous block
-          set.rec[((j - 1) * CV_REC + 3) * M + p] = static_cast<float>(dwp);
-          set.rec[((j - 1) * CV_REC + 4) * M + p] = static_cast<float>(dwv[0]);
-          set.rec[((j - 1) * CV_REC + 5) * M + p] = static_cast<float>(dwv[1]);
+      for (unsigned step = 0; step < steps; ++step) {
+        if (j < nj && step == static_cast<unsigned>(set.step_of[j])) {
+          if (j > 0) {  // close the previous block
+            put(j - 1, 3, A);
+            put(j - 1, 4, V[0]);
+            put(j - 1, 5, V[1]);
+          }
+          put(j, 0, x);
+          put(j, 1, v[0]);
+          put(j, 2, NF > 1 ? static_cast<double>(v[NFX - 1]) : 0.0);
+          A = 0.0;
+          V[0] = V[1] = 0.0;
+          ++j;
         }
-        set.rec[(j * CV_REC + 0) * M + p] = static_cast<float>(x);
-        set.rec[(j * CV_REC + 1) * M + p] = NF > 0 ? static_cast<float>(v[0]) : 0.0f;
-        set.rec[(j * CV_REC + 2) * M + p] = NF > 1 ? static_cast<float>(v[NFX - 1]) : 0.0f;
-        dwp = 0;
-        dwv[0] = dwv[1] = 0;
-        ++j;
-      }
-      // one step of ASVJStepper::multi_step with the draws the engine makes
-      const R G = normcdfinv<R>(uniform(g));
-      dwp += sqdt * G;
-      if constexpr (NF == 0) {
-        x += P->gdrift + P->gvol * G;
-      } else {
+        // one step of ASVJStepper::multi_step with the draws the engine makes
+        const R G = normcdfinv<R>(uniform(g));
         R vn[NFX];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
+          const double vd = static_cast<double>(v[f]);
           if (J->scheme[f] == 1) {
             const R Z = normcdfinv<R>(uniform(g));
             vn[f] = nv_step<R, NF>(P, f, v[f], Z);
-            dwv[f] += sqdt * Z;
+            // E[v'|v] = K0 (K0 v + K1 + K2^2) + K1 of the NV step
+            const double K0 = P->Kr[f][0], K1 = P->Kr[f][1], K2 = P->Kr[f][2];
+            V[f] += static_cast<double>(vn[f]) - (K0 * (K0 * vd + K1 + K2 * K2) + K1);
+          } else if (J->scheme[f] == 2) {
+            int N = 0;
+            vn[f] = evolve_one<R, NFX>(J, P, f, v[f], g, &N);
+            const double lam = P->Kd[f][0] * vd;
+            const double mN = N < nci_rows[f] ? nci_mean[f][N] : P->Kd[f][1] + 2.0 * N;
+            V[f] += static_cast<double>(vn[f]) - P->Kd[f][2] * mN + 2.0 * P->Kd[f][2] * (N - lam);
           } else {
             vn[f] = evolve_one<R, NFX>(J, P, f, v[f], g);
+            // E[v'|v] = C2 (alpha + C0 v), the exact CIR mean
+            V[f] += static_cast<double>(vn[f]) - P->Kd[f][2] * (P->Kd[f][1] + P->Kd[f][0] * vd);
           }
         }
         R a = 0, d = 0;
@@ -116,245 +233,514 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
           a += P->C[f][3] * (v[f] + vn[f]);
           d += P->C[f][0] + P->C[f][1] * v[f] + P->C[f][2] * vn[f];
         }
-        x += d + dsqrt<R>(a) * G;
+        const R sG = dsqrt<R>(a) * G;
+        x += d + sG;
+        A += static_cast<double>(sG);
 #pragma unroll
         for (int f = 0; f < NF; ++f) v[f] = vn[f];
       }
     }
-    }
-    set.rec[((set.nj - 1) * CV_REC + 3) * M + p] = static_cast<float>(dwp);
-    set.rec[((set.nj - 1) * CV_REC + 4) * M + p] = static_cast<float>(dwv[0]);
-    set.rec[((set.nj - 1) * CV_REC + 5) * M + p] = static_cast<float>(dwv[1]);
+    put(nj - 1, 3, A);
+    put(nj - 1, 4, V[0]);
+    put(nj - 1, 5, V[1]);
+    const R xm = x;
     R xs[1] = {x};
     euro_jump<R, 1>(J, PP, xs, g);
     set.ST[p] = static_cast<double>(dexp<R>(xs[0]));
-    // the compensated jump factor of the European's single terminal jump
-    if (set.dJ) set.dJ[p] = static_cast<float>(expm1(static_cast<double>(xs[0]) - static_cast<double>(x)));
+    if (set.Sm) set.Sm[p] = static_cast<double>(dexp<R>(xm));
   }
 }
 
 // ---------------------------------------------------------------------------
-// MLP weights in LDS for the MFMA evaluation
+// NCI table row means: m_N = integral over u in [0, 1) of the quantile map the
+// variance step applies to row N (engine default semantics): the exact chi2
+// quantile on [0, dx) for N >= 2, the cubic Hermite pieces on [dx, 0.999) (all
+// of [0, 0.999) for N < 2), the tail above 0.999 (exact mean there; the tail
+// rows agree with the exact quantile to ~1e-11). With k = d + 2N degrees of
+// freedom, the exact pieces are E[X 1{X < q}] = k P(k/2 + 1, q/2) and its
+// complement.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double reg_gamma_p(double a, double x) {
+  if (x <= 0.0) return 0.0;
+  if (x < a + 1.0) return igam_series(a, x, lgamma(a + 1.0));
+  return 1.0 - igamc_cf(a, x, lgamma(a));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) nci_rowmean_kernel(const R *__restrict__ tab, int rows, int grid, double dx,
+                                                          double d, double *__restrict__ mean) {
+  const int N = blockIdx.x;
+  if (N >= rows) return;
+  __shared__ double red[256];
+  const R *row = tab + 2 * static_cast<size_t>(N) * grid;
+  const int i0 = N >= 2 ? 1 : 0;
+  double s = 0.0;
+  for (int i = i0 + threadIdx.x; i < grid - 1; i += blockDim.x) {
+    const double y0 = row[2 * i], m0 = row[2 * i + 1], y1 = row[2 * i + 2], m1 = row[2 * i + 3];
+    s += 0.5 * (y0 + y1) + (m0 - m1) * (1.0 / 12.0);
+  }
+  red[threadIdx.x] = s * dx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double k = d + 2.0 * N, a = 0.5 * k + 1.0;
+    double m = red[0];
+    if (N >= 2) m += k * reg_gamma_p(a, 0.5 * chi2_quantile(k, dx));
+    m += k * (1.0 - reg_gamma_p(a, 0.5 * chi2_quantile(k, 0.999)));
+    mean[N] = m;
+  }
+}
+
+// E[P_k | S_T-] of the n >= 1 jump counts on the log-spot grid
+__global__ void cv_rb_table_kernel(CvSet set, double *__restrict__ tab) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= set.nk * CV_RB_PTS) return;
+  const int i = idx / set.nk, k = idx % set.nk;
+  const double y = set.rb_y0 + i * set.rb_h;
+  double g, dg;
+  rb_sum(set, exp(y), set.strikes[k], g, dg);
+  tab[2 * idx] = g;
+  tab[2 * idx + 1] = set.rb_h * dg;
+}
+
+// discounted (terminal-jump-expected) payoff of path p, strike k: the
+// training target before normalisation
+__device__ __forceinline__ double cv_target(const CvSet &c, uint64_t p, int k) {
+  const double K = c.strikes[k];
+  if (c.rb_on) {
+    const double Sm = c.Sm[p];
+    return rb_expect(c, k, Sm, log(Sm), Sm * exp(c.rb_m0), K);
+  }
+  return cv_payoff(c, c.ST[p], K);
+}
+
+// sums of df * target and its square per (strike, block); grid (blocks, nk)
+__global__ void __launch_bounds__(256) cv_moments_kernel(CvSet set, double *__restrict__ part) {
+  __shared__ double r0[256], r1[256];
+  const int k = blockIdx.y;
+  double s = 0.0, s2 = 0.0;
+  for (uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < set.Ms;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (!cv_valid(set, p)) continue;
+    const double y = set.df * cv_target(set, p, k);
+    s += y;
+    s2 += y * y;
+  }
+  r0[threadIdx.x] = s;
+  r1[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r0[threadIdx.x] += r0[threadIdx.x + w];
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * (static_cast<size_t>(k) * gridDim.x + blockIdx.x)] = r0[0];
+    part[2 * (static_cast<size_t>(k) * gridDim.x + blockIdx.x) + 1] = r1[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// network weights in LDS for the MFMA evaluation
+// ---------------------------------------------------------------------------
+// All products run on v_mfma_f32_16x16x16bf16_1k in the transposed form
+// out^T = W x^T (16 units x 16 rows per tile): the accumulator of output
+// tile m (lane l holds units 16m + 4(l/16) + 0..3 of row l % 16) is exactly
+// the B fragment of input chunk m of the next product, so activations never
+// leave the lane. Layer 1 is a product of W1 extended by the bias column
+// against (f, 1); the biases of layers 2 and 3 seed the accumulators, and
+// the columns of W1 seed the tangents. hd[0..3): the head's rows W4[k]
+// (fitting, checks) or, for pricing, W4[k] scaled by xs c_A,k, xs c_V,k and
+// xs c_J,k, so that a head product is the coefficient-weighted tangent.
+constexpr int CV_P1 = 20;  // bf16 pitch of the 16-column matrix W1
 struct CvLds {
-  short w2[CV_H * CV_PITCH], w3[CV_H * CV_PITCH];    // [n][k]
-  short w2t[CV_H * CV_PITCH], w3t[CV_H * CV_PITCH];  // [k][n]
-  float w1[CV_H * CV_IN], b1[CV_H], b2[CV_H], b3[CV_H], w4[CV_H];
+  short w2[CV_H * CV_PITCH], w3[CV_H * CV_PITCH];  // [out][in]
+  short hd[3][CV_OUT * CV_PITCH];
+  short w1[CV_H * CV_P1];                 // [unit][W1 0..4, b1, 0...]
+  alignas(16) float bias[2][CV_H];        // b2, b3
+  alignas(16) float w1c[3][CV_H];         // columns 1..3 of W1
 };
 
-__device__ __forceinline__ void cv_load_lds(CvLds &L, const float *__restrict__ prm) {
+__device__ __forceinline__ void cv_load_lds(CvLds &L, const float *__restrict__ prm, const double *__restrict__ coef,
+                                            float xs, int nk) {
   for (int i = threadIdx.x; i < CV_H * CV_H; i += blockDim.x) {
     const int n = i / CV_H, k = i % CV_H;
-    const short a = bf16_bits(prm[CV_W2 + i]), b = bf16_bits(prm[CV_W3 + i]);
-    L.w2[n * CV_PITCH + k] = a;
-    L.w3[n * CV_PITCH + k] = b;
-    L.w2t[k * CV_PITCH + n] = a;
-    L.w3t[k * CV_PITCH + n] = b;
+    L.w2[n * CV_PITCH + k] = bf16_bits(prm[CV_W2 + i]);
+    L.w3[n * CV_PITCH + k] = bf16_bits(prm[CV_W3 + i]);
+    if (!coef) {
+      L.hd[0][n * CV_PITCH + k] = bf16_bits(prm[CV_W4 + i]);
+    } else {
+      // row n of the head is strike n
+#pragma unroll
+      for (int h = 0; h < 3; ++h) {
+        const float sc = n < nk ? static_cast<float>(xs * coef[h * CV_OUT + n]) : 0.0f;
+        L.hd[h][n * CV_PITCH + k] = bf16_bits(sc * prm[CV_W4 + i]);
+      }
+    }
   }
-  for (int i = threadIdx.x; i < CV_H * CV_IN; i += blockDim.x) L.w1[i] = prm[CV_W1 + i];
+  for (int i = threadIdx.x; i < CV_H * 16; i += blockDim.x) {
+    const int n = i / 16, k = i % 16;
+    const float w = k < CV_IN ? prm[CV_W1 + n * CV_IN + k] : (k == CV_IN ? prm[CV_B1 + n] : 0.0f);
+    L.w1[n * CV_P1 + k] = bf16_bits(w);
+  }
   for (int i = threadIdx.x; i < CV_H; i += blockDim.x) {
-    L.b1[i] = prm[CV_B1 + i];
-    L.b2[i] = prm[CV_B2 + i];
-    L.b3[i] = prm[CV_B3 + i];
-    L.w4[i] = prm[CV_W4 + i];
+    L.bias[0][i] = prm[CV_B2 + i];
+    L.bias[1][i] = prm[CV_B3 + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L.w1c[c][i] = prm[CV_W1 + i * CV_IN + 1 + c];
   }
 }
 
-// Feature index n held by this lane in accumulator element a of tile nt
-// (v_mfma_f32_32x32x8bf16_1k: D[8(a/4) + 4(lane/32) + a%4][lane%32]).
-__device__ __forceinline__ int cv_feat(int nt, int a, int hi) { return 32 * nt + 8 * (a >> 2) + 4 * hi + (a & 3); }
+typedef _Float16 cv_h4 __attribute__((ext_vector_type(4)));
+
+// Unit held by lane l in element e of output tile m.
+__device__ __forceinline__ int cv_unit(int m, int e, int l) { return 16 * m + 4 * (l >> 4) + e; }
 
 // A zero the compiler cannot see through: LDS weight reads indexed with it
-// are re-issued on every call instead of being hoisted out of the caller's
-// path loop into registers (hundreds of VGPRs for the MFMA A fragments).
+// are re-issued on every product instead of being hoisted out of the
+// caller's path loop into registers.
 __device__ __forceinline__ int cv_opaque_zero() {
   int z = 0;
   asm volatile("" : "+s"(z));
   return z;
 }
 
-// out^T = W x^T for 64 outputs: A = W rows (LDS), B = x^T fragments.
-__device__ __forceinline__ void cv_layer(const short *__restrict__ W, const cv_v4s (&B)[8], cv_v16f (&acc)[2],
-                                         int lane, int z) {
-  const int r = lane & 31, hi = lane >> 5;
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt) {
-    cv_v16f c = {0};
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const cv_v4s a = *reinterpret_cast<const cv_v4s *>(W + (32 * nt + r) * CV_PITCH + 8 * s + 4 * hi + z);
-      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, B[s], c, 0, 0, 0);
-    }
-    acc[nt] = c;
-  }
+__device__ __forceinline__ cv_v4f cv_mfma(cv_v4s a, cv_v4s b, cv_v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// Input gradient d net / d f_i for i = 1, 2, 3 (log-moneyness, v1, v2) of the
-// row held by lanes (lane % 32, lane % 32 + 32), returned in both lanes.
-// Activations go straight into the BF16 B fragments of the next product
-// (fragment s = features 8s + 4(lane/32) + 0..3, the accumulator layout of
-// tile s/4); SiLU derivatives are kept as FP16.
-__device__ __forceinline__ void cv_grad(const CvLds &L, const float (&f)[CV_IN], float &g1, float &g2, float &g3,
-                                        int lane) {
-  const int hi = lane >> 5;
+// out^T = W x^T for 64 outputs and 64 inputs: acc[m] += W[16m.., :] x^T
+__device__ __forceinline__ void cv_layer(const short *__restrict__ W, const cv_v4s (&B)[4], cv_v4f (&acc)[4], int l) {
   const int z = cv_opaque_zero();
-  _Float16 d1[2][16], d2[2][16];
-  cv_v4s B[8];
-  // layer 1 on VALU (K = 6)
+  const short *base = W + (l & 15) * CV_PITCH + 4 * (l >> 4) + z;
 #pragma unroll
-  for (int nt = 0; nt < 2; ++nt)
+  for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int n = cv_feat(nt, a, hi) + z;
-      float y = L.b1[n];
-#pragma unroll
-      for (int i = 0; i < CV_IN; ++i) y += L.w1[n * CV_IN + i] * f[i];
-      const float sg = sigmoidf(y), h = y * sg;
-      d1[nt][a] = static_cast<_Float16>(sg + h * (1.0f - sg));
-      B[4 * nt + (a >> 2)][a & 3] = bf16_bits(h);
-    }
-  cv_v16f acc[2];
-  cv_layer(L.w2, B, acc, lane, z);
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const float y = acc[nt][a] + L.b2[cv_feat(nt, a, hi) + z];
-      const float sg = sigmoidf(y), h = y * sg;
-      d2[nt][a] = static_cast<_Float16>(sg + h * (1.0f - sg));
-      B[4 * nt + (a >> 2)][a & 3] = bf16_bits(h);
-    }
-  cv_layer(L.w3, B, acc, lane, z);
-  // delta3 = w4 * silu'(a3)
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int n = cv_feat(nt, a, hi) + z;
-      const float y = acc[nt][a] + L.b3[n];
-      const float sg = sigmoidf(y), h = y * sg;
-      B[4 * nt + (a >> 2)][a & 3] = bf16_bits(L.w4[n] * (sg + h * (1.0f - sg)));
-    }
-  cv_layer(L.w3t, B, acc, lane, z);  // W3^T delta3
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-    for (int a = 0; a < 16; ++a) B[4 * nt + (a >> 2)][a & 3] = bf16_bits(acc[nt][a] * static_cast<float>(d2[nt][a]));
-  cv_layer(L.w2t, B, acc, lane, z);  // W2^T delta2
-  float s1 = 0.f, s2 = 0.f, s3 = 0.f;
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int n = cv_feat(nt, a, hi) + z;
-      const float d = acc[nt][a] * static_cast<float>(d1[nt][a]);
-      s1 += L.w1[n * CV_IN + 1] * d;
-      s2 += L.w1[n * CV_IN + 2] * d;
-      s3 += L.w1[n * CV_IN + 3] * d;
-    }
-  g1 = s1 + __shfl_xor(s1, 32, 64);
-  g2 = s2 + __shfl_xor(s2, 32, 64);
-  g3 = s3 + __shfl_xor(s3, 32, 64);
+    for (int s = 0; s < 4; ++s)
+      acc[m] = cv_mfma(*reinterpret_cast<const cv_v4s *>(base + 16 * m * CV_PITCH + 16 * s), B[s], acc[m]);
 }
 
-__device__ __forceinline__ double cv_payoff(int payoff, int call, double S, double K) {
-  const double diff = call ? S - K : K - S;
-  if (payoff == 0) return diff > 0.0 ? diff : 0.0;
-  const double w = diff > 0.0 ? 1.0 : 0.0;
-  return payoff == 1 ? w : S * w;
+// the lane's units 16m + 4(l/16) + 0..3 of a 64-float vector in LDS
+__device__ __forceinline__ cv_v4f cv_units4(const float *__restrict__ v, int m, int l) {
+  return *reinterpret_cast<const cv_v4f *>(v + 16 * m + 4 * (l >> 4) + cv_opaque_zero());
 }
 
-// ---------------------------------------------------------------------------
-// evaluation: each wave takes paths p = wave, wave + n_waves, ...; lanes
-// (r, r + 32) hold row r = strike 32*tile + r. Per-wave partial sums
-// part[wave][k][4] = (sum P-CV, sum (P-CV)^2, sum P, sum P^2).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) cv_eval_kernel(const float *__restrict__ prm, CvNorm nm, CvSet set,
-                                                        double *__restrict__ part) {
-  __shared__ CvLds L;
-  cv_load_lds(L, prm);
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int r = lane & 31;
-  const uint64_t n_waves = static_cast<uint64_t>(gridDim.x) * (BLOCK / 64);
-  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (BLOCK / 64) + threadIdx.x / 64;
-  const uint64_t M = set.M;
-  const int n_tiles = (set.nk + 31) / 32;
-  // strike tiles outermost (wave-uniform trip count); the path records are
-  // read once per tile
-  for (int t = 0; t < n_tiles; ++t) {
-    const int k = 32 * t + r;
-    const bool live = k < set.nk;
-    const double K = live ? set.strikes[k] : static_cast<double>(nm.S0);
-    const float lk = static_cast<float>(log(K));
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (uint64_t p = wave; p < M; p += n_waves) {
-      double cv = 0.0;
-      float ux_last = 0.f;
-      for (int j = 0; j < set.nj; ++j) {
-        const float x = set.rec[(j * CV_REC + 0) * M + p];
-        const float v1 = set.rec[(j * CV_REC + 1) * M + p];
-        const float v2 = set.rec[(j * CV_REC + 2) * M + p];
-        const float dwp = set.rec[(j * CV_REC + 3) * M + p];
-        const float dw1 = set.rec[(j * CV_REC + 4) * M + p];
-        const float dw2 = set.rec[(j * CV_REC + 5) * M + p];
-        const float sv1 = sqrtf(fmaxf(v1, 0.f)), sv2 = sqrtf(fmaxf(v2, 0.f));
-        // stock and variance noise of the frozen integrand
-        float sx_perp, sx1, sx2;
-        if (set.nf == 0) {
-          sx_perp = set.vol;
-          sx1 = sx2 = 0.f;
-        } else {
-          const float vp = set.rperp[0] * fmaxf(v1, 0.f) + (set.nf > 1 ? set.rperp[1] * fmaxf(v2, 0.f) : 0.f);
-          sx_perp = sqrtf(vp);
-          sx1 = set.rho[0] * sv1;
-          sx2 = set.nf > 1 ? set.rho[1] * sv2 : 0.f;
-        }
-        const float ex = sx_perp * dwp + sx1 * dw1 + sx2 * dw2;  // stock martingale increment
-        const float e1 = set.sig[0] * sv1 * dw1, e2 = set.sig[1] * sv2 * dw2;
-        float f[CV_IN];
-        f[0] = set.t_of[j] * nm.ts;
-        f[1] = (x - lk) * nm.xs;
-        f[2] = v1 * nm.vs0;
-        f[3] = v2 * nm.vs1;
-        f[4] = static_cast<float>(set.T) * nm.ts;
-        f[5] = (lk - nm.logS0) * nm.xs;
-        float g1, g2, g3;
-        cv_grad(L, f, g1, g2, g3, lane);
-        const float ux = nm.S0 * nm.xs * g1, u1 = nm.S0 * nm.vs0 * g2, u2 = nm.S0 * nm.vs1 * g3;
-        cv += set.disc_of[j] * static_cast<double>(ux * ex + u1 * e1 + u2 * e2);
-        ux_last = ux;
-      }
-      // terminal jump: u_x dS/S at the last control point times the
-      // compensated jump factor (independent of the path, mean 0)
-      if (set.dJ) cv += set.disc_of[set.nj - 1] * static_cast<double>(ux_last * set.dJ[p]);
-      if (live) {
-        const double P = cv_payoff(set.payoff, set.call, set.ST[p], K);
-        const double Pa = P - cv;
-        a0 += Pa;
-        a1 += Pa * Pa;
-        a2 += P;
-        a3 += P * P;
-      }
+__device__ __forceinline__ void cv_to_bf16(const cv_v4f (&x)[4], cv_v4s (&B)[4]) {
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) B[m][e] = bf16_bits(x[m][e]);
+}
+
+// Value pass of the row held by lanes (l % 16) + 16 q: the SiLU derivatives
+// of the three hidden layers, FP16, dd[layer][m][e] for unit cv_unit(m, e, l).
+__device__ __forceinline__ void cv_value(const CvLds &L, const float (&f)[CV_IN], cv_h4 (&dd)[3][4], int l) {
+  // layer 1: (f0..f4, 1) against W1 extended by b1
+  cv_v4s Bf = {0, 0, 0, 0};
+  if ((l >> 4) == 0) {
+    Bf[0] = bf16_bits(f[0]);
+    Bf[1] = bf16_bits(f[1]);
+    Bf[2] = bf16_bits(f[2]);
+    Bf[3] = bf16_bits(f[3]);
+  } else if ((l >> 4) == 1) {
+    Bf[0] = bf16_bits(f[4]);
+    Bf[1] = bf16_bits(1.0f);
+  }
+  cv_v4f acc[4];
+  {
+    const int z = cv_opaque_zero();
+    const short *base = L.w1 + (l & 15) * CV_P1 + 4 * (l >> 4) + z;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      acc[m] = cv_mfma(*reinterpret_cast<const cv_v4s *>(base + 16 * m * CV_P1), Bf, cv_v4f{0.f, 0.f, 0.f, 0.f});
+  }
+  cv_v4s Bh[4];
+#pragma unroll
+  for (int layer = 0; layer < 3; ++layer) {
+    if (layer > 0) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc[m] = cv_units4(L.bias[layer - 1], m, l);
+      cv_layer(layer == 1 ? L.w2 : L.w3, Bh, acc, l);
     }
-    if (lane < 32 && live) {
-      double *o = part + (wave * 64 + k) * 4;
-      o[0] = a0;
-      o[1] = a1;
-      o[2] = a2;
-      o[3] = a3;
-    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float y = acc[m][e];
+        const float sg = sigmoidf(y), h = y * sg;
+        dd[layer][m][e] = static_cast<_Float16>(sg + h * (1.0f - sg));
+        if (layer < 2) Bh[m][e] = bf16_bits(h);
+      }
   }
 }
 
-// sums[k][c] = sum over waves of part[w][k][c], in wave order
-__global__ void cv_reduce_kernel(const double *__restrict__ part, int n_waves, int nk, double *__restrict__ sums) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nk * 4) return;
-  const int k = i / 4, c = i % 4;
+// Forward-mode tangent along feature i (1..3) through the hidden layers:
+// the last hidden layer's tangent t (FP32, accumulator layout).
+__device__ __forceinline__ void cv_tangent(const CvLds &L, const cv_h4 (&dd)[3][4], int i, cv_v4f (&t)[4], int l) {
+  cv_v4s Bt[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const cv_v4f w = cv_units4(L.w1c[i - 1], m, l);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bt[m][e] = bf16_bits(w[e] * static_cast<float>(dd[0][m][e]));
+  }
+#pragma unroll
+  for (int layer = 1; layer < 3; ++layer) {
+    cv_v4f acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
+    cv_layer(layer == 1 ? L.w2 : L.w3, Bt, acc, l);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float v = acc[m][e] * static_cast<float>(dd[layer][m][e]);
+        if (layer == 1) Bt[m][e] = bf16_bits(v);
+        else t[m][e] = v;
+      }
+  }
+}
+
+// head product: out[m] (+)= H[16m.., :] x^T of the FP32 hidden-layer vector x
+template <bool ADD>
+__device__ __forceinline__ void cv_head(const short *__restrict__ H, const cv_v4f (&x)[4], cv_v4f (&out)[4], int l) {
+  cv_v4s B[4];
+  cv_to_bf16(x, B);
+  if (!ADD)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) out[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
+  cv_layer(H, B, out, l);
+}
+
+// Orders a wave's LDS accesses (DS instructions of one wave execute in
+// issue order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void cv_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Terminal data of the 16 paths of a wave's batch, for the per-strike stage.
+struct CvPathLds {
+  double ST[16], Sm[16], Sm0[16], hw[4][16];  // hw: cubic Hermite weights
+  int cell[16];                               // table cell, -1: off the grid
+  int valid[16];
+};
+
+// ---------------------------------------------------------------------------
+// evaluation. Storage index t = i S + s holds the i-th path of stream s
+// (lanes of a wave read consecutive streams); each wave takes 16 storage
+// indices at a time. Per control point: the value pass, then each tangent
+// through the hidden layers, accumulated (weighted by the block increments)
+// into per-path FP32 hidden-layer sums TA (stock noise), TV (variance
+// noise) and TJ (terminal jump). After the last point the heads turn them
+// into the strikes' martingales: MODE 0/1 Z = c_A M_A + c_V M_V + c_J M_J
+// (heads pre-scaled by the coefficients), MODE 2 the three regressors. The
+// outputs pass through LDS to the layout lane = strike, and each lane
+// accumulates its strike's sums over the wave's paths in FP64, in path
+// order: MODE 0 the estimator Y = P - Z - c_D D; 1 the same with the plain
+// delta and gamma of the path (spot scaled by (S0 +- h)/S0,
+// European.hpp:40-74); 2 the products of pairs of (1, P, M_A, M_V, M_J, D)
+// for the fit. part[(wave * 64 + k) * NS + s] receives the wave's sums.
+// ---------------------------------------------------------------------------
+template <int NF, int MODE>
+__global__ void __launch_bounds__(BLOCK) cv_eval_kernel(const float *__restrict__ prm, CvNorm nm,
+                                                        const CvSet *__restrict__ sets,
+                                                        const CvEvalTask *__restrict__ tasks, int n_tasks) {
+  constexpr int NS = MODE == 2 ? CV_NS_FIT : (MODE == 1 ? CV_NS_PRICE : 4);
+  constexpr int NZ = MODE == 2 ? 3 : 1;
+  constexpr int WAVES = BLOCK / 64, ZP = CV_OUT + 4;
+  __shared__ CvLds L;
+  __shared__ double sK[CV_OUT];
+  __shared__ float sD[CV_OUT];
+  __shared__ alignas(16) float sZ[WAVES][NZ * 16 * ZP];
+  __shared__ CvPathLds sP[WAVES];
+  const int ti = cv_task_of(tasks, n_tasks);
+  const CvSet &set = sets[__builtin_amdgcn_readfirstlane(tasks[ti].set)];
+  const double *__restrict__ coef = tasks[ti].coef;
+  double *__restrict__ part = tasks[ti].part;
+  cv_load_lds(L, prm, MODE == 2 ? nullptr : coef, nm.xs, set.nk);
+  for (int i = threadIdx.x; i < CV_OUT; i += blockDim.x) {
+    const bool live = i < set.nk;
+    sK[i] = live ? set.strikes[i] : 0.0;
+    sD[i] = (live && coef && MODE != 2) ? static_cast<float>(coef[3 * CV_OUT + i]) : 0.0f;
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 63, wv = threadIdx.x / 64;
+  const int r = l & 15;
+  const uint64_t n_waves = tasks[ti].n_waves;
+  const uint64_t wave = static_cast<uint64_t>(blockIdx.x - tasks[ti].first_block) * WAVES + wv;
+  const uint64_t Ms = set.Ms;
+  const float c2x0 = set.c2x[0], c2x1 = set.c2x[1];
+  // weights of the tangent sums: MODE 2 unscaled regressors (head W4, so
+  // du/dx = xs W4 t, du/dv_f = vs_f W4 t_f); MODE 0/1 heads carry xs
+  const float sx = MODE == 2 ? nm.xs : 1.0f;
+  const float rv0 = MODE == 2 ? nm.vs0 : nm.vs0 / nm.xs, rv1 = MODE == 2 ? nm.vs1 : nm.vs1 / nm.xs;
+  const float fT = static_cast<float>(set.T) * nm.ts;
+  const int k = l;  // strike of the per-strike stage
+  const bool k_live = k < set.nk;
+  const double K = sK[k], cD = sD[k];
+  float *zb = sZ[wv];
+  CvPathLds &pd = sP[wv];
+  double acc[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) acc[i] = 0.0;
+
+  for (uint64_t p0 = wave * 16; p0 < Ms; p0 += n_waves * 16) {
+    const uint64_t p = p0 + r;
+    const bool valid = p < Ms && cv_valid(set, p);
+    const uint64_t pp = p < Ms ? p : Ms - 1;
+    cv_v4f TA[4], TV[4], TJ[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      TA[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
+      TV[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
+      TJ[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    const double ST = set.ST[pp];
+    const double Sm = set.Sm ? set.Sm[pp] : ST;
+    const float ej = set.Sm ? static_cast<float>(ST / Sm - 1.0) : 0.0f;
+    for (int j = 0; j < set.nj; ++j) {
+      const float *rec = set.rec + static_cast<size_t>(j) * CV_REC * Ms + pp;
+      const float x = rec[0], v1 = rec[Ms], v2 = rec[2 * Ms], A = rec[3 * Ms], V1 = rec[4 * Ms],
+                  V2 = NF > 1 ? rec[5 * Ms] : 0.0f;
+      const float f[CV_IN] = {set.t_of[j] * nm.ts, (x - nm.logS0) * nm.xs, v1 * nm.vs0, v2 * nm.vs1, fT};
+      const float dj = static_cast<float>(set.disc_of[j]);
+      const float Cv = c2x0 * V1 + (NF > 1 ? c2x1 * V2 : 0.0f);
+      const bool last = j == set.nj - 1;
+      cv_h4 dd[3][4];
+      cv_value(L, f, dd, l);
+      cv_v4f t[4];
+      // log-spot tangent: stock noise, the variance noise's stock part and
+      // (last point) the terminal jump
+      cv_tangent(L, dd, 1, t, l);
+      const float wa = dj * sx * A, wv1 = dj * sx * Cv, wj = dj * sx * ej;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          TA[m][e] += wa * t[m][e];
+          TV[m][e] += wv1 * t[m][e];
+          if (last) TJ[m][e] = wj * t[m][e];
+        }
+      // variance tangents (their feature scale relative to the log-spot's
+      // folded into the weight)
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) {
+        cv_tangent(L, dd, 2 + fi, t, l);
+        const float w = dj * (fi == 0 ? rv0 * V1 : rv1 * V2);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) TV[m][e] += w * t[m][e];
+      }
+    }
+    // terminal data of the batch's paths
+    if (l < 16) {
+      pd.ST[r] = ST;
+      pd.Sm[r] = Sm;
+      pd.valid[r] = valid ? 1 : 0;
+      if (set.rb_on) {
+        pd.Sm0[r] = Sm * exp(set.rb_m0);
+        const double tp = (log(Sm) - set.rb_y0) * set.rb_inv_h;
+        const bool on = tp >= 0.0 && tp < static_cast<double>(CV_RB_PTS - 1);
+        const int i = on ? static_cast<int>(tp) : -1;
+        const double u = on ? tp - static_cast<double>(i) : 0.0, u2 = u * u, u3 = u2 * u;
+        pd.cell[r] = i;
+        pd.hw[0][r] = 2.0 * u3 - 3.0 * u2 + 1.0;
+        pd.hw[1][r] = u3 - 2.0 * u2 + u;
+        pd.hw[2][r] = -2.0 * u3 + 3.0 * u2;
+        pd.hw[3][r] = u3 - u2;
+      }
+    }
+    // heads, then through LDS to lane = strike
+#pragma unroll
+    for (int q = 0; q < NZ; ++q) {
+      cv_v4f o[4];
+      if constexpr (MODE == 2) {
+        cv_head<false>(L.hd[0], q == 0 ? TA : (q == 1 ? TV : TJ), o, l);
+      } else {
+        cv_head<false>(L.hd[0], TA, o, l);
+        cv_head<true>(L.hd[1], TV, o, l);
+        if (set.Sm) cv_head<true>(L.hd[2], TJ, o, l);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        *reinterpret_cast<cv_v4f *>(zb + (q * 16 + r) * ZP + 16 * m + 4 * (l >> 4)) = o[m];
+    }
+    cv_wave_sync();
+    if (k_live) {
+      for (int i = 0; i < 16; ++i) {
+        if (!pd.valid[i]) continue;
+        const double STi = pd.ST[i];
+        const double P = cv_payoff(set, STi, K);
+        double D = 0.0;
+        if (set.rb_on) {
+          const int cell = pd.cell[i];
+          double g;
+          if (cell >= 0) {
+            const double *a = set.rb + 2 * (static_cast<size_t>(cell) * set.nk + k), *b = a + 2 * set.nk;
+            g = pd.hw[0][i] * a[0] + pd.hw[1][i] * a[1] + pd.hw[2][i] * b[0] + pd.hw[3][i] * b[1];
+          } else {
+            double dg;
+            rb_sum(set, pd.Sm[i], K, g, dg);
+          }
+          D = P - (set.rb_p0 * cv_payoff(set, pd.Sm0[i], K) + g);
+        }
+        if constexpr (MODE == 2) {
+          const double v[6] = {1.0, P, zb[i * ZP + k], zb[(16 + i) * ZP + k], zb[(32 + i) * ZP + k], D};
+          int q = 0;
+#pragma unroll
+          for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int w = u; w < 6; ++w) acc[q++] += v[u] * v[w];
+        } else {
+          const double Y = P - static_cast<double>(zb[i * ZP + k]) - cD * D;
+          acc[0] += Y;
+          acc[1] += Y * Y;
+          acc[2] += P;
+          acc[3] += P * P;
+          if constexpr (MODE == 1) {
+            const double pu = cv_payoff(set, STi * set.mult_up, K), pdn = cv_payoff(set, STi * set.mult_dn, K);
+            const double d = (pu - pdn) * set.inv2h, gm = (pu - 2.0 * P + pdn) * set.invh2;
+            acc[4] += d;
+            acc[5] += d * d;
+            acc[6] += gm;
+            acc[7] += gm * gm;
+          }
+        }
+      }
+    }
+    cv_wave_sync();
+  }
+  if (k_live)
+    for (int i = 0; i < NS; ++i) part[(wave * CV_OUT + k) * NS + i] = acc[i];
+}
+
+// sums[k][s] = sum over waves of part[w][k][s] of task blockIdx.y: one
+// block per (k, s), strided partial sums then a tree (fixed order)
+__global__ void __launch_bounds__(256) cv_reduce_kernel(const CvSet *__restrict__ sets,
+                                                        const CvEvalTask *__restrict__ tasks, int ns) {
+  __shared__ double red[256];
+  const CvEvalTask &T = tasks[blockIdx.y];
+  if (static_cast<int>(blockIdx.x) >= sets[T.set].nk * ns) return;
+  const double *__restrict__ part = T.part;
+  const int n_waves = static_cast<int>(T.n_waves);
+  const int k = blockIdx.x / ns, c = blockIdx.x % ns;
   double s = 0.0;
-  for (int w = 0; w < n_waves; ++w) s += part[(static_cast<size_t>(w) * 64 + k) * 4 + c];
-  sums[i] = s;
+  for (int w = threadIdx.x; w < n_waves; w += 256) s += part[(static_cast<size_t>(w) * CV_OUT + k) * ns + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) T.sums[blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -368,76 +754,63 @@ __device__ __forceinline__ uint64_t cv_hash(uint64_t x) {
 }
 
 struct CvTrainLds {
-  float w2[CV_H][CV_H + 1], w3[CV_H][CV_H + 1];
-  float w1[CV_H * CV_IN], b1[CV_H], b2[CV_H], b3[CV_H], w4[CV_H];
   float f[CV_TROWS][CV_IN];
   float h1[CV_TROWS][CV_H + 1], h2[CV_TROWS][CV_H + 1], h3[CV_TROWS][CV_H + 1];
   float d1[CV_TROWS][CV_H + 1], d2[CV_TROWS][CV_H + 1], d3[CV_TROWS][CV_H + 1];
-  float ypart[8][CV_TROWS];
-  float dy[CV_TROWS], target[CV_TROWS], loss[CV_TROWS];
+  float dy[CV_TROWS][CV_OUT + 1];
+  float loss[CV_TROWS][8];
+  int set[CV_TROWS];
+  unsigned long long path[CV_TROWS];
+  int cp[CV_TROWS];
 };
 
-// Rows: a random (set, path, control point, strike) per row of each
-// minibatch, from a counter hash of (seed, step, row). Feature and target
-// construction matches cv_eval_kernel. Writes the block's parameter
-// gradient of mean (u - target)^2 over the whole minibatch, and its loss.
-__device__ __forceinline__ void cv_row(const CvSet *__restrict__ sets, int n_sets, CvNorm nm, uint64_t h,
-                                       float (&f)[CV_IN], float &target) {
-  const CvSet &S = sets[h % static_cast<uint64_t>(n_sets)];
-  const uint64_t h2 = cv_hash(h);
-  const uint64_t p = h2 % S.M;
-  const int j = static_cast<int>((h2 >> 40) % static_cast<uint64_t>(S.nj));
-  const int k = static_cast<int>(cv_hash(h2) % static_cast<uint64_t>(S.nk));
-  const double K = S.strikes[k];
-  const float lk = static_cast<float>(log(K));
-  f[0] = S.t_of[j] * nm.ts;
-  f[1] = (S.rec[(j * CV_REC + 0) * S.M + p] - lk) * nm.xs;
-  f[2] = S.rec[(j * CV_REC + 1) * S.M + p] * nm.vs0;
-  f[3] = S.rec[(j * CV_REC + 2) * S.M + p] * nm.vs1;
-  f[4] = static_cast<float>(S.T) * nm.ts;
-  f[5] = (lk - nm.logS0) * nm.xs;
-  target = static_cast<float>(cv_payoff(S.payoff, S.call, S.ST[p], K) / S.disc_of[j] * S.inv_S0);
-}
-
+// Rows: a random (set, path, control point) per row of each minibatch, from
+// a counter hash of (seed, step, row). Targets: the normalised discounted
+// (terminal-jump-expected) payoff of each strike, masked beyond the set's
+// strike count. Writes the block's parameter gradient of the mean squared
+// error over the minibatch's rows and live strikes, and its loss. Weights
+// are read from global memory (L2 resident).
 __global__ void __launch_bounds__(BLOCK) cv_train_kernel(const float *__restrict__ prm, const CvSet *__restrict__ sets,
                                                          int n_sets, CvNorm nm, uint64_t seed,
-                                                         const int *__restrict__ step, int batch,
+                                                         const int *__restrict__ step, int batch, int nk_all,
                                                          float *__restrict__ gpart, float *__restrict__ lpart,
                                                          const float *__restrict__ fixed_f,
                                                          const float *__restrict__ fixed_t) {
   __shared__ CvTrainLds L;
   const int tid = threadIdx.x;
-  const int r = tid % CV_TROWS, q8 = tid / CV_TROWS;  // row, slice of 8 outputs
-  for (int i = tid; i < CV_H * CV_H; i += BLOCK) {
-    L.w2[i / CV_H][i % CV_H] = prm[CV_W2 + i];
-    L.w3[i / CV_H][i % CV_H] = prm[CV_W3 + i];
-  }
-  for (int i = tid; i < CV_H * CV_IN; i += BLOCK) L.w1[i] = prm[CV_W1 + i];
-  for (int i = tid; i < CV_H; i += BLOCK) {
-    L.b1[i] = prm[CV_B1 + i];
-    L.b2[i] = prm[CV_B2 + i];
-    L.b3[i] = prm[CV_B3 + i];
-    L.w4[i] = prm[CV_W4 + i];
-  }
+  const int r = tid % CV_TROWS, q8 = tid / CV_TROWS;  // row, slice of 8 units
   const int row = blockIdx.x * CV_TROWS + r;
   if (q8 == 0) {
-    float f[CV_IN], tg;
-    if (fixed_f) {  // caller-given rows (gradient checks)
+    float f[CV_IN];
+    if (fixed_f) {
       for (int i = 0; i < CV_IN; ++i) f[i] = fixed_f[row * CV_IN + i];
-      tg = fixed_t[row];
+      L.set[r] = -1;
     } else {
-      cv_row(sets, n_sets, nm, cv_hash(seed ^ cv_hash(static_cast<uint64_t>(*step) * 0x100000001ull + row)), f, tg);
+      const uint64_t h = cv_hash(seed ^ cv_hash(static_cast<uint64_t>(*step) * 0x100000001ull + row));
+      const int si = static_cast<int>(h % static_cast<uint64_t>(n_sets));
+      const CvSet &S = sets[si];
+      const uint64_t h2 = cv_hash(h);
+      uint64_t p = h2 % S.Ms;
+      if (!cv_valid(S, p)) p -= S.S;  // the stream's previous path
+      const int j = static_cast<int>((h2 >> 40) % static_cast<uint64_t>(S.nj));
+      f[0] = S.t_of[j] * nm.ts;
+      f[1] = (S.rec[(static_cast<size_t>(j) * CV_REC + 0) * S.Ms + p] - nm.logS0) * nm.xs;
+      f[2] = S.rec[(static_cast<size_t>(j) * CV_REC + 1) * S.Ms + p] * nm.vs0;
+      f[3] = S.rec[(static_cast<size_t>(j) * CV_REC + 2) * S.Ms + p] * nm.vs1;
+      f[4] = static_cast<float>(S.T) * nm.ts;
+      L.set[r] = si;
+      L.path[r] = p;
+      L.cp[r] = j;
     }
     for (int i = 0; i < CV_IN; ++i) L.f[r][i] = f[i];
-    L.target[r] = tg;
   }
   __syncthreads();
-  const float inv_b = 1.0f / static_cast<float>(batch);
+  const float inv_n = 1.0f / (static_cast<float>(batch) * static_cast<float>(nk_all));
   // forward
   for (int a = 0; a < 8; ++a) {
     const int n = q8 * 8 + a;
-    float z = L.b1[n];
-    for (int i = 0; i < CV_IN; ++i) z += L.w1[n * CV_IN + i] * L.f[r][i];
+    float z = prm[CV_B1 + n];
+    for (int i = 0; i < CV_IN; ++i) z += prm[CV_W1 + n * CV_IN + i] * L.f[r][i];
     const float sg = sigmoidf(z), h = z * sg;
     L.h1[r][n] = h;
     L.d1[r][n] = sg + h * (1.0f - sg);
@@ -445,50 +818,66 @@ __global__ void __launch_bounds__(BLOCK) cv_train_kernel(const float *__restrict
   __syncthreads();
   for (int a = 0; a < 8; ++a) {
     const int n = q8 * 8 + a;
-    float z = L.b2[n];
-    for (int k = 0; k < CV_H; ++k) z += L.w2[n][k] * L.h1[r][k];
+    float z = prm[CV_B2 + n];
+    for (int k = 0; k < CV_H; ++k) z += prm[CV_W2 + n * CV_H + k] * L.h1[r][k];
     const float sg = sigmoidf(z), h = z * sg;
     L.h2[r][n] = h;
     L.d2[r][n] = sg + h * (1.0f - sg);
   }
   __syncthreads();
-  float yp = 0.f;
   for (int a = 0; a < 8; ++a) {
     const int n = q8 * 8 + a;
-    float z = L.b3[n];
-    for (int k = 0; k < CV_H; ++k) z += L.w3[n][k] * L.h2[r][k];
+    float z = prm[CV_B3 + n];
+    for (int k = 0; k < CV_H; ++k) z += prm[CV_W3 + n * CV_H + k] * L.h2[r][k];
     const float sg = sigmoidf(z), h = z * sg;
     L.h3[r][n] = h;
     L.d3[r][n] = sg + h * (1.0f - sg);
-    yp += L.w4[n] * h;
   }
-  L.ypart[q8][r] = yp;
   __syncthreads();
-  if (q8 == 0) {
-    float y = prm[CV_B4];
-    for (int i = 0; i < 8; ++i) y += L.ypart[i][r];
-    const float e = y - L.target[r];
-    L.loss[r] = e * e;
-    L.dy[r] = 2.0f * e * inv_b;
+  // head, targets and output errors
+  float lsum = 0.0f;
+  for (int a = 0; a < 8; ++a) {
+    const int k = q8 * 8 + a;
+    float y = prm[CV_B4 + k];
+    for (int n = 0; n < CV_H; ++n) y += prm[CV_W4 + k * CV_H + n] * L.h3[r][n];
+    float tg = 0.0f;
+    bool live;
+    if (fixed_f) {
+      live = k < nk_all;
+      tg = live ? fixed_t[row * CV_OUT + k] : 0.0f;
+    } else {
+      const CvSet &S = sets[L.set[r]];
+      live = k < S.nk;
+      if (live) {
+        const double pv = cv_target(S, L.path[r], k) / S.disc_of[L.cp[r]];
+        tg = static_cast<float>((pv - static_cast<double>(S.mu[k])) * static_cast<double>(S.inv_sd[k]));
+      }
+    }
+    const float e = live ? y - tg : 0.0f;
+    lsum += e * e;
+    L.dy[r][k] = 2.0f * e * inv_n;
   }
+  L.loss[r][q8] = lsum;
   __syncthreads();
   // backward: deltas overwrite the activation derivatives
   for (int a = 0; a < 8; ++a) {
     const int n = q8 * 8 + a;
-    L.d3[r][n] *= L.dy[r] * L.w4[n];
+    float s = 0.f;
+    for (int k = 0; k < CV_OUT; ++k) s += prm[CV_W4 + k * CV_H + n] * L.dy[r][k];
+    L.d3[r][n] *= s;
   }
   __syncthreads();
   for (int a = 0; a < 8; ++a) {
     const int k = q8 * 8 + a;
     float s = 0.f;
-    for (int n = 0; n < CV_H; ++n) s += L.w3[n][k] * L.d3[r][n];
+    for (int n = 0; n < CV_H; ++n) s += prm[CV_W3 + n * CV_H + k] * L.d3[r][n];
     L.d2[r][k] *= s;
   }
   __syncthreads();
   for (int a = 0; a < 8; ++a) {
     const int k = q8 * 8 + a;
     float s = 0.f;
-    for (int n = 0; n < CV_H; ++n) s += L.w2[n][k] * L.d2[r][n];
+    for (int n = 0; n < CV_H; ++n) s += prm[CV_W2 + n * CV_H + k] * L.d2[r][n];
     L.d1[r][k] *= s;
   }
   __syncthreads();
@@ -496,13 +885,15 @@ __global__ void __launch_bounds__(BLOCK) cv_train_kernel(const float *__restrict
   float *g = gpart + static_cast<size_t>(blockIdx.x) * CV_NPARAM;
   for (int e = tid; e < CV_H * CV_H; e += BLOCK) {
     const int n = e / CV_H, k = e % CV_H;
-    float s2 = 0.f, s3 = 0.f;
+    float s2 = 0.f, s3 = 0.f, s4 = 0.f;
     for (int rr = 0; rr < CV_TROWS; ++rr) {
       s2 += L.d2[rr][n] * L.h1[rr][k];
       s3 += L.d3[rr][n] * L.h2[rr][k];
+      s4 += L.dy[rr][n] * L.h3[rr][k];
     }
     g[CV_W2 + e] = s2;
     g[CV_W3 + e] = s3;
+    g[CV_W4 + e] = s4;
   }
   for (int e = tid; e < CV_H * CV_IN; e += BLOCK) {
     const int n = e / CV_IN, i = e % CV_IN;
@@ -516,20 +907,17 @@ __global__ void __launch_bounds__(BLOCK) cv_train_kernel(const float *__restrict
       s1 += L.d1[rr][n];
       s2 += L.d2[rr][n];
       s3 += L.d3[rr][n];
-      s4 += L.dy[rr] * L.h3[rr][n];
+      s4 += L.dy[rr][n];
     }
     g[CV_B1 + n] = s1;
     g[CV_B2 + n] = s2;
     g[CV_B3 + n] = s3;
-    g[CV_W4 + n] = s4;
+    g[CV_B4 + n] = s4;
   }
   if (tid == 0) {
-    float s = 0.f, l = 0.f;
-    for (int rr = 0; rr < CV_TROWS; ++rr) {
-      s += L.dy[rr];
-      l += L.loss[rr];
-    }
-    g[CV_B4] = s;
+    float l = 0.f;
+    for (int rr = 0; rr < CV_TROWS; ++rr)
+      for (int q = 0; q < 8; ++q) l += L.loss[rr][q];
     lpart[blockIdx.x] = l;
   }
 }
@@ -538,7 +926,7 @@ __global__ void __launch_bounds__(BLOCK) cv_train_kernel(const float *__restrict
 // v[NPARAM]; step counter read here, advanced by cv_step_kernel.
 __global__ void cv_adam_kernel(float *__restrict__ prm, float *__restrict__ m, float *__restrict__ v,
                                const float *__restrict__ gpart, int n_blocks, const int *__restrict__ step, float lr,
-                               const float *__restrict__ lpart, float *__restrict__ loss_hist, int batch) {
+                               const float *__restrict__ lpart, float *__restrict__ loss_hist, float loss_scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = *step + 1;
   if (i < CV_NPARAM) {
@@ -556,7 +944,7 @@ __global__ void cv_adam_kernel(float *__restrict__ prm, float *__restrict__ m, f
   if (i == 0) {
     float l = 0.f;
     for (int b = 0; b < n_blocks; ++b) l += lpart[b];
-    loss_hist[t - 1] = l / static_cast<float>(batch);
+    loss_hist[t - 1] = l * loss_scale;
   }
 }
 
@@ -564,25 +952,28 @@ __global__ void cv_step_kernel(int *step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
-// debug: BF16 MFMA input gradient of n rows of features (n multiple of 32
-// handled per wave; rows beyond n are padded with zeros)
+// debug: BF16 MFMA tangents of n rows of features (rows beyond n padded
+// with zeros): out[(row * 64 + k) * 3 + t] = d out_k / d f_{1+t}
 __global__ void __launch_bounds__(BLOCK) cv_grad_debug_kernel(const float *__restrict__ prm,
                                                               const float *__restrict__ feats, int n,
                                                               float *__restrict__ out) {
   __shared__ CvLds L;
-  cv_load_lds(L, prm);
+  cv_load_lds(L, prm, nullptr, 1.0f, CV_OUT);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
+  const int l = threadIdx.x & 63;
   const int wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64;
-  const int row = wave * 32 + (lane & 31);
+  const int row = wave * 16 + (l & 15);
   float f[CV_IN];
   for (int i = 0; i < CV_IN; ++i) f[i] = row < n ? feats[row * CV_IN + i] : 0.f;
-  float g1, g2, g3;
-  cv_grad(L, f, g1, g2, g3, lane);
-  if (lane < 32 && row < n) {
-    out[row * 3 + 0] = g1;
-    out[row * 3 + 1] = g2;
-    out[row * 3 + 2] = g3;
+  cv_h4 dd[3][4];
+  cv_value(L, f, dd, l);
+  for (int t = 0; t < 3; ++t) {
+    cv_v4f tg[4], g[4];
+    cv_tangent(L, dd, 1 + t, tg, l);
+    cv_head<false>(L.hd[0], tg, g, l);
+    if (row < n)
+      for (int m = 0; m < 4; ++m)
+        for (int e = 0; e < 4; ++e) out[(static_cast<size_t>(row) * CV_OUT + cv_unit(m, e, l)) * 3 + t] = g[m][e];
   }
 }
 

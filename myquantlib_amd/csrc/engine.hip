@@ -304,6 +304,10 @@ __global__ void debug_chi2_kernel(const double *__restrict__ dof, const double *
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+namespace {
+struct CvRecord;
+}
+
 struct myql_ctx {
   int device = 0;
   int n_cu = 256;
@@ -334,6 +338,10 @@ struct myql_ctx {
   void *scratch = nullptr;
   bool noalloc = false;
   size_t scratch_off = 0;
+  // learned control variate: recordings (their buffers grow and are reused
+  // by later calls) and launch arrays
+  std::vector<std::unique_ptr<CvRecord>> cv_recs;
+  DevBuf cv_jobs, cv_sets, cv_tasks, cv_part, cv_sums, cv_coef;
 };
 
 // Device memory of myql_mc_calculate_noalloc: stream states, job blocks,
@@ -402,8 +410,9 @@ const uint64_t *stream_states(myql_ctx *c, uint64_t seed, int layout, uint64_t S
 
 struct NciInfo {
   const void *tab;
-  const void *tail;  // upper-tail rows
-  const double *lg;  // lgamma(d/2 + N), N < NCI_LG
+  const void *tail;    // upper-tail rows
+  const double *lg;    // lgamma(d/2 + N), N < NCI_LG
+  const double *mean;  // per row, the mean of its quantile map (default semantics)
   int rows, grid;
   double dx, inv_dx;
 };
@@ -433,7 +442,8 @@ int nci_nmax(const myql_heston_params &p, double dt, bool reference_rows) {
 
 // One buffer per (d, rows, precision, n_fb): main rows [rows x grid x 2],
 // tail rows [rows x NCI_TAIL x 2] (R precision), then NCI_LG doubles of
-// lgamma(d/2 + N).
+// lgamma(d/2 + N), then the rows' means (the learned control variate's
+// zero-mean variance increments).
 NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   const int rows = nmax + 1;
   uint64_t dbits;
@@ -445,7 +455,9 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   const size_t tail_b = static_cast<size_t>(rows) * NCI_TAIL * 2 * esz;
   auto info = [&](const DevBuf &b) {
     const char *p = static_cast<const char *>(b.p);
-    return NciInfo{p, p + main_b, reinterpret_cast<const double *>(p + main_b + tail_b), rows, NCI_GRID, dx, inv_dx};
+    return NciInfo{p, p + main_b, reinterpret_cast<const double *>(p + main_b + tail_b),
+                   reinterpret_cast<const double *>(p + main_b + tail_b + NCI_LG * sizeof(double)), rows, NCI_GRID,
+                   dx, inv_dx};
   };
   const auto key = std::make_tuple(dbits, rows, prec, n_fb);
   auto it = c->nci.find(key);
@@ -453,7 +465,7 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
   const int64_t n = static_cast<int64_t>(rows) * NCI_GRID;
   c->tmp.reserve(n * sizeof(double));
   auto buf = std::make_unique<DevBuf>();
-  buf->reserve(main_b + tail_b + NCI_LG * sizeof(double));
+  buf->reserve(main_b + tail_b + (NCI_LG + rows) * sizeof(double));
   NciInfo ni = info(*buf);
   // lgamma(d/2 + N) for the quantile's incomplete-gamma prefactors
   std::vector<double> lgh(NCI_LG);
@@ -473,11 +485,15 @@ NciInfo nci_table(myql_ctx *c, double d, int nmax, int prec, int n_fb) {
                        NCI_GRID, dx, inv_dx, d, n_fb, d_lg, static_cast<float *>(const_cast<void *>(ni.tab)));
     hipLaunchKernelGGL(nci_tail_kernel<float>, dim3(nbt), dim3(tb), 0, c->stream, d, rows, NCI_TAIL, NCI_TAIL_W0,
                        hw, d_lg, static_cast<float *>(const_cast<void *>(ni.tail)));
+    HIPCHECK(launch_nci_rowmean<float>(static_cast<const float *>(ni.tab), rows, NCI_GRID, dx, d,
+                                       const_cast<double *>(ni.mean), c->stream));
   } else {
     hipLaunchKernelGGL(nci_pack_kernel<double>, dim3(nb), dim3(tb), 0, c->stream, c->tmp.as<double>(), rows,
                        NCI_GRID, dx, inv_dx, d, n_fb, d_lg, static_cast<double *>(const_cast<void *>(ni.tab)));
     hipLaunchKernelGGL(nci_tail_kernel<double>, dim3(nbt), dim3(tb), 0, c->stream, d, rows, NCI_TAIL, NCI_TAIL_W0,
                        hw, d_lg, static_cast<double *>(const_cast<void *>(ni.tail)));
+    HIPCHECK(launch_nci_rowmean<double>(static_cast<const double *>(ni.tab), rows, NCI_GRID, dx, d,
+                                        const_cast<double *>(ni.mean), c->stream));
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));  // the scratch rows are reused by the next table
@@ -510,6 +526,8 @@ void set_nci(Job<R> &J, int f, const NciInfo &ni, bool reference) {
   J.nci_dx = ni.dx;
   J.nci_inv_dx = ni.inv_dx;
   J.nci_tail[f] = ni.tail;
+  J.nci_mean[f] = ni.mean;
+  J.nci_rows[f] = ni.rows;
   J.nci_u_lo = reference ? 0.0 : ni.dx;
   J.nci_rows_acc[f] = reference ? ni.rows - 1 : -1;
   J.nci_u_acc = NCI_U_ACC * ni.dx;
@@ -1432,6 +1450,7 @@ void mlmc_validate(const myql_model *m, const int32_t *schemes, const myql_instr
 // ---------------------------------------------------------------------------
 void cv_validate(const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg) {
   if (!jobs || n < 1 || !cfg) throw Fail(MYQL_E_INVALID_ARG, "control variate: no jobs or config");
+  if (n > CV_MAXJOBS) throw Fail(MYQL_E_UNSUPPORTED, "control variate: at most 16 jobs per call");
   if (cfg->points < 1 || cfg->points > MYQL_CV_MAX_POINTS) throw Fail(MYQL_E_INVALID_ARG, "points must be in 1..16");
   const myql_mc_job &a = jobs[0];
   for (int64_t i = 0; i < n; ++i) {
@@ -1445,8 +1464,16 @@ void cv_validate(const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg) 
     if (ib.kind != MYQL_INSTR_EUROPEAN) throw Fail(MYQL_E_UNSUPPORTED, "control variate: European options only");
     if (ib.payoff != ia.payoff || ib.option_type != ia.option_type)
       throw Fail(MYQL_E_INVALID_ARG, "control variate: all jobs must share payoff and option type");
-    if (ib.n_strikes > 64) throw Fail(MYQL_E_UNSUPPORTED, "control variate: at most 64 strikes per job");
+    if (ib.n_strikes > CV_OUT) throw Fail(MYQL_E_UNSUPPORTED, "control variate: at most 64 strikes per job");
+    if (ib.n_strikes != ia.n_strikes ||
+        std::memcmp(ib.strikes, ia.strikes, static_cast<size_t>(ib.n_strikes) * sizeof(double)) != 0)
+      throw Fail(MYQL_E_INVALID_ARG, "control variate: all jobs must share the strike strip (one head output per strike)");
     if (b.S0 != a.S0 || b.r != a.r || b.q != a.q) throw Fail(MYQL_E_INVALID_ARG, "control variate: jobs must share S0, r, q");
+    if (b.greek_mode == MYQL_GREEK_FULL)
+      throw Fail(MYQL_E_UNSUPPORTED, "control variate: GreekMode::Full is not supported (None or Essential)");
+    if (b.greek_mode != a.greek_mode) throw Fail(MYQL_E_INVALID_ARG, "control variate: jobs must share the Greek mode");
+    if (b.config->flags & MYQL_FLAG_REFERENCE_NCI_ROWS)
+      throw Fail(MYQL_E_UNSUPPORTED, "control variate: the reference NCI table semantics are not supported");
   }
 }
 
@@ -1468,29 +1495,33 @@ CvNorm cv_norm(const myql_mc_job *jobs, int64_t n) {
 // Owner of one recorded option set.
 struct CvRecord {
   CvSet set{};
-  DevBuf rec, st, dj, strikes, job;
+  std::vector<unsigned char> job;  // host copy of the Job<R> the paths run
+  DevBuf rec, st, sm, strikes, rb, mu, isd;
 };
 
-// Record `paths` base paths of job `jb` (seed, stream layout and precision
-// as given) at `points` control points.
+// Set up the recording of the base paths of job `jb` under config `cfg`
+// (paths, seed, stream layout, precision) at `points` control points: the
+// job, the buffers and the set's terminal-jump expectation tables. The paths
+// themselves are recorded by cv_record_paths, batched over sets.
 template <typename R>
 void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, int points, CvRecord &out) {
   const myql_model &m = *jb.model;
   const myql_instrument &in = *jb.instrument;
   const int nk = static_cast<int>(in.n_strikes);
-  JobSpec js{jb.model, jb.schemes, jb.instrument, MYQL_GREEK_NONE, &cfg, jb.S0, jb.r, jb.q, nullptr, 0, nk};
+  JobSpec js{jb.model, jb.schemes, jb.instrument, jb.greek_mode, &cfg, jb.S0, jb.r, jb.q, nullptr, 0, nk};
   Job<R> J{};
   std::vector<double> ks;
   build_job<R>(c, js, J, ks);
   J.strikes = nullptr;
   J.partial = nullptr;
-  out.job.reserve(sizeof(Job<R>));
-  HIPCHECK(hipMemcpyAsync(out.job.p, &J, sizeof(Job<R>), hipMemcpyHostToDevice, c->stream));
+  out.job.assign(reinterpret_cast<const unsigned char *>(&J), reinterpret_cast<const unsigned char *>(&J + 1));
   const uint64_t M = cfg.num_paths, steps = cfg.time_steps;
   const double T = in.maturity, dt = T / static_cast<double>(steps);
   CvSet &S = out.set;
   S = CvSet{};
   S.M = M;
+  S.S = J.S;
+  S.Ms = (M + J.S - 1) / J.S * J.S;
   const int nf = m.n_factors;
   S.nj = nf == 0 ? 1 : static_cast<int>(std::min<uint64_t>(points, steps));
   for (int j = 0; j <= S.nj; ++j) S.step_of[j] = static_cast<int>((static_cast<uint64_t>(j) * steps) / S.nj);
@@ -1499,38 +1530,179 @@ void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, in
     S.t_of[j] = static_cast<float>(tj);
     S.disc_of[j] = std::exp(jb.r * (T - tj));
   }
-  S.sqdt = static_cast<float>(nf == 0 ? std::sqrt(T) : std::sqrt(dt));
   S.nf = nf;
-  S.vol = static_cast<float>(m.vol);
-  for (int f = 0; f < 2; ++f) {
-    const bool on = f < nf;
-    S.rho[f] = on ? static_cast<float>(m.heston[f].rho) : 0.0f;
-    S.sig[f] = on && jb.schemes[f] == MYQL_SCHEME_NV ? static_cast<float>(m.heston[f].sigma) : 0.0f;
-    S.rperp[f] = on ? static_cast<float>(1.0 - m.heston[f].rho * m.heston[f].rho) : 0.0f;
+  for (int f = 0; f < nf; ++f) {
+    const myql_heston_params &p = m.heston[f];
+    S.c2x[f] = static_cast<float>((p.kappa * p.rho / p.sigma - 0.5) * dt * 0.5 + p.rho / p.sigma);
+    S.nci_mean[f] = jb.schemes[f] == MYQL_SCHEME_NCI ? J.nci_mean[f] : nullptr;
+    S.nci_rows[f] = jb.schemes[f] == MYQL_SCHEME_NCI ? J.nci_rows[f] : 0;
   }
-  // schemes without a recorded variance increment keep only the stock's
-  // independent part: the correlated stock term needs the variance noise too
-  for (int f = 0; f < nf; ++f)
-    if (jb.schemes[f] != MYQL_SCHEME_NV) S.rho[f] = 0.0f;
   out.strikes.reserve(nk * sizeof(double));
   HIPCHECK(hipMemcpyAsync(out.strikes.p, in.strikes, nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
   S.strikes = out.strikes.as<double>();
   S.nk = nk;
   S.payoff = in.payoff;
   S.call = in.option_type == MYQL_CALL ? 1 : 0;
+  S.mode = jb.greek_mode;
+  S.smooth = (jb.greek_mode != MYQL_GREEK_NONE && in.payoff != MYQL_PAYOFF_VANILLA) ? 1 : 0;
+  S.eps_scale = J.eps_scale;
+  S.mult_up = J.mult_up;
+  S.mult_dn = J.mult_dn;
+  S.inv2h = J.inv2h;
+  S.invh2 = J.invh2;
   S.T = T;
   S.df = std::exp(-jb.r * T);
-  S.inv_S0 = 1.0 / jb.S0;
-  out.rec.reserve(static_cast<size_t>(S.nj) * CV_REC * M * sizeof(float));
-  out.st.reserve(M * sizeof(double));
+  out.rec.reserve(static_cast<size_t>(S.nj) * CV_REC * S.Ms * sizeof(float));
+  out.st.reserve(S.Ms * sizeof(double));
   S.rec = out.rec.as<float>();
   S.ST = out.st.as<double>();
-  S.dJ = nullptr;
+  S.Sm = nullptr;
   if (m.jump_kind != MYQL_JUMP_NONE) {
-    out.dj.reserve(M * sizeof(float));
-    S.dJ = out.dj.as<float>();
+    out.sm.reserve(S.Ms * sizeof(double));
+    S.Sm = out.sm.as<double>();
   }
-  HIPCHECK(launch_cv_paths<R>(nf, out.job.as<Job<R>>(), S, J.S, c->stream));
+  // Merton terminal jump: E[P | S_T-] in closed form for vanilla and sharp
+  // digital payoffs: J = m0 + n mu + sqrt(n) delta Z, n ~ Poisson(lambda T),
+  // m0 the drift compensation (JumpPolicies.hpp:55-135)
+  S.rb_on = (m.jump_kind == MYQL_JUMP_MERTON && !S.smooth) ? 1 : 0;
+  if (S.rb_on) {
+    const double lt = m.jump[0] * T, mu = m.jump[1], dl = m.jump[2];
+    S.rb_m0 = -m.jump[0] * (std::exp(mu + 0.5 * dl * dl) - 1.0) * T;
+    S.rb_p0 = std::exp(-lt);
+    double pn = S.rb_p0, mass = S.rb_p0;
+    S.rb_n = 0;
+    for (int n = 1; n <= CV_RB_TERMS && mass < 1.0 - 1e-17; ++n) {
+      pn *= lt / n;
+      mass += pn;
+      S.rb_p[S.rb_n] = pn;
+      S.rb_m[S.rb_n] = S.rb_m0 + n * mu;
+      S.rb_s[S.rb_n] = std::sqrt(static_cast<double>(n)) * dl;
+      ++S.rb_n;
+    }
+    S.rb_y0 = std::log(jb.S0) - 3.0;
+    S.rb_h = 5.0 / (CV_RB_PTS - 1);
+    S.rb_inv_h = 1.0 / S.rb_h;
+    out.rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
+    S.rb = out.rb.as<double>();
+    HIPCHECK(launch_cv_rb_table(S, out.rb.as<double>(), c->stream));
+  }
+}
+
+// Device copies of sets (kept alive by the caller until it synchronises).
+const CvSet *cv_upload_sets(myql_ctx *c, const std::vector<CvSet> &sets, DevBuf &d_sets) {
+  d_sets.reserve(sets.size() * sizeof(CvSet));
+  HIPCHECK(hipMemcpyAsync(d_sets.p, sets.data(), sets.size() * sizeof(CvSet), hipMemcpyHostToDevice, c->stream));
+  return d_sets.as<CvSet>();
+}
+
+// Order of the sets for batched grids: by factor count (one grid each), the
+// most work per thread first, so the short sets fill the tail.
+std::vector<size_t> cv_batch_order(const std::vector<CvSet> &sets) {
+  std::vector<size_t> order(sets.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  auto work = [&](size_t i) {
+    const CvSet &t = sets[i];
+    return static_cast<double>(t.step_of[t.nj]) * static_cast<double>(t.Ms / t.S);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    if (sets[a].nf != sets[b].nf) return sets[a].nf < sets[b].nf;
+    return work(a) > work(b);
+  });
+  return order;
+}
+
+// Record the base paths of recs (all of precision R), one grid per factor
+// count.
+template <typename R>
+void cv_record_paths(myql_ctx *c, const std::vector<CvRecord *> &recs) {
+  DevBuf &d_jobs = c->cv_jobs, &d_sets = c->cv_sets, &d_tasks = c->cv_tasks;
+  const size_t n = recs.size();
+  if (n == 0) return;
+  std::vector<CvSet> sets(n);
+  std::vector<Job<R>> hj(n);
+  for (size_t i = 0; i < n; ++i) {
+    sets[i] = recs[i]->set;
+    std::memcpy(static_cast<void *>(&hj[i]), recs[i]->job.data(), sizeof(Job<R>));
+  }
+  const CvSet *ds = cv_upload_sets(c, sets, d_sets);
+  d_jobs.reserve(n * sizeof(Job<R>));
+  HIPCHECK(hipMemcpyAsync(d_jobs.p, hj.data(), n * sizeof(Job<R>), hipMemcpyHostToDevice, c->stream));
+  const std::vector<size_t> order = cv_batch_order(sets);
+  std::vector<CvPathTask> tasks(n);
+  std::vector<std::tuple<int, size_t, size_t, unsigned>> grids;  // nf, first task, tasks, blocks
+  for (size_t g0 = 0; g0 < n;) {
+    const int nf = sets[order[g0]].nf;
+    size_t g1 = g0;
+    unsigned blocks = 0;
+    for (; g1 < n && sets[order[g1]].nf == nf; ++g1) {
+      const size_t i = order[g1];
+      tasks[g1] = CvPathTask{static_cast<int>(i), static_cast<int>(i), blocks};
+      blocks += static_cast<unsigned>((sets[i].S + BLOCK - 1) / BLOCK);
+    }
+    grids.emplace_back(nf, g0, g1 - g0, blocks);
+    g0 = g1;
+  }
+  d_tasks.reserve(n * sizeof(CvPathTask));
+  HIPCHECK(hipMemcpyAsync(d_tasks.p, tasks.data(), n * sizeof(CvPathTask), hipMemcpyHostToDevice, c->stream));
+  for (const auto &g : grids)
+    HIPCHECK(launch_cv_paths<R>(std::get<0>(g), d_jobs.as<Job<R>>(), ds, d_tasks.as<CvPathTask>() + std::get<1>(g),
+                                static_cast<int>(std::get<2>(g)), std::get<3>(g), c->stream));
+}
+
+int cv_eval_blocks(myql_ctx *c, uint64_t M);
+
+// Evaluate sets (device copies ds) in `mode` (0 price, 1 price + plain
+// delta/gamma, 2 fitting sums), coefficients coefs[i] ([CV_NREG][64] on the
+// host; null in mode 2), one grid per factor count; returns the sums, nk x ns
+// per set at sums[i * 64 * ns], once the device has finished.
+void cv_eval_sets(myql_ctx *c, int mode, const DevBuf &prm, const CvNorm &nm, const std::vector<CvSet> &sets,
+                  const CvSet *ds, const std::vector<const double *> &coefs, std::vector<double> &sums) {
+  const size_t n = sets.size();
+  const int ns = mode == 2 ? CV_NS_FIT : (mode == 1 ? CV_NS_PRICE : 4);
+  const std::vector<size_t> order = cv_batch_order(sets);
+  std::vector<int> blocks(n);
+  size_t part_n = 0;
+  for (size_t i = 0; i < n; ++i) {
+    blocks[i] = cv_eval_blocks(c, sets[i].M);
+    part_n += static_cast<size_t>(blocks[i]) * (BLOCK / 64) * CV_OUT * ns;
+  }
+  DevBuf &part = c->cv_part, &d_sums = c->cv_sums, &d_coef = c->cv_coef, &d_tasks = c->cv_tasks;
+  part.reserve(part_n * sizeof(double));
+  d_sums.reserve(n * CV_OUT * ns * sizeof(double));
+  if (mode != 2) {
+    std::vector<double> h(n * CV_NREG * CV_OUT);
+    for (size_t i = 0; i < n; ++i) std::memcpy(&h[i * CV_NREG * CV_OUT], coefs[i], CV_NREG * CV_OUT * sizeof(double));
+    d_coef.reserve(h.size() * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(d_coef.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<CvEvalTask> tasks(n);
+  std::vector<std::tuple<int, size_t, size_t, unsigned, int>> grids;  // nf, first, count, blocks, max nk
+  size_t off = 0;
+  for (size_t g0 = 0; g0 < n;) {
+    const int nf = sets[order[g0]].nf;
+    size_t g1 = g0;
+    unsigned first = 0;
+    int max_nk = 0;
+    for (; g1 < n && sets[order[g1]].nf == nf; ++g1) {
+      const size_t i = order[g1];
+      tasks[g1] = CvEvalTask{static_cast<int>(i), first, static_cast<unsigned>(blocks[i] * (BLOCK / 64)),
+                             mode == 2 ? nullptr : d_coef.as<double>() + i * CV_NREG * CV_OUT, part.as<double>() + off,
+                             d_sums.as<double>() + i * CV_OUT * ns};
+      off += static_cast<size_t>(blocks[i]) * (BLOCK / 64) * CV_OUT * ns;
+      first += static_cast<unsigned>(blocks[i]);
+      max_nk = std::max(max_nk, sets[i].nk);
+    }
+    grids.emplace_back(nf, g0, g1 - g0, first, max_nk);
+    g0 = g1;
+  }
+  d_tasks.reserve(n * sizeof(CvEvalTask));
+  HIPCHECK(hipMemcpyAsync(d_tasks.p, tasks.data(), n * sizeof(CvEvalTask), hipMemcpyHostToDevice, c->stream));
+  for (const auto &g : grids)
+    HIPCHECK(launch_cv_eval(std::get<0>(g), mode, prm.as<float>(), nm, ds, d_tasks.as<CvEvalTask>() + std::get<1>(g),
+                            static_cast<int>(std::get<2>(g)), std::get<3>(g), std::get<4>(g), c->stream));
+  sums.assign(n * CV_OUT * ns, 0.0);
+  HIPCHECK(hipMemcpyAsync(sums.data(), d_sums.p, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
 void cv_init_params(uint32_t seed, std::vector<float> &p) {
@@ -1551,25 +1723,113 @@ void cv_init_params(uint32_t seed, std::vector<float> &p) {
   fill(CV_W1, CV_IN, CV_H, CV_H * CV_IN);
   fill(CV_W2, CV_H, CV_H, CV_H * CV_H);
   fill(CV_W3, CV_H, CV_H, CV_H * CV_H);
-  fill(CV_W4, CV_H, 1, CV_H);
+  fill(CV_W4, CV_H, CV_OUT, CV_OUT * CV_H);
+}
+
+int cv_eval_blocks(myql_ctx *c, uint64_t M) {
+  return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(c->n_cu) * 2,
+                                                                    (M + 127) / 128)));
+}
+
+// Per-strike least-squares coefficients of P on the regressors (M_A, M_V,
+// M_J, D) from the fitting sums of products of (1, P, R0..R3); regressors
+// without variance are left out.
+void cv_fit(const std::vector<double> &sums, int nk, uint64_t M, double *coef /* [CV_NREG][64] */) {
+  const double invM = 1.0 / static_cast<double>(M);
+  for (int k = 0; k < nk; ++k) {
+    double S[6][6];
+    int q = 0;
+    for (int u = 0; u < 6; ++u)
+      for (int w = u; w < 6; ++w) S[u][w] = S[w][u] = sums[static_cast<size_t>(k) * CV_NS_FIT + q++] * invM;
+    // covariances: index 1 = P, 2..5 = regressors (index 0 is the constant)
+    double C[4][4], b[4];
+    int act[4], na = 0;
+    for (int i = 0; i < 4; ++i) {
+      const double var = S[2 + i][2 + i] - S[0][2 + i] * S[0][2 + i];
+      if (var > 1e-300 && var > 1e-24 * (S[1][1] + 1e-300)) act[na++] = i;
+    }
+    for (int a = 0; a < na; ++a) {
+      const int i = act[a];
+      b[a] = S[1][2 + i] - S[0][1] * S[0][2 + i];
+      for (int e = 0; e < na; ++e) {
+        const int j = act[e];
+        C[a][e] = S[2 + i][2 + j] - S[0][2 + i] * S[0][2 + j];
+      }
+      C[a][a] *= 1.0 + 1e-12;
+    }
+    // Gaussian elimination with partial pivoting
+    double x[4] = {0, 0, 0, 0};
+    for (int col = 0; col < na; ++col) {
+      int piv = col;
+      for (int rw = col + 1; rw < na; ++rw)
+        if (std::fabs(C[rw][col]) > std::fabs(C[piv][col])) piv = rw;
+      for (int e = 0; e < na; ++e) std::swap(C[col][e], C[piv][e]);
+      std::swap(b[col], b[piv]);
+      if (!(std::fabs(C[col][col]) > 0.0)) continue;
+      for (int rw = col + 1; rw < na; ++rw) {
+        const double f = C[rw][col] / C[col][col];
+        for (int e = col; e < na; ++e) C[rw][e] -= f * C[col][e];
+        b[rw] -= f * b[col];
+      }
+    }
+    for (int rw = na - 1; rw >= 0; --rw) {
+      double s = b[rw];
+      for (int e = rw + 1; e < na; ++e) s -= C[rw][e] * x[e];
+      x[rw] = std::fabs(C[rw][rw]) > 0.0 ? s / C[rw][rw] : 0.0;
+    }
+    for (int i = 0; i < CV_NREG; ++i) coef[i * CV_OUT + k] = 0.0;
+    for (int a = 0; a < na; ++a) coef[act[a] * CV_OUT + k] = std::isfinite(x[a]) ? x[a] : 0.0;
+  }
 }
 
 void cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg, myql_cv_net *net) {
   if (cfg->batch < CV_TROWS || cfg->batch % CV_TROWS != 0) throw Fail(MYQL_E_INVALID_ARG, "batch must be a multiple of 32");
   if (cfg->steps < 1 || cfg->train_paths < 1) throw Fail(MYQL_E_INVALID_ARG, "need steps >= 1 and train_paths >= 1");
   const CvNorm nm = cv_norm(jobs, n);
+  const int nk = static_cast<int>(jobs[0].instrument->n_strikes);
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
-  std::vector<std::unique_ptr<CvRecord>> recs;
+  while (c->cv_recs.size() < static_cast<size_t>(n)) c->cv_recs.push_back(std::make_unique<CvRecord>());
   std::vector<CvSet> sets;
+  const int mblocks = 64;
+  DevBuf mpart;
+  mpart.reserve(static_cast<size_t>(nk) * mblocks * 2 * sizeof(double));
+  std::vector<CvRecord *> rp;
   for (int64_t i = 0; i < n; ++i) {
     myql_mc_config tc = *jobs[i].config;
     tc.num_paths = cfg->train_paths;
     tc.seed = cfg->train_seed;
     tc.precision = MYQL_FP32;
     tc.num_streams = 0;
-    recs.push_back(std::make_unique<CvRecord>());
-    cv_record<float>(c, jobs[i], tc, static_cast<int>(cfg->points), *recs.back());
-    sets.push_back(recs.back()->set);
+    cv_record<float>(c, jobs[i], tc, static_cast<int>(cfg->points), *c->cv_recs[i]);
+    rp.push_back(c->cv_recs[i].get());
+  }
+  cv_record_paths<float>(c, rp);
+  for (int64_t i = 0; i < n; ++i) {
+    CvRecord &rc = *c->cv_recs[i];
+    // per-strike scale of the training targets
+    HIPCHECK(launch_cv_moments(rc.set, mpart.as<double>(), mblocks, c->stream));
+    std::vector<double> h(static_cast<size_t>(nk) * mblocks * 2);
+    HIPCHECK(hipMemcpyAsync(h.data(), mpart.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<float> mu(nk), isd(nk);
+    const double invM = 1.0 / static_cast<double>(rc.set.M);
+    for (int k = 0; k < nk; ++k) {
+      double s = 0.0, s2 = 0.0;
+      for (int b = 0; b < mblocks; ++b) {
+        s += h[2 * (static_cast<size_t>(k) * mblocks + b)];
+        s2 += h[2 * (static_cast<size_t>(k) * mblocks + b) + 1];
+      }
+      const double mean = s * invM, var = std::max(s2 * invM - mean * mean, 0.0);
+      mu[k] = static_cast<float>(mean);
+      isd[k] = var > 1e-20 ? static_cast<float>(1.0 / std::sqrt(var)) : 0.0f;
+    }
+    rc.mu.reserve(nk * sizeof(float));
+    rc.isd.reserve(nk * sizeof(float));
+    HIPCHECK(hipMemcpyAsync(rc.mu.p, mu.data(), nk * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(rc.isd.p, isd.data(), nk * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    rc.set.mu = rc.mu.as<float>();
+    rc.set.inv_sd = rc.isd.as<float>();
+    sets.push_back(rc.set);
   }
   DevBuf d_sets, prm, mom, var, gpart, lpart, hist, step;
   d_sets.reserve(sets.size() * sizeof(CvSet));
@@ -1590,7 +1850,8 @@ void cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_con
   HIPCHECK(hipMemsetAsync(step.p, 0, sizeof(int), c->stream));
   CvTrainArgs a{prm.as<float>(), mom.as<float>(), var.as<float>(), gpart.as<float>(), lpart.as<float>(),
                 hist.as<float>(), d_sets.as<CvSet>(), static_cast<int>(n), nm, cfg->train_seed, step.as<int>(),
-                static_cast<int>(cfg->batch), static_cast<float>(cfg->lr), true, nullptr, nullptr};
+                static_cast<int>(cfg->batch), nk, static_cast<float>(cfg->lr),
+                static_cast<float>(1.0 / (static_cast<double>(cfg->batch) * nk)), true, nullptr, nullptr};
   // one Adam step (train, reduce + update, counter) captured as a graph and
   // replayed; the step counter lives on the device, so replays need no
   // argument changes
@@ -1616,6 +1877,21 @@ void cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_con
   (void)hipGraphDestroy(graph);
   HIPCHECK(ge);
   HIPCHECK(se);
+  // per-(maturity, strike) coefficients of the regressors, fitted on the
+  // training paths (independent of any pricing paths)
+  std::memset(net->coef, 0, sizeof(net->coef));
+  {
+    std::vector<double> fs;
+    cv_eval_sets(c, 2, prm, nm, sets, d_sets.as<CvSet>(), std::vector<const double *>(sets.size(), nullptr), fs);
+    for (int64_t i = 0; i < n; ++i) {
+      const std::vector<double> h(fs.begin() + i * CV_OUT * CV_NS_FIT,
+                                  fs.begin() + i * CV_OUT * CV_NS_FIT + static_cast<size_t>(nk) * CV_NS_FIT);
+      cv_fit(h, nk, sets[i].M, &net->coef[i][0][0]);
+      net->maturity[i] = jobs[i].instrument->maturity;
+    }
+  }
+  net->n_jobs = static_cast<int32_t>(n);
+  net->n_strikes = nk;
   HIPCHECK(hipEventRecord(c->ev1, c->stream));
   std::vector<float> h(cfg->steps);
   HIPCHECK(hipMemcpyAsync(net->params, prm.p, CV_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -1632,35 +1908,27 @@ void cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_con
   std::memcpy(net->norm, nv, sizeof(nv));
 }
 
-template <typename R>
-void cv_price_one(myql_ctx *c, const myql_mc_job &jb, const myql_cv_config *cfg, const DevBuf &prm, const CvNorm &nm,
-                  myql_cv_result *cr, double &ms_total) {
-  CvRecord rec;
-  HIPCHECK(hipEventRecord(c->ev0, c->stream));
-  cv_record<R>(c, jb, *jb.config, static_cast<int>(cfg->points), rec);
-  const CvSet &S = rec.set;
+// Per-strike results of a priced set from its sums (ns = 4 or 8 per strike).
+void cv_results(const myql_mc_job &jb, const CvSet &S, int mode, const double *sums, double device_ms,
+                myql_cv_result *cr) {
+  const int ns = mode == 1 ? CV_NS_PRICE : 4;
   const uint64_t M = S.M;
-  const int blocks = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(c->n_cu) * 8,
-                                                                                (M + 3) / 4)));
-  DevBuf part, sums;
-  part.reserve(static_cast<size_t>(blocks) * (BLOCK / 64) * 64 * 4 * sizeof(double));
-  sums.reserve(64 * 4 * sizeof(double));
-  HIPCHECK(launch_cv_eval(prm.as<float>(), nm, S, part.as<double>(), blocks, sums.as<double>(), c->stream));
-  HIPCHECK(hipEventRecord(c->ev1, c->stream));
-  std::vector<double> h(S.nk * 4);
-  HIPCHECK(hipMemcpyAsync(h.data(), sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHECK(hipStreamSynchronize(c->stream));
-  float ms = 0.f;
-  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  ms_total += ms;
   const double invM = 1.0 / static_cast<double>(M), df = S.df;
   myql_mc_result &res = *jb.result;
   for (int k = 0; k < S.nk; ++k) {
-    const double mc = h[4 * k] * invM, vc = std::max(h[4 * k + 1] * invM - mc * mc, 0.0);
-    const double mp = h[4 * k + 2] * invM, vp = std::max(h[4 * k + 3] * invM - mp * mp, 0.0);
+    const double *s = sums + static_cast<size_t>(k) * ns;
+    const double mc = s[0] * invM, vc = std::max(s[1] * invM - mc * mc, 0.0);
+    const double mp = s[2] * invM, vp = std::max(s[3] * invM - mp * mp, 0.0);
     const bool use = vc < vp;
     if (res.price) res.price[k] = (use ? mc : mp) * df;
     if (res.price_std_err) res.price_std_err[k] = std::sqrt((use ? vc : vp) * invM) * df;
+    if (mode == 1) {
+      const double md = s[4] * invM, vd = s[5] * invM - md * md, mg = s[6] * invM, vg = s[7] * invM - mg * mg;
+      if (res.delta) res.delta[k] = md * df;
+      if (res.delta_std_err) res.delta_std_err[k] = std::sqrt(vd * invM) * df;
+      if (res.gamma) res.gamma[k] = mg * df;
+      if (res.gamma_std_err) res.gamma_std_err[k] = std::sqrt(vg * invM) * df;
+    }
     if (cr) {
       if (cr->plain_price) cr->plain_price[k] = mp * df;
       if (cr->plain_std_err) cr->plain_std_err[k] = std::sqrt(vp * invM) * df;
@@ -1669,11 +1937,36 @@ void cv_price_one(myql_ctx *c, const myql_mc_job &jb, const myql_cv_config *cfg,
     }
   }
   res.total_steps = M * jb.config->time_steps;
-  res.device_ms = ms;
+  res.device_ms = device_ms;
 }
+
+// Device bytes of a job's recording.
+size_t cv_record_bytes(const myql_mc_job &jb, int points) {
+  const uint64_t M = jb.config->num_paths;
+  const uint64_t nj = jb.model->n_factors == 0 ? 1 : std::min<uint64_t>(points, jb.config->time_steps);
+  return static_cast<size_t>(M) * (nj * CV_REC * sizeof(float) + 2 * sizeof(double));
+}
+
+// The jobs are priced in chunks whose recordings together fit kCvChunkBytes
+// of device memory (at least one job per chunk): per chunk, every job's
+// paths in one grid per precision and factor count, every set's evaluation
+// in one grid per factor count and Greek mode, one synchronisation. Each
+// job's device_ms is the whole call's device time.
+constexpr size_t kCvChunkBytes = size_t(32) << 30;
 
 void cv_price(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_config *cfg, const myql_cv_net *net,
               myql_cv_result *cr) {
+  if (net->n_strikes != jobs[0].instrument->n_strikes || net->n_jobs < 1 || net->n_jobs > CV_MAXJOBS)
+    throw Fail(MYQL_E_INVALID_ARG, "control variate: the network was trained for another strike strip");
+  std::vector<const double *> coef(n);
+  for (int64_t i = 0; i < n; ++i) {
+    // the coefficients fitted for this maturity
+    int t = -1;
+    for (int j = 0; j < net->n_jobs; ++j)
+      if (net->maturity[j] == jobs[i].instrument->maturity) t = j;
+    if (t < 0) throw Fail(MYQL_E_INVALID_ARG, "control variate: no coefficients trained for this maturity");
+    coef[i] = &net->coef[t][0][0];
+  }
   CvNorm nm;
   nm.ts = net->norm[0];
   nm.xs = net->norm[1];
@@ -1681,17 +1974,63 @@ void cv_price(myql_ctx *c, const myql_mc_job *jobs, int64_t n, const myql_cv_con
   nm.vs1 = net->norm[3];
   nm.logS0 = net->norm[4];
   nm.S0 = net->norm[5];
+  const int points = static_cast<int>(cfg->points);
   DevBuf prm;
   prm.reserve(CV_NPARAM * sizeof(float));
   HIPCHECK(hipMemcpyAsync(prm.p, net->params, CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  double ms = 0.0;
-  for (int64_t i = 0; i < n; ++i) {
-    if (jobs[i].config->precision == MYQL_FP32)
-      cv_price_one<float>(c, jobs[i], cfg, prm, nm, cr ? cr + i : nullptr, ms);
-    else
-      cv_price_one<double>(c, jobs[i], cfg, prm, nm, cr ? cr + i : nullptr, ms);
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  // per job: its set and sums, kept until the device time is known
+  std::vector<CvSet> all_sets(n);
+  std::vector<std::vector<double>> all_sums(n);
+  std::vector<int> modes(n);
+  for (int64_t i0 = 0; i0 < n;) {
+    int64_t i1 = i0;
+    size_t bytes = 0;
+    while (i1 < n && (i1 == i0 || bytes + cv_record_bytes(jobs[i1], points) <= kCvChunkBytes))
+      bytes += cv_record_bytes(jobs[i1++], points);
+    while (c->cv_recs.size() < static_cast<size_t>(i1 - i0)) c->cv_recs.push_back(std::make_unique<CvRecord>());
+    std::vector<CvRecord *> r32, r64;
+    for (int64_t i = i0; i < i1; ++i) {
+      CvRecord &rc = *c->cv_recs[i - i0];
+      if (jobs[i].config->precision == MYQL_FP32) {
+        cv_record<float>(c, jobs[i], *jobs[i].config, points, rc);
+        r32.push_back(&rc);
+      } else {
+        cv_record<double>(c, jobs[i], *jobs[i].config, points, rc);
+        r64.push_back(&rc);
+      }
+      modes[i] = jobs[i].greek_mode == MYQL_GREEK_ESSENTIAL ? 1 : 0;
+    }
+    cv_record_paths<float>(c, r32);
+    cv_record_paths<double>(c, r64);
+    for (int mode = 0; mode < 2; ++mode) {
+      std::vector<CvSet> sets;
+      std::vector<const double *> cf;
+      std::vector<int64_t> idx;
+      for (int64_t i = i0; i < i1; ++i)
+        if (modes[i] == mode) {
+          sets.push_back(c->cv_recs[i - i0]->set);
+          cf.push_back(coef[i]);
+          idx.push_back(i);
+        }
+      if (sets.empty()) continue;
+      const CvSet *ds = cv_upload_sets(c, sets, c->cv_sets);
+      std::vector<double> sums;
+      cv_eval_sets(c, mode, prm, nm, sets, ds, cf, sums);
+      const int ns = mode == 1 ? CV_NS_PRICE : 4;
+      for (size_t q = 0; q < idx.size(); ++q) {
+        all_sets[idx[q]] = sets[q];
+        all_sums[idx[q]].assign(sums.begin() + q * CV_OUT * ns, sums.begin() + (q + 1) * CV_OUT * ns);
+      }
+    }
+    i0 = i1;
   }
-  (void)ms;
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  for (int64_t i = 0; i < n; ++i)
+    cv_results(jobs[i], all_sets[i], modes[i], all_sums[i].data(), ms, cr ? cr + i : nullptr);
 }
 
 int fail(myql_ctx *c, const Fail &e) {
@@ -2295,11 +2634,11 @@ int myql_debug_cv_input_grad(myql_ctx *c, const float *params, const float *feat
     DevBuf p, f, o;
     p.reserve(CV_NPARAM * sizeof(float));
     f.reserve(n * CV_IN * sizeof(float));
-    o.reserve(n * 3 * sizeof(float));
+    o.reserve(n * CV_OUT * 3 * sizeof(float));
     HIPCHECK(hipMemcpyAsync(p.p, params, CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(f.p, feats, n * CV_IN * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(launch_cv_grad_debug(p.as<float>(), f.as<float>(), static_cast<int>(n), o.as<float>(), c->stream));
-    HIPCHECK(hipMemcpyAsync(out, o.p, n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(out, o.p, n * CV_OUT * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->err.clear();
     return MYQL_OK;
@@ -2319,16 +2658,16 @@ int myql_debug_cv_param_grad(myql_ctx *c, const float *params, const float *feat
     DevBuf p, f, t, gp, lp, stp;
     p.reserve(CV_NPARAM * sizeof(float));
     f.reserve(n * CV_IN * sizeof(float));
-    t.reserve(n * sizeof(float));
+    t.reserve(n * CV_OUT * sizeof(float));
     gp.reserve(static_cast<size_t>(nb) * CV_NPARAM * sizeof(float));
     lp.reserve(nb * sizeof(float));
     stp.reserve(sizeof(int));
     HIPCHECK(hipMemcpyAsync(p.p, params, CV_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(f.p, feats, n * CV_IN * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemcpyAsync(t.p, targets, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(t.p, targets, n * CV_OUT * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemsetAsync(stp.p, 0, sizeof(int), c->stream));
     CvTrainArgs a{p.as<float>(), nullptr, nullptr, gp.as<float>(), lp.as<float>(), nullptr, nullptr, 0, CvNorm{},
-                  0, stp.as<int>(), static_cast<int>(n), 0.f, false, f.as<float>(), t.as<float>()};
+                  0, stp.as<int>(), static_cast<int>(n), CV_OUT, 0.f, 1.f, false, f.as<float>(), t.as<float>()};
     HIPCHECK(launch_cv_train_step(a, c->stream));
     std::vector<float> g(static_cast<size_t>(nb) * CV_NPARAM), l(nb);
     HIPCHECK(hipMemcpyAsync(g.data(), gp.p, g.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -2341,7 +2680,7 @@ int myql_debug_cv_param_grad(myql_ctx *c, const float *params, const float *feat
       for (int b = 0; b < nb; ++b) s += g[static_cast<size_t>(b) * CV_NPARAM + i];
       grad[i] = static_cast<float>(s);
     }
-    if (loss) *loss = ls / static_cast<double>(n);
+    if (loss) *loss = ls / (static_cast<double>(n) * CV_OUT);
     c->err.clear();
     return MYQL_OK;
   } catch (const Fail &e) {

@@ -69,6 +69,8 @@ struct Job {
   const uint64_t *streams;         // S x 4 stream states
   const void *nci[2];              // interleaved {y, dx*m} rows, R precision
   const double *nci_lgam[2];       // lgamma(d/2 + N) for N < nci_lg_n
+  const double *nci_mean[2];       // per table row, the mean of its quantile map
+  int nci_rows[2];
   int nci_lg_n;
   const void *nci_tail[2];         // upper-tail rows {q, hw*dq/dw} on w = -log(1-u); null = exact
   double tail_w0, tail_inv_hw;

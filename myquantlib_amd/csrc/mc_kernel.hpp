@@ -469,7 +469,8 @@ template <typename R, int NV, int NFX, bool TD = false, int KT = 0>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
                                               const VarParams<R> *__restrict__ const *P, int f,
                                               R (&v)[NV][NFX], R (&vn)[NV][NFX], Rng &g,
-                                              double (*vt)[NFX] = nullptr, double (*vnt)[NFX] = nullptr) {
+                                              double (*vt)[NFX] = nullptr, double (*vnt)[NFX] = nullptr,
+                                              int *n_out = nullptr) {
   const int sch = J->scheme[f];
   auto is_t = [](int k) { return TD && (k == KT || k == KT + 1); };
   // a slot's new variance from its (unscaled) table value or exact quantile
@@ -605,11 +606,13 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         putd(k, q);
       }
     }
+    if (n_out) *n_out = Ns[0];  // slot 0's Poisson count (control-variate recording)
   } else {  // Exact (Poisson-gamma mixture); variable draws, so never lockstep
     if constexpr (NV == 1) {
       const double lam = P[0]->Kd[f][0] * static_cast<double>(v[0][f]);
       const int N = poisson_count_packed(lam, uniform(g));
       vn[0][f] = gamma_draw<R>(P[0]->Kd[f][1], N, g) * P[0]->Kr[f][2];
+      if (n_out) *n_out = N;
     }
   }
 }

@@ -24,11 +24,11 @@ __device__ __forceinline__ R nv_step(const VarParams<R> *__restrict__ P, int f, 
 // single-slot variance step with the scheme's own draws
 template <typename R, int NFX>
 __device__ __forceinline__ R evolve_one(const Job<R> *__restrict__ J, const VarParams<R> *__restrict__ P, int f,
-                                        R v, Rng &g) {
+                                        R v, Rng &g, int *n_out = nullptr) {
   R va[1][NFX], vn[1][NFX];
   va[0][f] = v;
   const VarParams<R> *__restrict__ PP[1] = {P};
-  evolve_factor<R, 1, NFX>(J, PP, f, va, vn, g);
+  evolve_factor<R, 1, NFX>(J, PP, f, va, vn, g, nullptr, nullptr, n_out);
   return vn[0][f];
 }
 

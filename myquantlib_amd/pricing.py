@@ -338,18 +338,22 @@ class Engine:
         return out
 
     def cv_input_grad(self, params: np.ndarray, feats: np.ndarray) -> np.ndarray:
+        """Device (BF16 MFMA) tangents of the 64 outputs along features 1..3
+        for rows of 5 features: array (n, 64, 3)."""
         p = np.ascontiguousarray(params, dtype=np.float32)
-        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 6)
-        out = np.zeros((f.shape[0], 3), dtype=np.float32)
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 5)
+        out = np.zeros((f.shape[0], 64, 3), dtype=np.float32)
         pf = C.POINTER(C.c_float)
         self._check(self.lib.myql_debug_cv_input_grad(self.ctx, p.ctypes.data_as(pf), f.ctypes.data_as(pf),
                                                       f.shape[0], out.ctypes.data_as(pf)))
         return out
 
     def cv_param_grad(self, params: np.ndarray, feats: np.ndarray, targets: np.ndarray):
+        """FP32 training gradient of the mean over rows and the 64 outputs of
+        (net - target)^2: rows of 5 features, targets (n, 64)."""
         p = np.ascontiguousarray(params, dtype=np.float32)
-        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 6)
-        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 5)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 64)
         g = np.zeros(capi.CV_NPARAM, dtype=np.float32)
         loss = C.c_double(0.0)
         pf = C.POINTER(C.c_float)
