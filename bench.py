@@ -109,13 +109,13 @@ def model_bates_kou():
     return make_model(1, [Heston(2.0, 0.04, 0.5, -0.7, 0.04)], jump_kind=jk, jump=jp)
 
 
-def cv_leg(eng, eval_paths: int, bates_kou: bool, payoff: int = 0, points: int = 8):
+def cv_leg(eng, eval_paths: int, bates_kou: bool, payoff: int = 0, points: int = 16):
     """Plain FP32, plain FP64 and learned-CV FP32 on the strip's 512 options
     (vanilla calls, price only; or auto-smoothed cash digitals, which the
     reference smooths in Essential mode, priced in that mode): ms to 1e-4
     relative SE for each. The CV network is trained once (2^20 training
     paths per maturity, seed 7, 2,000 Adam steps of 8,192 rows, lr 1e-3,
-    8 control points; the regressor coefficients are fitted on the same
+    16 control points; the regressor coefficients are fitted on the same
     training paths) and then evaluated on independent paths (seed 42), so
     its time to the target is the training time plus the evaluation time
     scaled to the target: t_train + t_eval (worst se / price / 1e-4)^2."""
@@ -145,6 +145,20 @@ def cv_leg(eng, eval_paths: int, bates_kou: bool, payoff: int = 0, points: int =
     _, wc = ms_to_target(res, tc)
     mc = tt + tc * (wc / 1e-4) ** 2
     vr = np.concatenate([r["var_ratio"] for r in res])
+    # the CV prices of the vanillas against the GPU Gil-Pelaez pricer, in CV
+    # standard errors (options with price >= 0.1)
+    fz = None
+    if payoff == capi.PAYOFF_VANILLA:
+        from myquantlib_amd.pricing import InstrumentSpec
+        m = model_bates_kou() if bates_kou else model()
+        fr = eng.fourier_batch([(m, InstrumentSpec(strikes=STRIKES, maturity=T), capi.GREEK_NONE, S0, R, Q)
+                                for T in MATURITIES])
+        z = []
+        for r, f in zip(res, fr):
+            ok = f["price"] >= 0.1
+            z.extend(np.abs((r["price"] - f["price"]) / r["price_std_err"])[ok])
+        z = np.asarray(z)
+        fz = {"max_abs_z": float(z.max()), "frac_within_3se": float(np.mean(z < 3))}
     used = int(sum(int(np.sum(r["used_cv"])) for r in res))
     # the options that set the metric, plain and CV, and their variance ratios
     def worst_option(rs):
@@ -170,6 +184,7 @@ def cv_leg(eng, eval_paths: int, bates_kou: bool, payoff: int = 0, points: int =
             "speedup_vs_plain_fp64": m64 / mc,
             "worst_option_plain": {"T_K_var_ratio": worst_option(plain)},
             "worst_option_cv": {"T_K_var_ratio": worst_option(res)},
+            "cv_vs_fourier": fz,
             "variance_ratio_median": float(np.median(vr)), "variance_ratio_min": float(np.min(vr)),
             "variance_ratio_max": float(np.max(vr)),
             "options_using_cv": used, "options": int(vr.size), "train_loss": net.final_loss}

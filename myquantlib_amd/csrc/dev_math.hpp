@@ -227,6 +227,8 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
     if (fabs(W) < 3.0) {
       giles_asym_S(lam, ilam, W, S, Del);
     } else {
+      // uncontracted, as the oracle: (1 - R) + R log R vanishes as R -> 1
+#pragma clang fp contract(off)
       if constexpr (DEFER) {
         *rare = true;
         return 0.0;
@@ -262,8 +264,16 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
       }
       if (X > 0.5 * lam && X < 2.0 * lam) {
         const double Xi = 1.0 / X;
-        double Eta = X * ilam;
-        Eta = sqrt(2.0 * (1.0 - Eta + Eta * log(Eta)) / Eta);
+        double Eta;
+        {
+          // no FMA contraction here: fma(-X, ilam, 1) keeps the unrounded
+          // product, so at X = lam (integer intensities) 1 - Eta came out
+          // -5.6e-17 instead of 0 and the square root NaN, which skipped
+          // the correction below and returned N + 1
+#pragma clang fp contract(off)
+          Eta = X * ilam;
+          Eta = sqrt(2.0 * (1.0 - Eta + Eta * log(Eta)) / Eta);
+        }
         if (X > lam) Eta = -Eta;
         // Temme expansion coefficients (kTemmeB1 / kTemmeB0): odd / even
         // Horner chains in 1/X
@@ -403,6 +413,18 @@ __device__ __forceinline__ int poisson_count(double mean, double exp_mean, doubl
     if (p == 0.0) break;
   }
   return x;
+}
+
+// Jump count of one Poisson(mean) draw from one uniform. Below mean 10 the
+// sequential inversion above, as Boost does (JumpPolicies.hpp:48,60); from
+// 10 on Boost switches to transformed rejection (PTRD, a variable number of
+// draws), replaced here by the exact inverse CDF (Giles' method, the NCI
+// scheme's Poisson quantile) of the same single uniform: a different draw
+// sequence, the same distribution.
+__device__ static __noinline__ double poissinv(double lam, double U);
+__device__ __forceinline__ int jump_count(double mean, double exp_mean, double u) {
+  if (mean < 10.0) return poisson_count(mean, exp_mean, u);
+  return static_cast<int>(poissinv(mean, u));
 }
 
 // Exponential(rate) by inversion (replaces Boost's ziggurat: one draw each).

@@ -653,10 +653,9 @@ void fill_variant(VarParams<R> &V, const myql_model &m, const int32_t *schemes, 
   // The reference draws jump counts with Boost's poisson_distribution
   // (JumpPolicies.hpp:38,48,86,98), which inverts one uniform for means
   // below 10 (restated here) and switches to transformed rejection (PTRD,
-  // a variable number of draws) from 10 on -- not restated, so refused
-  // rather than priced from different draws.
-  if (V.jmean >= 10.0)
-    throw Fail(MYQL_E_UNSUPPORTED, "jump intensity x span >= 10 (Boost PTRD Poisson sampler) is not supported");
+  // a variable number of draws) from 10 on; from 10 on the kernels invert
+  // the same single uniform exactly instead (jump_count): the same
+  // distribution from a different draw sequence.
 }
 
 struct JobSpec {
@@ -1579,12 +1578,18 @@ void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, in
       S.rb_s[S.rb_n] = std::sqrt(static_cast<double>(n)) * dl;
       ++S.rb_n;
     }
-    S.rb_y0 = std::log(jb.S0) - 3.0;
-    S.rb_h = 5.0 / (CV_RB_PTS - 1);
-    S.rb_inv_h = 1.0 / S.rb_h;
-    out.rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
-    S.rb = out.rb.as<double>();
-    HIPCHECK(launch_cv_rb_table(S, out.rb.as<double>(), c->stream));
+    // counts beyond the tabulated terms would bias E[P | S_T-]: then the
+    // estimator goes without that term (D = 0), and stays unbiased
+    if (mass < 1.0 - 1e-14) {
+      S.rb_on = 0;
+    } else {
+      S.rb_y0 = std::log(jb.S0) - 3.0;
+      S.rb_h = 5.0 / (CV_RB_PTS - 1);
+      S.rb_inv_h = 1.0 / S.rb_h;
+      out.rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
+      S.rb = out.rb.as<double>();
+      HIPCHECK(launch_cv_rb_table(S, out.rb.as<double>(), c->stream));
+    }
   }
 }
 

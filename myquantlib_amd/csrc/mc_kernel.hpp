@@ -243,7 +243,7 @@ __device__ static R gamma_draw(double alpha_base, int N, Rng &g) {
 template <typename R>
 __device__ __forceinline__ R log_jump(const Job<R> *__restrict__ J, const VarParams<R> *__restrict__ P,
                                       Rng &g) {
-  const int n = poisson_count(P->jmean, P->jexpm, uniform(g));
+  const int n = jump_count(P->jmean, P->jexpm, uniform(g));
   if (n == 0) return P->jdrift;
   if (J->jump_kind == 1) {
     const R Z = normcdfinv<R>(uniform(g));
@@ -271,7 +271,7 @@ __device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
   int nmax = 0;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    n[k] = poisson_count(P[k]->jmean, P[k]->jexpm, ua);
+    n[k] = jump_count(P[k]->jmean, P[k]->jexpm, ua);
     nmax = n[k] > nmax ? n[k] : nmax;
   }
   R tot[NV];

@@ -208,11 +208,12 @@ void oracle_uniforms(uint64_t *state, int64_t n, double *out) {
   std::memcpy(state, g.s, 32);
 }
 
-// n Poisson(mean) variates by sequential inversion from stream seed_state(seed)
+// n jump counts of mean `mean` (jump_count: sequential inversion below 10,
+// the exact inverse CDF from 10) from stream seed_state(seed)
 void oracle_poisson_samples(double mean, int64_t n, uint64_t seed, int32_t *out) {
   Xoshiro g = seed_state(seed);
   const double em = std::exp(-mean);
-  for (int64_t i = 0; i < n; ++i) out[i] = poisson_inversion(mean, em, g);
+  for (int64_t i = 0; i < n; ++i) out[i] = jump_count(mean, em, g);
 }
 
 // n independent CIR variance paths of n_steps steps of size dt with one

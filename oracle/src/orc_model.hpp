@@ -336,7 +336,7 @@ static inline JumpWs prepare_jump(const myql_model &m, double span) {
 
 static inline double log_jump(const JumpWs &w, Xoshiro &g) {
   if (w.kind == MYQL_JUMP_NONE) return 0.0;
-  const int n = poisson_inversion(w.mean, w.exp_mean, g);
+  const int n = jump_count(w.mean, w.exp_mean, g);
   if (n == 0) return w.drift;
   if (w.kind == MYQL_JUMP_MERTON) {
     const double Z = normcdfinv(uniform(g));

@@ -10,6 +10,10 @@
 //                of the normal quantile exactly as the reference does.
 //   poisson_inversion  Boost poisson_distribution<int,double> for means below
 //                10 (sequential inversion): JumpPolicies.hpp:48,60.
+//   jump_count   the jump counts: poisson_inversion below mean 10; from 10
+//                on Boost's PTRD rejection sampler is replaced by
+//                poissinvcdf of one uniform (documented deviation, pinned
+//                against scipy's Poisson quantile and pmf).
 //   exponential  Boost exponential_distribution is a ziggurat whose tables
 //                are not available here; replaced by inversion
 //                -log1p(-u)/rate, one uniform per variate (documented
@@ -242,6 +246,16 @@ static inline int poisson_inversion(double mean, double exp_mean, Xoshiro &g) {
     if (p == 0.0) break;  // u left above a rounded-down total: stop
   }
   return x;
+}
+
+// Jump count: sequential inversion below mean 10 (Boost's branch there);
+// from 10 on Boost's PTRD rejection sampler is replaced by the exact inverse
+// CDF of the same single uniform (poissinvcdf): the same distribution, a
+// different draw sequence (the engine does the same).
+static inline int jump_count(double mean, double exp_mean, Xoshiro &g) {
+  if (mean < 10.0) return poisson_inversion(mean, exp_mean, g);
+  ORC_OPS(60);
+  return static_cast<int>(poissinvcdf(mean, uniform(g)));
 }
 
 static inline double exponential(double rate, Xoshiro &g) {

@@ -8,6 +8,7 @@ Every call goes through the C-ABI (libmyql_hip.so). Tolerances:
 """
 import numpy as np
 import pytest
+from scipy import stats
 
 from myquantlib_amd import capi
 from myquantlib_amd.pricing import (Heston, InstrumentSpec, make_model, mc_config, merton, kou,
@@ -87,6 +88,23 @@ def test_poissinv_device_exact(engine, oracle):
     out = engine.poissinv(lam, u)
     ref = np.array([oracle.poissinvcdf(a, b) for a, b in zip(lam, u)])
     assert np.array_equal(out, ref)
+
+
+def test_poissinv_device_integer_intensities(engine, oracle):
+    """Integer intensities put the count X = lam in Giles' Temme correction
+    with X / lam = 1 exactly (jump means lambda T = 10, 20, ...). u around
+    F(lam - 1) is where a contracted 1 - X/lam went negative on the device
+    and returned lam instead of lam - 1."""
+    rng = np.random.default_rng(2)
+    for lam in (10.0, 11.0, 20.0, 37.0, 100.0):
+        c = stats.poisson.cdf(lam - 1, lam)
+        u = np.concatenate([c + rng.uniform(-2e-3, 2e-3, 3000), rng.uniform(0, 1, 1000)])
+        out = engine.poissinv(np.full(u.size, lam), u)
+        ref = np.array([oracle.poissinvcdf(lam, b) for b in u])
+        assert np.array_equal(out, ref), lam
+        ppf = stats.poisson.ppf(u, lam)
+        ok = np.abs(stats.poisson.cdf(ppf, lam) - u) > 1e-12
+        assert np.array_equal(out[ok], ppf[ok]), lam
 
 
 def test_chi2_quantile_device(engine, oracle):
@@ -336,21 +354,30 @@ def test_fused_asian_barrier_equal_separate_calls(engine, prec, mode, nf):
             assert np.array_equal(np.asarray(f_res[f]), np.asarray(s_res[f])), f
 
 
-def test_jump_means_from_ten_are_refused(engine):
-    """Boost's poisson_distribution (the reference's jump counts) inverts one
-    uniform below mean 10 and switches to its PTRD rejection sampler from 10
-    on; that branch is not restated, so the engine refuses such calls instead
-    of pricing them from different draws."""
-    from myquantlib_amd.pricing import EngineError
-    jk, jp = merton(6.0, -0.05, 0.1)
+@pytest.mark.parametrize("jump", ["merton", "kou"])
+def test_jump_means_from_ten(engine, oracle, jump):
+    """Jump means from 10 on: Boost's poisson_distribution switches to its
+    PTRD rejection sampler there; the engine and the oracle invert the same
+    single uniform exactly instead (Giles' method). On identical draws the
+    GPU equals the oracle (FP64, Full strip with its T-bumped means on either
+    side of 10 for T = 2, lambda 5: means 10 +- 5e-4), and the price agrees
+    with the Gil-Pelaez pricer."""
+    jk, jp = merton(5.0, -0.05, 0.1) if jump == "merton" else kou(5.0, 0.4, 10.0, 5.0)
     m = make_model(1, [H1], jump_kind=jk, jump=jp)
-    inst = InstrumentSpec(strikes=[100.0], maturity=2.0)          # European: mean = 6 x 2
-    with pytest.raises(EngineError) as e:
-        engine.mc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE, mc_config(1 << 12, 50), 100.0, 0.02, 0.0)
-    assert e.value.code == capi.E_UNSUPPORTED and "PTRD" in str(e.value)
-    ok = engine.mc(m, [capi.SCHEME_NV], InstrumentSpec(strikes=[100.0], maturity=1.5), capi.GREEK_NONE,
-                   mc_config(1 << 12, 50), 100.0, 0.02, 0.0)      # mean 9: inversion, priced
-    assert np.isfinite(ok["price"][0])
+    inst = InstrumentSpec(strikes=[80.0, 100.0, 120.0], maturity=2.0)    # European: mean 5 x 2
+    cfg = mc_config(1 << 12, 50, seed=5, precision=capi.FP64, num_streams=1024)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NV], inst, capi.GREEK_FULL, cfg)
+    compare(g, c, FULL, 1e-9)
+    big = engine.mc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE,
+                    mc_config(1 << 20, 200, seed=7, precision=capi.FP64), 100.0, 0.02, 0.0)
+    f = engine.fourier(m, inst, capi.GREEK_NONE, 100.0, 0.02, 0.0)
+    z = np.abs(np.asarray(big["price"]) - np.asarray(f["price"])) / np.asarray(big["price_std_err"])
+    assert z.max() < 4.0, z
+    # an asian (log_jump on every step) with mean 5 x 2.5 per step span
+    asian = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=5.0)
+    g, c = _run_both(engine, oracle, m, [capi.SCHEME_NV], asian, capi.GREEK_NONE,
+                     mc_config(1 << 10, 2, seed=5, precision=capi.FP64, num_streams=256))
+    compare(g, c, ["price", "price_std_err"], 1e-9)
 
 
 def test_nci_fine_steps_agree_with_fourier(engine):

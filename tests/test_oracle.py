@@ -116,15 +116,24 @@ def test_poissinvcdf_brackets(oracle):
     assert math.isnan(oracle.poissinvcdf(3.0, 1.0))
 
 
-def test_poisson_inversion_matches_ppf(oracle):
-    for mean in (0.005, 0.5, 2.0, 7.5):
-        n = 4000
+def test_jump_counts_match_ppf(oracle):
+    """Jump counts invert one uniform: sequentially below mean 10 (Boost's
+    branch), by Giles' method from 10 on (replacing Boost's PTRD rejection
+    sampler). Both must give scipy's Poisson quantile of that uniform, and
+    the counts' mean and variance the Poisson values."""
+    for mean in (0.005, 0.5, 2.0, 7.5, 10.0, 25.0, 60.0, 300.0):
+        n = 4000 if mean < 10 else 40000
         x = oracle.poisson_samples(mean, n, 99)
         u = oracle.uniforms(oracle.seed_state(99), n)
         ref = stats.poisson.ppf(u, mean)
         cdf_gap = np.abs(stats.poisson.cdf(ref, mean) - u)
         ok = cdf_gap > 1e-12
-        assert np.array_equal(x[ok], ref[ok].astype(np.int32))
+        assert ok.mean() > 0.999
+        assert np.array_equal(x[ok], ref[ok].astype(np.int32)), mean
+        if mean >= 10:
+            se = math.sqrt(mean / n)
+            assert abs(x.mean() - mean) < 5 * se, mean
+            assert abs(x.var() / mean - 1.0) < 5 * math.sqrt(2.0 / n), mean
 
 
 def test_chi2_quantile_golden(oracle):
