@@ -383,6 +383,15 @@ int myql_debug_normcdfinv(myql_ctx *ctx, const double *p, int64_t n,
 /* Device Giles inverse Poisson CDF on n (lambda, U) pairs (FP64). */
 int myql_debug_poissinv(myql_ctx *ctx, const double *lambda, const double *u,
                         int64_t n, double *out);
+/* Debug: the Poisson counts the path kernels use for n draws of n_slots
+ * (1 or 5) slots sharing one uniform u[i], intensities lambda[i * n_slots +
+ * k] (slot 0 the base path, the others bumped): an FP32 first pass decides
+ * Giles' exact quantile where u clears its error bounds, the bumped slots
+ * follow from slot 0's margins, and the FP64 method (myql_debug_poissinv)
+ * finishes the rest. out[i * n_slots + k] equals Giles' count for
+ * (lambda[i * n_slots + k], u[i]). No counterpart in the reference. */
+int myql_debug_poisson_counts(myql_ctx *ctx, const double *lambda, int32_t n_slots, const double *u, int64_t n,
+                              int32_t *out);
 /* Device chi-squared quantile (inverse regularised incomplete gamma). */
 int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
                              int64_t n, double *out);

@@ -134,7 +134,7 @@ EXPORTED = [
     "myql_create", "myql_destroy", "myql_last_error", "myql_tuning_report", "myql_mc_calculate",
     "myql_mc_calculate_noalloc", "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_fourier_calculate_batch", "myql_cv_train",
     "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
-    "myql_debug_poissinv", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
+    "myql_debug_poissinv", "myql_debug_poisson_counts", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
     "myql_debug_cv_input_grad",
     "myql_debug_cv_param_grad",
 ]
@@ -180,6 +180,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_normcdfinv.argtypes = [vp, _PD, C.c_int64, _PD, C.POINTER(C.c_float)]
     lib.myql_debug_poissinv.restype = C.c_int
     lib.myql_debug_poissinv.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
+    lib.myql_debug_poisson_counts.restype = C.c_int
+    lib.myql_debug_poisson_counts.argtypes = [vp, _PD, C.c_int32, _PD, C.c_int64, C.POINTER(C.c_int32)]
     lib.myql_debug_chi2_quantile.restype = C.c_int
     lib.myql_debug_chi2_quantile.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
     lib.myql_debug_path_payoffs.restype = C.c_int

@@ -204,6 +204,11 @@ struct PoisMargin {
   double up, lo, p_hi, p_lo;
   bool asym;
   double S, Del;
+  // error allowances of the margins and of S (FP64 pass: rounding; FP32
+  // first pass: its error bounds), and that pass's W in FP32
+  double slack = 1e-12, s_tol = 1e-9;
+  bool fp32 = false;
+  float Wf = 0.0f;
 };
 
 // DEFER: instead of running the rare branches (the tail Newton iteration for
@@ -399,6 +404,113 @@ __device__ __forceinline__ bool giles_asym_only(double lam, double W, int &N) {
   if (X < 10.0) return false;
   N = static_cast<int>(X);
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson counts, FP32 first pass. Giles' method returns the exact quantile
+// of Poisson(lam) at U except where U lies within rounding (~1e-12) of a CDF
+// value. This pass finds that quantile in FP32 with an error bound and
+// returns it only where U clears the bound; elsewhere, and in the method's
+// rare branches (|W| >= 3, the Temme / Stirling correction band), it
+// returns -1 and the FP64 method decides. So its counts are the method's.
+//  - lam > 4: the central asymptotic S in FP32 from W = Phi^-1(U) in FP32;
+//    |S32 - S64| < tol = 1e-4 + 2e-6 lam (FP32 rounding of terms up to
+//    lam + 3 sqrt(lam), and W's FP32 error times sqrt(lam)). floor(S) when
+//    S >= 10 + tol and S clears the integers and the correction band
+//    [X, X + 2 Del) by tol; the bottom-up sum when S < 10 - tol (the method
+//    sums from 0 there too).
+//  - lam <= 4, or S < 10: F(n) = sum of p(j), p(j) = p(j-1) lam / j,
+//    p(0) = e^-lam, in FP32 (relative error below 1e-5 for the counts, at
+//    most ~25, this branch reaches); n when F(n) - U and U - F(n-1) both
+//    exceed 3e-5.
+// ---------------------------------------------------------------------------
+__constant__ static const float kInvN[48] = {0.0f, 1.0f / 1, 1.0f / 2, 1.0f / 3, 1.0f / 4, 1.0f / 5, 1.0f / 6, 1.0f / 7, 1.0f / 8, 1.0f / 9, 1.0f / 10, 1.0f / 11, 1.0f / 12, 1.0f / 13, 1.0f / 14, 1.0f / 15, 1.0f / 16, 1.0f / 17, 1.0f / 18, 1.0f / 19, 1.0f / 20, 1.0f / 21, 1.0f / 22, 1.0f / 23, 1.0f / 24, 1.0f / 25, 1.0f / 26, 1.0f / 27, 1.0f / 28, 1.0f / 29, 1.0f / 30, 1.0f / 31, 1.0f / 32, 1.0f / 33, 1.0f / 34, 1.0f / 35, 1.0f / 36, 1.0f / 37, 1.0f / 38, 1.0f / 39, 1.0f / 40, 1.0f / 41, 1.0f / 42, 1.0f / 43, 1.0f / 44, 1.0f / 45, 1.0f / 46, 1.0f / 47};
+
+__device__ __forceinline__ float giles_w32(double U) {
+  const double V = 1.0 - U;
+  const float W = normcdfinv_f(fmin(U, V));
+  return U > V ? -W : W;
+}
+
+__device__ __forceinline__ void giles_asym_S32(float lam, float W, float &S, float &Del) {
+  const float sl = __builtin_amdgcn_sqrtf(lam);
+  S = sl * W + (1.0f / 3.0f + (1.0f / 6.0f) * W * W) * (1.0f - W * __builtin_amdgcn_rcpf(12.0f * sl));
+  Del = (((1.0f / 160.0f) * (W * W) + 1.0f / 80.0f) * (W * W) + 1.0f / 40.0f) * __builtin_amdgcn_rcpf(lam);
+  S = lam + (S + Del);
+}
+
+// The asymptotic branch's count in FP32 (lam > 4, |W| < 3), or -1 when S is
+// within tol of 10, an integer or the correction band.
+__device__ __forceinline__ int giles_asym32(double lam, float W, float &S, float &Del, float &tol) {
+  const float lf = static_cast<float>(lam);
+  giles_asym_S32(lf, W, S, Del);
+  tol = 1e-4f + 2e-6f * lf;
+  const float X = floorf(S);
+  const float fr = S - X;
+  if (S >= 10.0f + tol && fr >= 2.0f * Del + tol && fr <= 1.0f - tol) return static_cast<int>(X);
+  return -1;
+}
+
+// The bottom-up sum's count in FP32, or -1 when U lies within 3e-5 of F(n)
+// or F(n-1) (or no count below 48 reaches U). The loop steps all active
+// lanes together, so j is wave-uniform and 1/j a scalar load.
+__device__ __forceinline__ int pois_sum32(double lam, double U, PoisMargin *mg) {
+  const float lf = static_cast<float>(lam), uf = static_cast<float>(U);
+  float p = expf(-lf), F = p, Fp = 0.0f, pp = 0.0f;
+  int n = 0;
+  bool run = F < uf;
+  for (int j = 1; j < 48; ++j) {
+    if (!__any(run)) break;
+    if (run) {
+      pp = p;
+      Fp = F;
+      p = p * lf * kInvN[j];
+      F += p;
+      n = j;
+      run = F < uf;
+    }
+  }
+  constexpr float tol = 3e-5f;
+  const float up = F - uf, lo = uf - Fp;
+  if (run || !(up > tol) || !(n == 0 || lo > tol)) return -1;
+  if (mg) {
+    mg->valid = true;
+    mg->up = up;
+    mg->lo = n > 0 ? static_cast<double>(lo) : U;
+    mg->p_hi = p;
+    mg->p_lo = n > 0 ? pp : 0.0f;
+    mg->slack = tol;
+  }
+  return n;
+}
+
+// The count Giles' method returns for (lam, U), 0 <= U < 1, from the FP32
+// first pass, or -1 (undecided). W = giles_w32(U), read when lam > 4. mg
+// receives the margins bumped_count reads.
+__device__ __forceinline__ int poisson_first32(double lam, double U, float W, PoisMargin *mg) {
+  if (mg) {
+    mg->valid = false;
+    mg->asym = false;
+    mg->fp32 = true;
+    mg->Wf = W;
+  }
+  if (U <= 0.0) return 0;
+  if (lam > 4.0) {
+    if (!(fabsf(W) < 2.999f)) return -1;
+    float S, Del, tol;
+    const int N = giles_asym32(lam, W, S, Del, tol);
+    if (N >= 0) {
+      if (mg) {
+        mg->asym = true;
+        mg->S = S;
+        mg->Del = Del;
+        mg->s_tol = tol;
+      }
+      return N;
+    }
+    if (!(S < 10.0f - tol)) return -1;
+  }
+  return pois_sum32(lam, U, mg);
 }
 
 // Poisson(mean) by sequential inversion on one uniform (Boost's method for

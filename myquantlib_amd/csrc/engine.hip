@@ -292,6 +292,28 @@ __global__ void debug_poissinv_kernel(const double *__restrict__ lam, const doub
   out[i] = poissinv(lam[i], u[i]);
 }
 
+// The path kernels' Poisson counts (FP32 first pass, bumped slots from slot
+// 0's margins, the FP64 method on wave-packed leftovers) for NV slots that
+// share one uniform; lanes past n run on placeholder inputs, since every
+// lane of a wave takes part in the packing.
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) debug_poisson_counts_kernel(const double *__restrict__ lam,
+                                                                     const double *__restrict__ u, int64_t n,
+                                                                     int32_t *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+  const bool live = i < n;
+  double l[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) l[k] = live ? lam[i * NV + k] : 1.0;
+  const double uu = live ? u[i] : 0.5;
+  int Ns[NV];
+  if constexpr (NV == 1) Ns[0] = poisson_count_packed(l[0], uu);
+  else poisson_counts_lockstep<NV>(l, uu, Ns);
+  if (live)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[i * NV + k] = Ns[k];
+}
+
 __global__ void debug_chi2_kernel(const double *__restrict__ dof, const double *__restrict__ p, int64_t n,
                                   double *__restrict__ out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2570,6 +2592,34 @@ int myql_debug_normcdfinv(myql_ctx *c, const double *p, int64_t n, double *out64
 int myql_debug_poissinv(myql_ctx *c, const double *lambda, const double *u, int64_t n, double *out) {
   if (!lambda || !u || !out) return MYQL_E_INVALID_ARG;
   return debug_map(c, 1, lambda, u, n, out, nullptr);
+}
+
+int myql_debug_poisson_counts(myql_ctx *c, const double *lambda, int32_t n_slots, const double *u, int64_t n,
+                              int32_t *out) {
+  if (!c || !lambda || !u || !out || (n_slots != 1 && n_slots != 5)) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (n <= 0) return MYQL_OK;
+    DevBuf dl, du, dout;
+    dl.reserve(n * n_slots * sizeof(double));
+    du.reserve(n * sizeof(double));
+    dout.reserve(n * n_slots * sizeof(int32_t));
+    HIPCHECK(hipMemcpyAsync(dl.p, lambda, n * n_slots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(du.p, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const unsigned nb = static_cast<unsigned>((n + BLOCK - 1) / BLOCK);
+    if (n_slots == 1)
+      hipLaunchKernelGGL(debug_poisson_counts_kernel<1>, dim3(nb), dim3(BLOCK), 0, c->stream, dl.as<double>(),
+                         du.as<double>(), n, dout.as<int32_t>());
+    else
+      hipLaunchKernelGGL(debug_poisson_counts_kernel<5>, dim3(nb), dim3(BLOCK), 0, c->stream, dl.as<double>(),
+                         du.as<double>(), n, dout.as<int32_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, dout.p, n * n_slots * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
 }
 
 int myql_debug_set_launch_shape(myql_ctx *c, int32_t precision, int32_t n_factors, int32_t lockstep_width,

@@ -389,8 +389,10 @@ __device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, u
 //    [X, X + 2 Del) (Del scaling as 1/lam), floor(S) is n0. Else the slot
 //    reruns the branch with the shared W.
 // Giles' method returns the exact quantile except where U lies within
-// rounding of a CDF value; the 1e-12 slack keeps those cases out, so a count
-// decided here equals the method's.
+// rounding of a CDF value; the slack (1e-12 after the FP64 method, the error
+// bounds after the FP32 first pass, whose S also carries its tolerance)
+// keeps those cases out, so a count decided here equals the method's. After
+// the FP32 pass the rerun of the asymptotic branch is in FP32 too.
 __device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, double il0, double W,
                                             const PoisMargin &mg) {
   const double dl = fabs(lamk - lam0);
@@ -401,34 +403,40 @@ __device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, do
     if (!(x <= 0.5)) return -1;
     const double g = dl * (1.0 + 2.0 * x);
     const bool rise = lamk > lam0;
+    const double sl = mg.slack;
     if (rise) {
-      if (!(n0 == 0 || mg.lo > 1e-12)) return -1;
-      if (mg.up > g * mg.p_hi + 1e-12) return n0;
+      if (!(n0 == 0 || mg.lo > sl)) return -1;
+      if (mg.up > g * mg.p_hi + sl) return n0;
     } else {
-      if (!(mg.up > 1e-12)) return -1;
-      if (n0 == 0 || mg.lo > g * mg.p_lo + 1e-12) return n0;
+      if (!(mg.up > sl)) return -1;
+      if (n0 == 0 || mg.lo > g * mg.p_lo + sl) return n0;
     }
     const double n = static_cast<double>(rise ? n0 : n0 - 1);
     const double p0 = rise ? mg.p_hi : mg.p_lo;
     const double q = 2.0 * n * il0, c = n * il0 - 1.0;
     const double B = dl * dl * dl * (1.0 / 6.0) * p0 * (1.0 + 2.0 * x) * ((q + 1.0) * (q + 1.0) + 2.0 * q * il0) +
-                     1e-12;
+                     sl;
     if (rise) {
       const double m = mg.up - p0 * dl * (1.0 + 0.5 * c * dl);  // F(n0; lamk) - U
       if (m > B) return n0;
       const double n1 = static_cast<double>(n0 + 1);
-      if (m < -B && mg.p_hi * lam0 * (1.0 - dl) + (m - B) * n1 > 1e-12 * n1) return n0 + 1;
+      if (m < -B && mg.p_hi * lam0 * (1.0 - dl) + (m - B) * n1 > sl * n1) return n0 + 1;
     } else {
       const double m = mg.lo - p0 * dl * (1.0 - 0.5 * c * dl);  // U - F(n0 - 1; lamk)
       if (m > B) return n0;
-      if (m < -B && mg.p_lo * (1.0 - n * dl * il0) + (m - B) > 1e-12) return n0 - 1;
+      if (m < -B && mg.p_lo * (1.0 - n * dl * il0) + (m - B) > sl) return n0 - 1;
     }
     return -1;
   }
   if (mg.asym && dl <= 0.5 * lam0 && lam0 - dl > 4.0) {
     const double f0 = mg.S - static_cast<double>(n0);
-    const double e = 2.0 * dl + 1e-9;
+    const double e = 2.0 * dl + mg.s_tol;
     if (f0 + e < 1.0 && f0 - e >= 2.0 * mg.Del * (1.0 + 2.0 * dl * il0)) return n0;
+  }
+  if (mg.fp32) {
+    if (!(lamk > 4.0 && fabsf(mg.Wf) < 2.999f)) return -1;
+    float S, Del, tol;
+    return giles_asym32(lamk, mg.Wf, S, Del, tol);
   }
   int n = n0;
   return giles_asym_only(lamk, W, n) ? n : -1;
@@ -438,14 +446,54 @@ __device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, do
 // wavefront (giles_rare); the same count as poissinv. Every active lane of
 // the wave must reach the call together.
 __device__ __forceinline__ int poisson_count_packed(double lam, double u) {
-  const double W = (lam > 4.0 && u > 0.0) ? giles_w(u) : 0.0;
-  bool rare = false;
-  int Ns[1] = {static_cast<int>(poissinv_w<true>(lam, u, W, &rare))};
+  const float W = (lam > 4.0 && u > 0.0) ? giles_w32(u) : 0.0f;
+  int Ns[1] = {poisson_first32(lam, u, W, nullptr)};
+  const bool rare = Ns[0] < 0;
   if (__any(rare)) {
     const double lams[1] = {lam};
     giles_rare<1>(lams, u, rare ? 1u : 0u, Ns);
   }
   return Ns[0];
+}
+
+// Poisson counts of NV slots (slot 0 the base path, the others bumped) from
+// their shared uniform u1, each equal to what Giles' method returns for that
+// slot's intensity:
+//  - slot 0 takes the FP32 first pass (normal quantile W of u1 computed
+//    once); its bottom-up sum also yields the CDF margins F(N0) - u1 and
+//    u1 - F(N0 - 1);
+//  - a bumped slot k takes N0 when its intensity equals slot 0's, else its
+//    count follows from those margins moved by the intensity change, or from
+//    slot 0's asymptotic estimate, or from rerunning the asymptotic branch
+//    alone with the same W (bumped_count);
+//  - anything else (the first pass's undecided cases and the method's rare
+//    branches) is finished by the FP64 method on wave-packed items
+//    (giles_rare).
+// Every active lane of the wave must reach the call together.
+template <int NV>
+__device__ __forceinline__ void poisson_counts_lockstep(const double (&lams)[NV], double u1, int (&Ns)[NV]) {
+  bool big = false;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    big = big || lams[k] > 4.0;
+    Ns[k] = 0;
+  }
+  if (u1 > 0.0) {  // u1 == 0: every count is 0
+    const float W = big ? giles_w32(u1) : 0.0f;
+    PoisMargin mg;
+    const int n0 = poisson_first32(lams[0], u1, W, &mg);
+    const bool r0 = n0 < 0;
+    unsigned pend = r0 ? 1u : 0u;
+    Ns[0] = r0 ? 0 : n0;
+    const double il0 = 1.0 / lams[0];
+#pragma unroll
+    for (int k = 1; k < NV; ++k) {
+      const int n = r0 ? -1 : bumped_count(lams[0], lams[k], n0, il0, W, mg);
+      if (n >= 0) Ns[k] = n;
+      else pend |= 1u << k;
+    }
+    if (__any(pend != 0u)) giles_rare<NV>(lams, u1, pend, Ns);
+  }
 }
 
 // NCI upper tail (u >= 0.999), row N: cubic Hermite in w = -log(1 - u) with
@@ -506,45 +554,15 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     const R h00 = R(2) * t3 - R(3) * t2 + R(1), h10 = t3 - R(2) * t2 + t;
     const R h01 = R(-2) * t3 + R(3) * t2, h11 = t3 - t2;
     const R *__restrict__ tab = static_cast<const R *>(J->nci[f]);
-    // Poisson counts of all slots from the shared uniform u1, each equal to
-    // what Giles' method returns for that slot's intensity:
-    //  - slot 0 runs the method (normal quantile W of u1 computed once); its
-    //    bottom-up summation also yields the CDF margins F(N0) - u1 and
-    //    u1 - F(N0 - 1);
-    //  - a bumped slot k takes N0 when its intensity equals slot 0's, else
-    //    its count follows from those margins moved by the intensity change,
-    //    or from slot 0's asymptotic estimate, or from rerunning the
-    //    asymptotic branch alone with the same W (bumped_count);
-    //  - anything else (and the method's rare branches) is finished by a
-    //    full inversion on wave-packed items (giles_rare).
     int Ns[NV];
     if constexpr (NV == 1) {
       Ns[0] = poisson_count_packed(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1);
     } else {
       double lams[NV];
-      bool big = false;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
+      for (int k = 0; k < NV; ++k)
         lams[k] = P[k]->Kd[f][0] * (is_t(k) ? vt[k - KT][f] : static_cast<double>(v[k][f]));
-        big = big || lams[k] > 4.0;
-        Ns[k] = 0;
-      }
-      if (u1 > 0.0) {  // u1 == 0: every count is 0
-        const double W = big ? giles_w(u1) : 0.0;
-        PoisMargin mg;
-        bool r0 = false;
-        const int n0 = static_cast<int>(poissinv_w<true>(lams[0], u1, W, &r0, &mg));
-        unsigned pend = r0 ? 1u : 0u;
-        Ns[0] = n0;
-        const double il0 = 1.0 / lams[0];
-#pragma unroll
-        for (int k = 1; k < NV; ++k) {
-          const int n = r0 ? -1 : bumped_count(lams[0], lams[k], n0, il0, W, mg);
-          if (n >= 0) Ns[k] = n;
-          else pend |= 1u << k;
-        }
-        if (__any(pend != 0u)) giles_rare<NV>(lams, u1, pend, Ns);
-      }
+      poisson_counts_lockstep<NV>(lams, u1, Ns);
     }
     // the table (chi2 quantiles of d + 2N degrees of freedom) and u2 are
     // common to all slots, so slots sharing slot 0's count share its

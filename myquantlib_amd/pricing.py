@@ -400,6 +400,21 @@ class Engine:
         self._check(rc)
         return out
 
+    def poisson_counts(self, lam: np.ndarray, u: np.ndarray) -> np.ndarray:
+        """The path kernels' Poisson counts: lam is (n, slots) with slots 1 or
+        5 sharing the uniforms u (n,); returns int32 (n, slots)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        if lam.ndim == 1:
+            lam = lam[:, None]
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(lam.shape, dtype=np.int32)
+        pd = C.POINTER(C.c_double)
+        rc = self.lib.myql_debug_poisson_counts(self.ctx, lam.ctypes.data_as(pd), lam.shape[1],
+                                                u.ctypes.data_as(pd), u.size,
+                                                out.ctypes.data_as(C.POINTER(C.c_int32)))
+        self._check(rc)
+        return out
+
     def path_payoffs(self, instr: "InstrumentSpec", mode: int, S0: float, fd_bump: float, logS) -> tuple:
         """Device trackers and payoffs on log-price paths logS (n_paths x
         (n_steps + 1)): (base, up, dn), each n_paths x n_strikes."""

@@ -90,6 +90,45 @@ def test_poissinv_device_exact(engine, oracle):
     assert np.array_equal(out, ref)
 
 
+def _near_cdf_uniforms(lam, rng, n_per):
+    """Uniforms at distances 1e-7 .. 1e-3 either side of F(n) for the counts
+    around the mean: where the FP32 first pass must defer to the FP64 method."""
+    ks = np.arange(max(0, int(lam - 3 * np.sqrt(lam) - 2)), int(lam + 3 * np.sqrt(lam) + 3))
+    c = stats.poisson.cdf(ks, lam)
+    d = np.concatenate([10.0 ** rng.uniform(-7, -3, n_per), -(10.0 ** rng.uniform(-7, -3, n_per))])
+    u = (c[:, None] + d[None, :]).ravel()
+    return u[(u > 0) & (u < 1)]
+
+
+def test_poisson_first_pass_counts_equal_giles(engine, oracle):
+    """The path kernels' Poisson counts (FP32 first pass, FP64 method for the
+    undecided) against the oracle's Giles inversion, on random (lambda, u)
+    and on uniforms placed within 1e-7..1e-3 of the CDF values, for one slot
+    and for five slots sharing u (base and bumps of 0, +-1e-5 and +-1e-2)."""
+    rng = np.random.default_rng(11)
+    lam = np.concatenate([rng.uniform(0, 4, 20000), rng.uniform(4, 40, 20000), 10 ** rng.uniform(1.6, 3, 5000),
+                          [4.0, 4.0000001, 10.0, 20.0]])
+    u = rng.uniform(0, 1, lam.size)
+    u[:30] = 0.0
+    u[30:60] = 1e-300
+    lams, us = [lam], [u]
+    for l0 in (0.3, 2.0, 3.99, 4.01, 6.5, 9.0, 12.0, 25.0, 60.0, 250.0):
+        uu = _near_cdf_uniforms(l0, rng, 40)
+        lams.append(np.full(uu.size, l0) * np.exp(rng.uniform(-0.05, 0.05, uu.size)))
+        us.append(uu)
+    lam, u = np.concatenate(lams), np.concatenate(us)
+    ref = np.array([oracle.poissinvcdf(a, b) for a, b in zip(lam, u)]).astype(np.int32)
+    got = engine.poisson_counts(lam, u)[:, 0]
+    bad = np.nonzero(got != ref)[0]
+    assert bad.size == 0, (lam[bad[:5]], u[bad[:5]], got[bad[:5]], ref[bad[:5]])
+    bumps = np.array([0.0, 1e-5, -1e-5, 1e-2, -1e-2])
+    L5 = lam[:, None] * (1.0 + bumps[None, :])
+    got5 = engine.poisson_counts(L5, u)
+    ref5 = np.array([[oracle.poissinvcdf(a, b) for a in row] for row, b in zip(L5, u)]).astype(np.int32)
+    bad = np.argwhere(got5 != ref5)
+    assert bad.size == 0, [(L5[i, k], u[i], got5[i, k], ref5[i, k]) for i, k in bad[:5]]
+
+
 def test_poissinv_device_integer_intensities(engine, oracle):
     """Integer intensities put the count X = lam in Giles' Temme correction
     with X / lam = 1 exactly (jump means lambda T = 10, 20, ...). u around
