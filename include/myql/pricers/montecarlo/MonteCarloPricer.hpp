@@ -54,11 +54,12 @@ inline myql_mc_config to_capi(const MonteCarloConfig &c) {
 }
 }  // namespace myql::detail
 
+// RNG is accepted for source compatibility only (the reference's default is
+// boost::random::xoshiro256pp, which callers may spell out): whatever it
+// names, the GPU engine draws from per-lane xoshiro256++ streams.
 template <typename Model, typename Stepper, typename Instrument, GreekMode Mode = GreekMode::None,
           typename RNG = myql::xoshiro256pp>
 class MonteCarloPricer {
-  static_assert(std::is_same_v<RNG, myql::xoshiro256pp>,
-                "the GPU engine draws from per-lane xoshiro256++ streams only");
   static_assert(std::is_same_v<typename Stepper::Policy, typename Instrument::Tracker>,
                 "the stepper's tracker does not match the instrument's");
 

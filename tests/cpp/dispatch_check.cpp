@@ -1,16 +1,15 @@
-// The reference's runtime dispatcher (myql/dispatcher/*.hpp: build_model,
-// build_instrument, build_european, price_mc, price_fourier, DispatchResult)
-// compiled unchanged over the drop-in headers and linked to the GPU engine.
-// price_mc / price_fourier visit every (model, instrument) pair of AnyModel
-// x AnyMCInstrument / AnyEuropeanInstrument, so compiling this file
-// instantiates the drop-in MonteCarloPricer and FourierPricer for all of
-// them. Each case prints one JSON line; the Python binding of the same
-// C-ABI must give the same numbers.
-//
-// Built only where the reference's headers are present (the dispatcher is
-// its code, not this repository's): the include order puts include/ (the
-// drop-in) first, so every myql/ header the dispatcher includes except
-// myql/dispatcher/ resolves to the drop-in.
+// Caller of the runtime dispatcher (myql/dispatcher/*.hpp: build_model,
+// build_instrument, build_european, price_mc, price_fourier, DispatchResult),
+// linked to the GPU engine. Built twice: over the drop-in's own dispatcher
+// (tests/cpp/dispatch_check_dropin, include/ alone on the include path) and,
+// where the reference's headers are present, over the reference's
+// dispatcher headers compiled where they lie with every other myql/ header
+// from the drop-in (oracle/_ref/dispatch_check). price_mc / price_fourier
+// take every (model, instrument) pair of AnyModel x AnyMCInstrument /
+// AnyEuropeanInstrument, so the reference build instantiates the drop-in
+// MonteCarloPricer and FourierPricer for all of them. Each case prints one
+// JSON line; the Python binding of the same C-ABI must give the same
+// numbers.
 #include <myql/dispatcher/PricingDispatch.hpp>
 
 #include <cstdio>
@@ -44,7 +43,7 @@ void emit(const std::string &name, const DispatchResult &r) {
     put_vec(f ? "vegas_std_err1" : "vegas_std_err0", r.vegas_std_err[f]);
   put_vec("thetas_std_err", r.thetas_std_err);
   put_vec("rhos_std_err", r.rhos_std_err);
-  std::printf("}\n");
+  std::printf(", \"time_ms\": %.17g}\n", r.time_ms);
 }
 
 void emit_error(const std::string &name, const std::exception &e) {

@@ -119,19 +119,36 @@ REF_INCLUDE = "/root/reference/include"
 
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REF_INCLUDE, "myql", "dispatcher")),
                     reason="needs the reference's dispatcher headers")
-def test_reference_dispatcher_needs_only_dropin_headers():
-    """The reference's runtime dispatcher (myql/dispatcher/*.hpp), preprocessed
-    with include/ first and the reference's include/ second, takes every
-    other myql/ header it includes from the drop-in: the reference supplies
-    only the dispatcher itself (oracle/_ref/dispatch_check, run by
-    test_gpu_dispatcher.py, is that build)."""
+def test_reference_dispatcher_needs_only_dropin_headers(tmp_path):
+    """The reference's runtime dispatcher (myql/dispatcher/*.hpp), reached
+    through links placed first on the include path (as oracle/Makefile
+    builds oracle/_ref/dispatch_check), takes every other myql/ header it
+    includes from the drop-in: the reference supplies only the dispatcher."""
+    stage = tmp_path / "myql" / "dispatcher"
+    stage.mkdir(parents=True)
+    ref_disp = os.path.join(REF_INCLUDE, "myql", "dispatcher")
+    for h in os.listdir(ref_disp):
+        os.symlink(os.path.join(ref_disp, h), stage / h)
     src = os.path.join(ROOT, "tests", "cpp", "dispatch_check.cpp")
-    out = subprocess.run(["g++", "-std=c++20", "-M", "-I", os.path.join(ROOT, "include"), "-I", REF_INCLUDE, src],
+    out = subprocess.run(["g++", "-std=c++20", "-M", "-I", str(tmp_path), "-I", os.path.join(ROOT, "include"),
+                          "-I", REF_INCLUDE, src], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    deps = out.stdout.replace("\\\n", " ").split()
+    staged = sorted(os.path.basename(d) for d in deps if d.startswith(str(tmp_path)))
+    assert staged == sorted(os.listdir(ref_disp)), staged
+    assert not [d for d in deps if d.startswith(REF_INCLUDE)]
+
+
+def test_dropin_dispatcher_is_self_contained():
+    """The drop-in's own dispatcher (include/myql/dispatcher) compiles from
+    include/ alone: every myql/ header the caller reaches is the drop-in's."""
+    src = os.path.join(ROOT, "tests", "cpp", "dispatch_check.cpp")
+    out = subprocess.run(["g++", "-std=c++20", "-M", "-I", os.path.join(ROOT, "include"), src],
                          capture_output=True, text=True)
     assert out.returncode == 0, out.stderr
-    deps = [d for d in out.stdout.replace("\\\n", " ").split() if d.startswith(REF_INCLUDE)]
-    assert deps, "the dispatcher headers were not found"
-    assert all(os.path.dirname(d) == os.path.join(REF_INCLUDE, "myql", "dispatcher") for d in deps), deps
+    deps = [d for d in out.stdout.replace("\\\n", " ").split() if "/myql/" in d]
+    assert any(d.endswith("myql/dispatcher/PricingDispatch.hpp") for d in deps)
+    assert all(d.startswith(os.path.join(ROOT, "include")) for d in deps), deps
 
 
 @pytest.mark.parametrize("snippet,needle", [

@@ -1,13 +1,13 @@
-"""The reference's runtime dispatcher over the drop-in headers, on the GPU.
-
-oracle/_ref/dispatch_check is the reference's myql/dispatcher/*.hpp
-(build_model, build_instrument, build_european, price_mc, price_fourier,
-DispatchResult) compiled unchanged over include/myql and linked to the
-engine, with tests/cpp/dispatch_check.cpp as its caller. It is built by
-__graft_entry__.build() where the reference's headers are present; these
-tests skip where it was not built. Every case must give exactly the numbers
-the Python binding of the same C-ABI gives for the same call, and SchemeNV
-outside its domain must surface the reference's exception text.
+"""The runtime dispatcher on the GPU: the drop-in's own
+include/myql/dispatcher (tests/cpp/dispatch_check_dropin) and, where the
+reference's headers are present, the reference's myql/dispatcher/*.hpp
+compiled unchanged over the drop-in's other headers
+(oracle/_ref/dispatch_check). Both are built by __graft_entry__.build() from
+tests/cpp/dispatch_check.cpp (build_model, build_instrument, build_european,
+price_mc, price_fourier, DispatchResult). Every case must give exactly the
+numbers the Python binding of the same C-ABI gives for the same call;
+SchemeNV outside its domain must surface the reference's exception text;
+the drop-in sets DispatchResult::time_ms, which the reference leaves 0.
 """
 from __future__ import annotations
 
@@ -22,25 +22,38 @@ from myquantlib_amd import capi
 from myquantlib_amd.pricing import Heston, InstrumentSpec, kou, make_model, mc_config, merton
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "oracle", "_ref", "dispatch_check")
+BINS = {"reference": os.path.join(ROOT, "oracle", "_ref", "dispatch_check"),
+        "dropin": os.path.join(ROOT, "tests", "cpp", "dispatch_check_dropin")}
 S0, R, Q = 100.0, 0.02, 0.0
 H1 = Heston(2.0, 0.04, 0.3, -0.7, 0.04)
 H2 = Heston(1.0, 0.03, 0.25, -0.5, 0.02)
 H_BATES = Heston(2.0, 0.04, 1.0, -0.7, 0.04)
 STRIP = dict(strikes=[90.0, 100.0, 110.0], maturity=0.5)
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.path.exists(BIN),
-                                 reason="oracle/_ref/dispatch_check is built only where the reference's "
-                                        "dispatcher headers are present")]
+pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def cases():
-    print("running oracle/_ref/dispatch_check", flush=True)
-    out = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+def _run(which):
+    b = BINS[which]
+    if not os.path.exists(b):
+        pytest.skip(f"{os.path.relpath(b, ROOT)} not built (the reference's dispatcher is built only where its "
+                    "headers are present)")
+    print(f"running {os.path.relpath(b, ROOT)}", flush=True)
+    out = subprocess.run([b], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     return {d["case"]: d for d in (json.loads(line) for line in out.stdout.splitlines())}
+
+
+@pytest.fixture(scope="module", params=["dropin", "reference"])
+def cases(request):
+    return _run(request.param)
+
+
+def test_dropin_sets_time_ms():
+    got = _run("dropin")
+    for name, d in got.items():
+        if "error" not in d and name != "scheme_nv_violation":
+            assert d["time_ms"] > 0.0, name
 
 
 def same(a, b):
