@@ -58,6 +58,13 @@ FLOP_PER_PATH_STEP = 103.69
 # raise the count:
 #   python oracle/count_flops.py --paths-log2 16 --reference-nci-rows -> 143.61
 FLOP_PER_PATH_STEP_REF_ROWS = 143.61
+# The same count for the batched Double-Heston workload (each set priced as
+# the reference's two calls, weighted by the sets' path counts) and for the
+# MLMC workload (per path-step as the engine counts them: fine plus coarse):
+#   python oracle/count_flops.py --workload batched -> 137.35
+#   python oracle/count_flops.py --workload mlmc    -> 45.53
+FLOP_PER_PATH_STEP_BATCHED = 137.35
+FLOP_PER_PATH_STEP_MLMC = 45.53
 # MI355X published FP32 vector peak (packed dual-issue FP32; scalar FP32 and
 # FP64 vector issue peak at 78.6 TFLOP/s)
 FP32_PEAK_TFLOPS = 157.3
@@ -373,11 +380,17 @@ def batched_leg(eng, n_sets: int = 1024, steps: int = 252, flags: int = 0):
     n_exact = sum(s.count(capi.SCHEME_EXACT) for _h, s, _p in sets)
     print(f"# batched double heston: {len(J)} calls, {wall:.0f} ms, {ps:.3e} path-steps",
           file=sys.stderr, flush=True)
+    dev_s = res[0]["device_ms"] / 1e3
+    achieved = ps * FLOP_PER_PATH_STEP_BATCHED / dev_s / 1e12
     return {"workload": f"{n_sets} Double-Heston sets x (arithmetic Asian call K 100, down-and-out "
                         f"call K 100 B 80), T 1, {steps} steps, 2^U[14,20] paths per set, price only, "
                         "FP32 paths, one batch",
             "calls": len(J), "paths": int(sum(p for _h, _s, p in sets)), "path_steps": ps,
+            "simulated_path_steps": ps // 2,
             "ms": wall, "device_ms": res[0]["device_ms"], "path_steps_per_s": ps / (wall / 1e3),
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS, "flop_per_path_step": FLOP_PER_PATH_STEP_BATCHED,
+                         "basis": "the reference's two calls per set (path_steps)"},
             "factors_exact": n_exact, "factors_nv": 2 * n_sets - n_exact,
             "price_range": [float(min(r["price"][0] for r in res)), float(max(r["price"][0] for r in res))]}
 
@@ -532,6 +545,11 @@ def mlmc_leg(eng):
             "epsilon": eps, "price": res["price"], "price_std_err": res["price_std_err"],
             "ms": wall, "device_ms": res["device_ms"], "total_path_steps": res["total_steps"],
             "path_steps_per_s": res["total_steps"] / (wall / 1e3),
+            "roofline": {"bound": "valu", "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "achieved": res["total_steps"] * FLOP_PER_PATH_STEP_MLMC / (res["device_ms"] / 1e3) / 1e12,
+                         "frac": res["total_steps"] * FLOP_PER_PATH_STEP_MLMC / (res["device_ms"] / 1e3) / 1e12
+                         / FP32_PEAK_TFLOPS,
+                         "flop_per_path_step": FLOP_PER_PATH_STEP_MLMC},
             "paths_per_level": [int(x) for x in res["paths_per_level"]],
             "alpha": res["alpha_estimated"], "beta": res["beta_estimated"]}
 

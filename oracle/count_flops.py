@@ -9,8 +9,10 @@ inversion, jumps, payoffs, Greek estimators and the per-strike sums -- and is
 divided by the path-steps of the call (paths x steps x 7 variants).
 
     python oracle/count_flops.py [--paths-log2 16] [--reference-nci-rows]
+    python oracle/count_flops.py --workload batched | mlmc
 
-prints one JSON line with the per-maturity and overall flops per path-step;
+prints one JSON line with the per-maturity and overall flops per path-step
+(or the batched Double-Heston or MLMC workload's);
 bench.py's FLOP_PER_PATH_STEP is this script's overall figure at 2^16 paths,
 FLOP_PER_PATH_STEP_REF_ROWS the same under the reference's NCI rows and
 fallback (MYQL_FLAG_REFERENCE_NCI_ROWS).
@@ -55,12 +57,71 @@ def count(paths: int, maturities=None, reference_rows: bool = False) -> dict:
             "per_maturity": per}
 
 
+def count_batched(n_sets: int = 1024, paths: int = 1 << 8, steps: int = 252) -> dict:
+    """The batched Double-Heston workload (bench.batched_leg): every set's
+    flops per path-step when priced as the reference's two calls (the
+    arithmetic Asian, then the down-and-out call) on `paths` paths, weighted
+    by the set's path count in the workload (its two calls' path-steps)."""
+    import bench
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec, make_model, mc_config
+    from oracle.oracle import Oracle
+
+    o = Oracle(ops=True)
+    asian = InstrumentSpec(kind=capi.INSTR_ARITH_ASIAN, strikes=[100.0], maturity=1.0)
+    dao = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=80.0, maturity=1.0,
+                         barrier_dir=capi.BARRIER_DOWN, barrier_action=capi.BARRIER_OUT)
+    flops, ps, n_exact = 0.0, 0, 0
+    for hs, schemes, set_paths in bench.double_heston_sets(1024)[:n_sets]:
+        m = make_model(2, hs)
+        cfg = mc_config(paths, steps, seed=42, stream_layout=capi.STREAMS_SEEDED, num_streams=1)
+        o.op_count(reset=True)
+        for inst in (asian, dao):
+            o.mc(m, schemes, inst, capi.GREEK_NONE, cfg, bench.S0, bench.R, bench.Q, nthreads=1)
+        rate = o.op_count(reset=True) / (2 * paths * steps)
+        flops += rate * 2 * set_paths * steps
+        ps += 2 * set_paths * steps
+        n_exact += schemes.count(capi.SCHEME_EXACT)
+    return {"workload": "batched_double_heston", "sets": n_sets, "counted_paths_per_set": paths,
+            "factors_exact": n_exact, "flop_per_path_step": flops / ps}
+
+
+def count_mlmc(n0: int = 20000) -> dict:
+    """The MLMC workload (bench.mlmc_leg: Heston NV up-and-out call, levels
+    0..6 on base 4 steps) with N0 paths per level and a tolerance loose
+    enough that no level adds paths, per path-step as the engine counts them
+    (level 0: base steps; level l: the fine and coarse steps,
+    base 2^l x 3/2)."""
+    import bench
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import Heston, InstrumentSpec, make_model, mlmc_config
+    from oracle.oracle import Oracle
+
+    o = Oracle(ops=True)
+    m = make_model(1, [Heston(2.0, 0.04, 0.3, -0.7, 0.04)])
+    inst = InstrumentSpec(kind=capi.INSTR_BARRIER, strikes=[100.0], barrier=130.0, maturity=1.0,
+                          barrier_dir=capi.BARRIER_UP, barrier_action=capi.BARRIER_OUT)
+    cfg = mlmc_config(epsilon=1.0, L_min=6, L_max=6, N0=n0, base_steps=4, seed=42,
+                      stream_layout=capi.STREAMS_SEEDED, num_streams=1)
+    o.op_count(reset=True)
+    res = o.mlmc(m, [capi.SCHEME_NV], inst, capi.GREEK_NONE, cfg, bench.S0, bench.R, bench.Q, nthreads=1)
+    ops = o.op_count(reset=True)
+    return {"workload": "mlmc_up_and_out", "paths_per_level": [int(x) for x in res["paths_per_level"]],
+            "path_steps": int(res["total_steps"]), "flop_per_path_step": ops / res["total_steps"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--paths-log2", type=int, default=16)
     ap.add_argument("--reference-nci-rows", action="store_true")
+    ap.add_argument("--workload", choices=["strip", "batched", "mlmc"], default="strip")
     args = ap.parse_args()
-    print(json.dumps(count(1 << args.paths_log2, reference_rows=args.reference_nci_rows)))
+    if args.workload == "batched":
+        print(json.dumps(count_batched()))
+    elif args.workload == "mlmc":
+        print(json.dumps(count_mlmc()))
+    else:
+        print(json.dumps(count(1 << args.paths_log2, reference_rows=args.reference_nci_rows)))
 
 
 if __name__ == "__main__":

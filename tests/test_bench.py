@@ -50,3 +50,14 @@ def test_more_than_one_gpu_is_refused():
                          text=True, timeout=120)
     assert out.returncode == 2
     assert "one GPU" in json.loads(out.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_flop_counts_of_the_batched_and_mlmc_legs():
+    """bench.py's flops per path-step of the batched Double-Heston and MLMC
+    legs are what the oracle's op-counting build gives for those workloads
+    (oracle/count_flops.py --workload batched / mlmc)."""
+    import bench
+    from oracle.count_flops import count_batched, count_mlmc
+    assert abs(count_mlmc()["flop_per_path_step"] - bench.FLOP_PER_PATH_STEP_MLMC) < 0.01
+    got = count_batched(n_sets=1024, paths=1 << 8)["flop_per_path_step"]
+    assert abs(got - bench.FLOP_PER_PATH_STEP_BATCHED) < 0.01
