@@ -302,7 +302,7 @@ int myql_fourier_calculate_batch(myql_ctx *ctx, const myql_fourier_job *jobs, in
  * every strike K_k of the strip at once (a 64-output head over the strikes,
  * shared by the call's maturities), u_k(t, X) ~ E[e^{-r(T-t)} P_k | X_t]:
  * an MLP 5 -> 64 -> 64 -> 64 -> 64 with SiLU activations (12,864
- * parameters), trained in FP32 on engine paths and evaluated on BF16 MFMA
+ * parameters), trained in FP32 on engine paths and evaluated on FP16 MFMA
  * (FP32 accumulate) in forward mode. Per path and strike the estimator is
  *   P_k - c_A M_A - c_V M_V - c_J M_J - c_D D
  * with zero-mean terms
@@ -421,7 +421,7 @@ int myql_debug_set_launch_shape(myql_ctx *ctx, int32_t precision, int32_t n_fact
                                 int32_t block, int32_t unroll, int32_t grid_factor,
                                 int32_t four_wave);
 /* Control-variate network on n caller feature rows (n x 5, scaled
- * features): the BF16 MFMA forward-mode tangents of the 64 outputs along
+ * features): the FP16 MFMA forward-mode tangents of the 64 outputs along
  * f1, f2, f3 used by the pricing kernel (out: n x 64 x 3), and the FP32
  * training gradient of the mean over rows and outputs of (net - target)^2
  * (targets n x 64; n a multiple of 32; grad: MYQL_CV_NPARAM; loss: the

@@ -32,7 +32,7 @@
 //                       grid
 //   cv_moments_kernel   per-strike target scales of the training paths
 //   cv_eval_kernel      per 16 paths and control point: forward-mode
-//                       tangents of the hidden layers on BF16 MFMA (FP32
+//                       tangents of the hidden layers on FP16 MFMA (FP32
 //                       accumulate) summed against the block increments,
 //                       the heads once per path, then per (path, strike)
 //                       the estimator (pricing) or the regressors' cross
@@ -47,16 +47,14 @@
 
 namespace myql {
 
-__device__ __forceinline__ short bf16_bits(float x) {
-  const __bf16 b = static_cast<__bf16>(x);
-  return __builtin_bit_cast(short, b);
-}
+// FP16 bit pattern of x (round to nearest even)
+__device__ __forceinline__ short h16_bits(float x) { return __builtin_bit_cast(short, static_cast<_Float16>(x)); }
 
 typedef short cv_v4s __attribute__((ext_vector_type(4)));
 typedef float cv_v4f __attribute__((ext_vector_type(4)));
 
 // logistic function with the hardware reciprocal (1 ulp; the network runs
-// in BF16 anyway)
+// in FP16 on the matrix cores anyway)
 __device__ __forceinline__ float sigmoidf(float a) { return __builtin_amdgcn_rcpf(1.0f + __expf(-a)); }
 
 __device__ __forceinline__ double norm_cdf(double x) { return 0.5 * erfc(-x * 0.70710678118654752440); }
@@ -348,7 +346,7 @@ __global__ void __launch_bounds__(256) cv_moments_kernel(CvSet set, double *__re
 // ---------------------------------------------------------------------------
 // network weights in LDS for the MFMA evaluation
 // ---------------------------------------------------------------------------
-// All products run on v_mfma_f32_16x16x16bf16_1k in the transposed form
+// All products run on v_mfma_f32_16x16x16_f16 in the transposed form
 // out^T = W x^T (16 units x 16 rows per tile): the accumulator of output
 // tile m (lane l holds units 16m + 4(l/16) + 0..3 of row l % 16) is exactly
 // the B fragment of input chunk m of the next product, so activations never
@@ -357,7 +355,12 @@ __global__ void __launch_bounds__(256) cv_moments_kernel(CvSet set, double *__re
 // the columns of W1 seed the tangents. hd[0..3): the head's rows W4[k]
 // (fitting, checks) or, for pricing, W4[k] scaled by xs c_A,k, xs c_V,k and
 // xs c_J,k, so that a head product is the coefficient-weighted tangent.
-constexpr int CV_P1 = 20;  // bf16 pitch of the 16-column matrix W1
+// The value pass runs in base-2 units: pre-activations y' = y log2(e) (W1,
+// b1, b2, b3 scaled by log2(e)), so the sigmoid is 1 / (1 + 2^-y') with no
+// scaling multiply, and the activations it passes on are h' = h log2(e),
+// which W2 and W3 unscaled turn into the next y' (log2(e) ln(2) = 1).
+constexpr int CV_P1 = 20;  // FP16 pitch of the 16-column matrix W1
+constexpr float kCvLog2e = 1.44269504088896341f, kCvLn2 = 0.69314718055994531f;
 struct CvLds {
   short w2[CV_H * CV_PITCH], w3[CV_H * CV_PITCH];  // [out][in]
   short hd[3][CV_OUT * CV_PITCH];
@@ -370,33 +373,33 @@ __device__ __forceinline__ void cv_load_lds(CvLds &L, const float *__restrict__ 
                                             float xs, int nk) {
   for (int i = threadIdx.x; i < CV_H * CV_H; i += blockDim.x) {
     const int n = i / CV_H, k = i % CV_H;
-    L.w2[n * CV_PITCH + k] = bf16_bits(prm[CV_W2 + i]);
-    L.w3[n * CV_PITCH + k] = bf16_bits(prm[CV_W3 + i]);
+    L.w2[n * CV_PITCH + k] = h16_bits(prm[CV_W2 + i]);
+    L.w3[n * CV_PITCH + k] = h16_bits(prm[CV_W3 + i]);
     if (!coef) {
-      L.hd[0][n * CV_PITCH + k] = bf16_bits(prm[CV_W4 + i]);
+      L.hd[0][n * CV_PITCH + k] = h16_bits(prm[CV_W4 + i]);
     } else {
       // row n of the head is strike n
 #pragma unroll
       for (int h = 0; h < 3; ++h) {
         const float sc = n < nk ? static_cast<float>(xs * coef[h * CV_OUT + n]) : 0.0f;
-        L.hd[h][n * CV_PITCH + k] = bf16_bits(sc * prm[CV_W4 + i]);
+        L.hd[h][n * CV_PITCH + k] = h16_bits(sc * prm[CV_W4 + i]);
       }
     }
   }
   for (int i = threadIdx.x; i < CV_H * 16; i += blockDim.x) {
     const int n = i / 16, k = i % 16;
     const float w = k < CV_IN ? prm[CV_W1 + n * CV_IN + k] : (k == CV_IN ? prm[CV_B1 + n] : 0.0f);
-    L.w1[n * CV_P1 + k] = bf16_bits(w);
+    L.w1[n * CV_P1 + k] = h16_bits(w * kCvLog2e);
   }
   for (int i = threadIdx.x; i < CV_H; i += blockDim.x) {
-    L.bias[0][i] = prm[CV_B2 + i];
-    L.bias[1][i] = prm[CV_B3 + i];
+    L.bias[0][i] = prm[CV_B2 + i] * kCvLog2e;
+    L.bias[1][i] = prm[CV_B3 + i] * kCvLog2e;
 #pragma unroll
     for (int c = 0; c < 3; ++c) L.w1c[c][i] = prm[CV_W1 + i * CV_IN + 1 + c];
   }
 }
 
-typedef _Float16 cv_h4 __attribute__((ext_vector_type(4)));
+typedef cv_v4f cv_h4;  // SiLU derivatives of one output tile
 
 // Unit held by lane l in element e of output tile m.
 __device__ __forceinline__ int cv_unit(int m, int e, int l) { return 16 * m + 4 * (l >> 4) + e; }
@@ -410,8 +413,11 @@ __device__ __forceinline__ int cv_opaque_zero() {
   return z;
 }
 
+typedef _Float16 cv_v4h __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ cv_v4f cv_mfma(cv_v4s a, cv_v4s b, cv_v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(cv_v4h, a), __builtin_bit_cast(cv_v4h, b), c, 0, 0,
+                                                0);
 }
 
 // out^T = W x^T for 64 outputs and 64 inputs: acc[m] += W[16m.., :] x^T
@@ -430,26 +436,36 @@ __device__ __forceinline__ cv_v4f cv_units4(const float *__restrict__ v, int m, 
   return *reinterpret_cast<const cv_v4f *>(v + 16 * m + 4 * (l >> 4) + cv_opaque_zero());
 }
 
-__device__ __forceinline__ void cv_to_bf16(const cv_v4f (&x)[4], cv_v4s (&B)[4]) {
+typedef unsigned cv_v2u __attribute__((ext_vector_type(2)));
+typedef __fp16 cv_v2h __attribute__((ext_vector_type(2)));
+
+// four FP32 values as a B fragment: two packed conversions (round toward
+// zero), one instruction per pair
+__device__ __forceinline__ cv_v4s cv_pack4(cv_v4f x) {
+  const cv_v2h lo = __builtin_amdgcn_cvt_pkrtz(x[0], x[1]), hi = __builtin_amdgcn_cvt_pkrtz(x[2], x[3]);
+  return __builtin_bit_cast(cv_v4s, cv_v2u{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
+}
+
+__device__ __forceinline__ void cv_to_h16(const cv_v4f (&x)[4], cv_v4s (&B)[4]) {
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) B[m][e] = bf16_bits(x[m][e]);
+  for (int m = 0; m < 4; ++m) B[m] = cv_pack4(x[m]);
 }
 
 // Value pass of the row held by lanes (l % 16) + 16 q: the SiLU derivatives
-// of the three hidden layers, FP16, dd[layer][m][e] for unit cv_unit(m, e, l).
+// of the three hidden layers, dd[layer][m][e] for unit cv_unit(m, e, l).
+// With sigma = 1 / (1 + 2^-y'), h' = y' sigma: dh/dy = sigma + h (1 - sigma)
+// = sigma + ln(2) h' (1 - sigma).
 __device__ __forceinline__ void cv_value(const CvLds &L, const float (&f)[CV_IN], cv_h4 (&dd)[3][4], int l) {
   // layer 1: (f0..f4, 1) against W1 extended by b1
   cv_v4s Bf = {0, 0, 0, 0};
   if ((l >> 4) == 0) {
-    Bf[0] = bf16_bits(f[0]);
-    Bf[1] = bf16_bits(f[1]);
-    Bf[2] = bf16_bits(f[2]);
-    Bf[3] = bf16_bits(f[3]);
+    Bf[0] = h16_bits(f[0]);
+    Bf[1] = h16_bits(f[1]);
+    Bf[2] = h16_bits(f[2]);
+    Bf[3] = h16_bits(f[3]);
   } else if ((l >> 4) == 1) {
-    Bf[0] = bf16_bits(f[4]);
-    Bf[1] = bf16_bits(1.0f);
+    Bf[0] = h16_bits(f[4]);
+    Bf[1] = h16_bits(1.0f);
   }
   cv_v4f acc[4];
   {
@@ -468,14 +484,17 @@ __device__ __forceinline__ void cv_value(const CvLds &L, const float (&f)[CV_IN]
       cv_layer(layer == 1 ? L.w2 : L.w3, Bh, acc, l);
     }
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+      cv_v4f h;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const float y = acc[m][e];
-        const float sg = sigmoidf(y), h = y * sg;
-        dd[layer][m][e] = static_cast<_Float16>(sg + h * (1.0f - sg));
-        if (layer < 2) Bh[m][e] = bf16_bits(h);
+        const float sg = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-y));
+        h[e] = y * sg;
+        dd[layer][m][e] = (h[e] - h[e] * sg) * kCvLn2 + sg;
       }
+      if (layer < 2) Bh[m] = cv_pack4(h);
+    }
   }
 }
 
@@ -484,11 +503,7 @@ __device__ __forceinline__ void cv_value(const CvLds &L, const float (&f)[CV_IN]
 __device__ __forceinline__ void cv_tangent(const CvLds &L, const cv_h4 (&dd)[3][4], int i, cv_v4f (&t)[4], int l) {
   cv_v4s Bt[4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const cv_v4f w = cv_units4(L.w1c[i - 1], m, l);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) Bt[m][e] = bf16_bits(w[e] * static_cast<float>(dd[0][m][e]));
-  }
+  for (int m = 0; m < 4; ++m) Bt[m] = cv_pack4(cv_units4(L.w1c[i - 1], m, l) * dd[0][m]);
 #pragma unroll
   for (int layer = 1; layer < 3; ++layer) {
     cv_v4f acc[4];
@@ -496,13 +511,11 @@ __device__ __forceinline__ void cv_tangent(const CvLds &L, const cv_h4 (&dd)[3][
     for (int m = 0; m < 4; ++m) acc[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
     cv_layer(layer == 1 ? L.w2 : L.w3, Bt, acc, l);
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float v = acc[m][e] * static_cast<float>(dd[layer][m][e]);
-        if (layer == 1) Bt[m][e] = bf16_bits(v);
-        else t[m][e] = v;
-      }
+    for (int m = 0; m < 4; ++m) {
+      const cv_v4f v = acc[m] * dd[layer][m];
+      if (layer == 1) Bt[m] = cv_pack4(v);
+      else t[m] = v;
+    }
   }
 }
 
@@ -510,7 +523,7 @@ __device__ __forceinline__ void cv_tangent(const CvLds &L, const cv_h4 (&dd)[3][
 template <bool ADD>
 __device__ __forceinline__ void cv_head(const short *__restrict__ H, const cv_v4f (&x)[4], cv_v4f (&out)[4], int l) {
   cv_v4s B[4];
-  cv_to_bf16(x, B);
+  cv_to_h16(x, B);
   if (!ADD)
 #pragma unroll
     for (int m = 0; m < 4; ++m) out[m] = cv_v4f{0.f, 0.f, 0.f, 0.f};
@@ -952,7 +965,7 @@ __global__ void cv_step_kernel(int *step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
-// debug: BF16 MFMA tangents of n rows of features (rows beyond n padded
+// debug: FP16 MFMA tangents of n rows of features (rows beyond n padded
 // with zeros): out[(row * 64 + k) * 3 + t] = d out_k / d f_{1+t}
 __global__ void __launch_bounds__(BLOCK) cv_grad_debug_kernel(const float *__restrict__ prm,
                                                               const float *__restrict__ feats, int n,

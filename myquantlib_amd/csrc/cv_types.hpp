@@ -14,7 +14,7 @@ constexpr int CV_W1 = 0, CV_B1 = CV_W1 + CV_H * CV_IN, CV_W2 = CV_B1 + CV_H, CV_
 static_assert(CV_NPARAM == 12864, "parameter layout");
 constexpr int CV_MAXJ = 16;      // control points per path
 constexpr int CV_REC = 6;        // recorded fields per (control point, path): x, v1, v2, A, V1, V2
-constexpr int CV_PITCH = 72;     // bf16 row pitch of the LDS weight copies (bank spread)
+constexpr int CV_PITCH = 72;     // FP16 row pitch of the LDS weight copies (bank spread)
 constexpr int CV_TROWS = 32;     // rows per training block
 constexpr int CV_NREG = 4;       // regressors per strike: M_A, M_V, M_J, D
 constexpr int CV_RB_PTS = 2048;  // grid of the terminal-jump expectation tables

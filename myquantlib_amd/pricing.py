@@ -338,7 +338,7 @@ class Engine:
         return out
 
     def cv_input_grad(self, params: np.ndarray, feats: np.ndarray) -> np.ndarray:
-        """Device (BF16 MFMA) tangents of the 64 outputs along features 1..3
+        """Device (FP16 MFMA) tangents of the 64 outputs along features 1..3
         for rows of 5 features: array (n, 64, 3)."""
         p = np.ascontiguousarray(params, dtype=np.float32)
         f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 5)

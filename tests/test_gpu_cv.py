@@ -2,7 +2,7 @@
 implementation: checked against the FP64 numpy network in oracle/cv_mlp.py,
 the engine's own plain estimator on the same paths, and Fourier prices).
 
-  * the BF16 MFMA forward-mode tangents of the pricing kernel against FP64;
+  * the FP16 MFMA forward-mode tangents of the pricing kernel against FP64;
   * the FP32 training gradient against FP64 backpropagation;
   * the plain estimate of a CV run equals myql_mc_calculate on the same
     seed (same streams, partition and draws), and so do its Essential-mode
@@ -34,9 +34,11 @@ def test_mfma_tangents_match_fp64(engine):
     print("cv tangents: 200 rows", flush=True)
     g = engine.cv_input_grad(p, f)
     ref = cv_mlp.tangents(p.astype(np.float64), f.astype(np.float64))
-    # BF16 operands (8-bit mantissa) through three products
+    # FP16 operands (11-bit mantissa; the activations and tangents are
+    # converted rounding toward zero) through three products
     err = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
-    assert err < 0.03, err
+    print(f"max error / max |tangent|: {err:.2e}", flush=True)
+    assert err < 0.01, err
 
 
 def test_training_gradient_matches_fp64(engine):
