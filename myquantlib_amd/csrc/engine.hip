@@ -873,6 +873,12 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
       const double d = 4.0 * p.kappa * p.theta / (p.sigma * p.sigma);
       NciInfo ni = nci_table(c, d, nmax_max[f], cfg.precision, nci_nmax(p, dt, true));
       set_nci(J, f, ni, ref_rows);
+      if (cfg.precision == MYQL_FP32 && full) {
+        // FP64 rows for the maturity-bumped variants stepped in FP64
+        const NciInfo n64 = nci_table(c, d, nmax_max[f], MYQL_FP64, nci_nmax(p, dt, true));
+        J.nci64[f] = static_cast<const double *>(n64.tab);
+        J.nci_tail64[f] = static_cast<const double *>(n64.tail);
+      }
     }
   for (size_t i = 0; i < vd.size(); ++i)
     fill_variant(J.var[vd[i].id], vd[i].m, js.schemes, vd[i].dt, vd[i].r, q, vd[i].T, steps, pd,

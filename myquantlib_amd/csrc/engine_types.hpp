@@ -73,6 +73,10 @@ struct Job {
   int nci_rows[2];
   int nci_lg_n;
   const void *nci_tail[2];         // upper-tail rows {q, hw*dq/dw} on w = -log(1-u); null = exact
+  // FP64 copies of the main and tail rows for FP32 Full-mode jobs, read by
+  // the maturity-bumped variants those kernels step in FP64 (null otherwise)
+  const double *nci64[2];
+  const double *nci_tail64[2];
   double tail_w0, tail_inv_hw;
   int tail_pts;
   int nci_grid;

@@ -56,7 +56,9 @@ __device__ __forceinline__ double sqrt_nr(double a) {
 // maturity-bumped variants (T+ in slot 1 + 2 max(NF, 1), T- next; the
 // reference's T +- t_bump replays, MonteCarloPricer.hpp:374-395) in FP64:
 // theta and charm difference those two paths and divide by 2 t_bump
-// (2e-4), which FP32 rounding of the log-price would not survive. Their
+// (2e-4), which FP32 rounding of the log-price would not survive. Their NCI
+// variance steps read the FP64 chi-squared rows (Job::nci64), so they take
+// the FP64 kernel's Poisson counts too. Their
 // terminal spots keep FP64 precision through the FP32 stage: T+ as a
 // hi + lo pair of floats (lo in the r+ variant's slot, unused by Europeans,
 // whose rate-bumped paths are derived from the base path), T- relative to
@@ -496,18 +498,33 @@ __device__ __forceinline__ void poisson_counts_lockstep(const double (&lams)[NV]
   }
 }
 
-// NCI upper tail (u >= 0.999), row N: cubic Hermite in w = -log(1 - u) with
-// exact slopes, before the slot's scale factor
-template <typename R>
-__device__ __forceinline__ R nci_tail_value(const Job<R> *__restrict__ J, int f, int N, double u2) {
+// NCI upper tail (u >= 0.999), row N of tail rows `tail` (element type T):
+// cubic Hermite in w = -log(1 - u) with exact slopes, before the slot's
+// scale factor
+template <typename T, typename R>
+__device__ __forceinline__ T nci_tail_rows(const Job<R> *__restrict__ J, const T *__restrict__ tail, int N,
+                                           double u2) {
   double tp = (-log1p(-u2) - J->tail_w0) * J->tail_inv_hw;
   if (tp < 0.0) tp = 0.0;
   int j = static_cast<int>(tp);
   if (j >= J->tail_pts - 1) j = J->tail_pts - 2;
-  const R tt = static_cast<R>(tp - static_cast<double>(j)), tt2 = tt * tt, tt3 = tt2 * tt;
-  const R *row = static_cast<const R *>(J->nci_tail[f]) + 2 * (static_cast<size_t>(N) * J->tail_pts + j);
-  return (R(2) * tt3 - R(3) * tt2 + R(1)) * row[0] + (tt3 - R(2) * tt2 + tt) * row[1] +
-         (R(-2) * tt3 + R(3) * tt2) * row[2] + (tt3 - tt2) * row[3];
+  const T tt = static_cast<T>(tp - static_cast<double>(j)), tt2 = tt * tt, tt3 = tt2 * tt;
+  const T *row = tail + 2 * (static_cast<size_t>(N) * J->tail_pts + j);
+  return (T(2) * tt3 - T(3) * tt2 + T(1)) * row[0] + (tt3 - T(2) * tt2 + tt) * row[1] +
+         (T(-2) * tt3 + T(3) * tt2) * row[2] + (tt3 - tt2) * row[3];
+}
+
+template <typename R>
+__device__ __forceinline__ R nci_tail_value(const Job<R> *__restrict__ J, int f, int N, double u2) {
+  return nci_tail_rows<R, R>(J, static_cast<const R *>(J->nci_tail[f]), N, u2);
+}
+
+// Row N of the FP64 main table at grid interval i, fraction t
+__device__ __forceinline__ double nci_row64(const double *__restrict__ tab, int grid, int N, int i, double t) {
+  const double t2 = t * t, t3 = t2 * t;
+  const double *row = tab + 2 * (static_cast<size_t>(N) * grid + i);
+  return (2.0 * t3 - 3.0 * t2 + 1.0) * row[0] + (t3 - 2.0 * t2 + t) * row[1] + (-2.0 * t3 + 3.0 * t2) * row[2] +
+         (t3 - t2) * row[3];
 }
 
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
@@ -601,10 +618,37 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         q0 = chi2_quantile_lg(dof, u2, lg);
       }
     }
+    // exact chi-squared quantile of slot k on row N (slot 0's when shared)
+    auto exact_q = [&](int k, int N) {
+      if (ex0 && N == Ns[0] && P[k]->Kd[f][1] == P[0]->Kd[f][1]) return q0;
+      const double dof = P[k]->Kd[f][1] + 2.0 * N;
+      const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
+      return chi2_quantile_lg(dof, u2, lg);
+    };
+    // The maturity-bumped slots of FP32 kernels read the FP64 rows, so their
+    // variance paths are the FP64 kernel's: a table value rounded to FP32
+    // shifts the next step's Poisson intensity, and where the uniform lies
+    // near a CDF value the count, which would part those paths from the FP64
+    // ones and from each other.
+    double td = 0.0, y0d = 0.0;
+    if constexpr (TD) {
+      td = pos - static_cast<double>(i);
+      if (main0) y0d = nci_row64(J->nci64[f], J->nci_grid, Ns[0], i, td);
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const int N = Ns[k];
       const bool tab_k = in_tab(N, k);
+      if constexpr (TD) {
+        if (is_t(k)) {
+          double y;
+          if (tab_k && u2 < 0.999) y = (main0 && N == Ns[0]) ? y0d : nci_row64(J->nci64[f], J->nci_grid, N, i, td);
+          else if (tab_k && J->nci_tail[f] != nullptr) y = nci_tail_rows<double>(J, J->nci_tail64[f], N, u2);
+          else y = exact_q(k, N);
+          vnt[k - KT][f] = y * P[k]->Kd[f][2];
+          continue;
+        }
+      }
       if (main0 && N == Ns[0] && tab_k) {
         put(k, y0);
       } else if (tab_k && u2 < 0.999) {
@@ -613,15 +657,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       } else if (tab_k && u2 >= 0.999 && J->nci_tail[f] != nullptr) {
         put(k, (tl0 && N == Ns[0]) ? yt0 : nci_tail_value<R>(J, f, N, u2));
       } else {
-        double q;
-        if (ex0 && N == Ns[0] && P[k]->Kd[f][1] == P[0]->Kd[f][1]) {
-          q = q0;
-        } else {
-          const double dof = P[k]->Kd[f][1] + 2.0 * N;
-          const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
-          q = chi2_quantile_lg(dof, u2, lg);
-        }
-        putd(k, q);
+        putd(k, exact_q(k, N));
       }
     }
     if (n_out) *n_out = Ns[0];  // slot 0's Poisson count (control-variate recording)
@@ -651,7 +687,12 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
   }
   R x[NV], v[NV][NFX];
   Trk<R> tr[PD ? NV : 1];
-  const R x0 = static_cast<R>(J->logS0);
+  // Europeans carry the log-return x - log S0 (no tracker compares it with a
+  // log-space level) and stage S0 e^x: in FP32 a log-price near log S0 ~ 4.6
+  // would round every path's start by up to 2.4e-7 -- one systematic shift of
+  // all terminal spots -- and each step's sum by as much again, where the
+  // log-return's rounding scales with its own size.
+  const R x0 = PD ? static_cast<R>(J->logS0) : R(0);
   // FP64 state of the maturity-bumped slots (t_slots_fp64)
   constexpr bool TD = t_slots_fp64<R, NV, NFX, PD>();
   constexpr int KT = t_slot0<NFX>();
@@ -666,7 +707,7 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
   if constexpr (TD) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      xt[i] = static_cast<double>(x0);
+      xt[i] = 0.0;
 #pragma unroll
       for (int f = 0; f < NFX; ++f) vt[i][f] = P[KT + i]->v0d[f];
     }
@@ -736,7 +777,7 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     if (TD && k == KT) {
-      const double S = exp(xt[0]);
+      const double S = J->S0 * exp(xt[0]);
       const R hi = static_cast<R>(S);
       stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = hi;
       stage[(F_SPOT * MAXV + V_RUP) * BLK + tid] = static_cast<R>(S - static_cast<double>(hi));
@@ -749,7 +790,7 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
     if constexpr (PD) {
       finalize_stage<R, BLK>(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
     } else {
-      stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = dexp<R>(x[k]);
+      stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = static_cast<R>(J->S0) * dexp<R>(x[k]);
     }
   }
 }

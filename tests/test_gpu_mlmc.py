@@ -55,8 +55,6 @@ CASES = {
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_mlmc_equals_plain_mc_at_finest_level(engine, name, prec):
-    if prec == capi.FP32 and name not in ("heston_up_and_out", "bates_european_nci"):
-        pytest.skip("FP32 paths checked on the MLMC workload's barrier and the NCI European")
     m, sch, inst = CASES[name]
     base, L = 4, 3
     steps = base << L

@@ -199,11 +199,14 @@ def test_bates_nci_full_strip(engine, oracle, prec, tol, flags):
 # FP32 paths against the FP64 oracle on identical draws, every Full field:
 # max over options (price >= 0.1) of |fp32 - fp64| / max(|fp64|, its SE).
 # The maturity-bumped variants behind theta and charm are stepped in FP64
-# inside the FP32 kernel; what remains is paths that part ways where FP32
-# rounding of a table value moves a Poisson count across a CDF boundary
-# (noise, largest in the most amplified estimators: gamma, vanna, charm).
-FP32_FULL_BOUNDS = {"price": 1e-4, "delta": 1e-4, "rho": 1e-4, "vega": 1e-3, "theta": 0.05, "gamma": 0.05,
-                    "vanna": 0.05, "charm": 0.2}
+# inside the FP32 kernel from the FP64 chi-squared rows, so their paths are
+# the FP64 ones (theta and charm differ by ~1e-4 of SE at the full strip).
+# The base and v0-bumped paths keep FP32 variance: where FP32 rounding of a
+# table value moves a Poisson count across a CDF boundary their paths part
+# ways from the FP64 ones (noise, largest in the most amplified estimators:
+# gamma and vanna).
+FP32_FULL_BOUNDS = {"price": 1e-4, "delta": 1e-4, "rho": 1e-4, "vega": 1e-3, "theta": 0.01, "gamma": 0.05,
+                    "vanna": 0.05, "charm": 0.01}
 
 
 def test_bates_nci_full_strip_fp32_every_field(engine, oracle):
