@@ -3,7 +3,11 @@
 // Giles level loop runs on the host inside myql_mlmc_calculate; every
 // level's coupled fine/coarse paths run in the HIP level kernels. Scalar
 // instruments only, as in the reference (the level differences are
-// scalars); the Greek members of the result stay zero. Link with -lmyql_hip.
+// scalars). Under GreekMode::Essential and Full, delta and gamma are filled
+// with their multilevel estimates (the reference leaves every Greek member
+// zero); Full mode's other Greek members stay zero. MLMCConfig::estimator
+// selects the multilevel Richardson-Romberg estimator (no counterpart in the
+// reference). Link with -lmyql_hip.
 #pragma once
 #include <myql_capi.h>
 
@@ -32,6 +36,7 @@ struct MLMCConfig {
   int stream_layout = MYQL_STREAMS_JUMP;
   size_t num_streams = 0;
   unsigned flags = 0;
+  int estimator = MYQL_MLMC_GILES;  // or MYQL_MLMC_ML2R
 };
 
 template <typename Model, typename CoupledStepperType, typename Instrument,
@@ -69,6 +74,7 @@ class MultiLevelMonteCarloPricer {
     c.stream_layout = cfg_.stream_layout;
     c.num_streams = cfg_.num_streams;
     c.flags = cfg_.flags;
+    c.estimator = cfg_.estimator;
     std::vector<uint64_t> paths(cfg_.L_max + 1, 0);
     std::vector<double> means(cfg_.L_max + 1, 0.0), vars(cfg_.L_max + 1, 0.0);
     myql_mlmc_result res{};
@@ -85,6 +91,12 @@ class MultiLevelMonteCarloPricer {
     out.total_steps = static_cast<size_t>(res.total_steps);
     out.alpha_estimated = res.alpha_estimated;
     out.beta_estimated = res.beta_estimated;
+    if constexpr (Mode != GreekMode::None) {
+      out.delta = res.delta;
+      out.delta_std_err = res.delta_std_err;
+      out.gamma = res.gamma;
+      out.gamma_std_err = res.gamma_std_err;
+    }
     for (int l = 0; l < res.n_levels; ++l) {
       out.paths_per_level.push_back(static_cast<size_t>(paths[l]));
       out.mean_diffs.push_back(means[l]);

@@ -194,7 +194,25 @@ typedef struct {
   myql_mc_result *result;
 } myql_mc_job;
 
-/* ---- MLMCConfig (MultiLevelMonteCarloPricer.hpp:16-30) ----------------- */
+/* ---- MLMCConfig (MultiLevelMonteCarloPricer.hpp:16-30) -----------------
+ * estimator (no counterpart in the reference, whose README.md:332-335 lists
+ * Richardson-Romberg and ML2R as future work):
+ *  MYQL_MLMC_GILES  the reference's multilevel estimator: sum of the level
+ *      means, N_l from the level variances, levels added until the last
+ *      level's mean / (2^alpha - 1) is below epsilon / sqrt 2
+ *      (MultiLevelMonteCarloPricer.hpp:46-241).
+ *  MYQL_MLMC_ML2R   multilevel Richardson-Romberg (Lemaire & Pages, 2017)
+ *      on the same coupled levels (refiners n_l = 2^l): the level means are
+ *      weighted by W_l = sum_{j >= l} w_j, where w solves the Vandermonde
+ *      system sum_j w_j = 1, sum_j w_j n_j^(-alpha k) = 0 (k = 1..L) with
+ *      alpha = the config's alpha (the weak order), so the bias terms
+ *      h^alpha .. h^(alpha L) of the finest level cancel; N_l minimises the
+ *      cost at variance epsilon^2 / 2 of the weighted sum; levels are added
+ *      until two successive depths' estimates differ by less than
+ *      epsilon / sqrt 2. L_min = L_max = 1 is the Richardson-Romberg
+ *      extrapolation 2^alpha/(2^alpha - 1) P_fine - P_coarse/(2^alpha - 1). */
+enum { MYQL_MLMC_GILES = 0, MYQL_MLMC_ML2R = 1 };
+
 typedef struct {
   double epsilon;
   uint64_t L_min, L_max, N0, base_steps;
@@ -206,10 +224,21 @@ typedef struct {
   int32_t stream_layout;
   uint64_t num_streams;
   uint32_t flags;        /* MYQL_FLAG_* */
+  int32_t estimator;     /* MYQL_MLMC_GILES (0) or MYQL_MLMC_ML2R */
 } myql_mlmc_config;
 
-/* MlmcResult<None, double> (PricingTypes.hpp:102-115). The three arrays are
- * caller-owned with L_max + 1 entries; n_levels reports how many are used. */
+/* MlmcResult<Mode, double> (PricingTypes.hpp:102-140). The arrays are
+ * caller-owned with L_max + 1 entries (level_weights may be NULL); n_levels
+ * reports how many are used. Under GreekMode::Essential and Full, delta
+ * and gamma are the multilevel estimates of the spot-bumped payoffs'
+ * central differences -- from the up and down payoff buffers the
+ * reference's evaluate_level computes and drops
+ * (MultiLevelMonteCarloPricer.hpp:321-324) -- on the price's level sample
+ * sizes, with their standard errors; under None they are 0, and the
+ * reference fills none of them in any mode (no counterpart). Full mode's
+ * other Greeks are not estimated by either (the drop-in returns them as 0,
+ * as the reference does). level_weights: the weight of each level's mean in
+ * the estimate (1 for MYQL_MLMC_GILES). */
 typedef struct {
   double price, price_std_err;
   uint64_t total_steps;
@@ -218,6 +247,8 @@ typedef struct {
   double *mean_diffs, *var_diffs;
   double alpha_estimated, beta_estimated;
   double device_ms;
+  double delta, delta_std_err, gamma, gamma_std_err;
+  double *level_weights;
 } myql_mlmc_result;
 
 typedef struct myql_ctx myql_ctx;

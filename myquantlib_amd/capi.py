@@ -26,6 +26,7 @@ FP64, FP32 = range(2)
 STREAMS_JUMP, STREAMS_SEEDED = range(2)
 FLAG_REFERENCE_NCI_ROWS = 1
 FLAG_AUTOTUNE = 2
+MLMC_GILES, MLMC_ML2R = range(2)
 
 
 class HestonParams(C.Structure):
@@ -97,7 +98,8 @@ class MLMCConfig(C.Structure):
                 ("N0", C.c_uint64), ("base_steps", C.c_uint64), ("alpha", C.c_double),
                 ("beta", C.c_double), ("adaptive_rates", C.c_int32), ("seed", C.c_uint64),
                 ("fd_bump", C.c_double), ("precision", C.c_int32),
-                ("stream_layout", C.c_int32), ("num_streams", C.c_uint64), ("flags", C.c_uint32)]
+                ("stream_layout", C.c_int32), ("num_streams", C.c_uint64), ("flags", C.c_uint32),
+                ("estimator", C.c_int32)]
 
 
 class MLMCResult(C.Structure):
@@ -105,7 +107,10 @@ class MLMCResult(C.Structure):
                 ("total_steps", C.c_uint64), ("n_levels", C.c_int32),
                 ("paths_per_level", C.POINTER(C.c_uint64)), ("mean_diffs", _PD),
                 ("var_diffs", _PD), ("alpha_estimated", C.c_double),
-                ("beta_estimated", C.c_double), ("device_ms", C.c_double)]
+                ("beta_estimated", C.c_double), ("device_ms", C.c_double),
+                ("delta", C.c_double), ("delta_std_err", C.c_double),
+                ("gamma", C.c_double), ("gamma_std_err", C.c_double),
+                ("level_weights", _PD)]
 
 
 CV_NPARAM, CV_MAX_POINTS, CV_MAX_JOBS, CV_NREG = 12864, 16, 16, 4

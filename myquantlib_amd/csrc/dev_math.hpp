@@ -193,22 +193,21 @@ __device__ __forceinline__ void giles_asym_S(double lam, double ilam, double W, 
   S = lam + (S + Del);
 }
 
-// Distances of U from the CDF values around the returned count N, from the
-// bottom-up summation: up = F(N) - U >= 0, lo = U - F(N-1) > 0 (lo = U for
-// N = 0), and the probabilities p(N), p(N-1). valid = false when the count
-// came from elsewhere (asymptotic branch, rounding-ambiguous correction).
-// asym = true when the central asymptotic branch alone decided N = floor(S),
-// with S and its error term Del kept.
+// What the FP32 first pass (poisson_first32) knows about its count N, for
+// the bumped slots' counts (bumped_count). valid: N came from the bottom-up
+// sum, with up = F(N) - U >= 0, lo = U - F(N-1) > 0 (lo = U for N = 0), the
+// probabilities p(N), p(N-1).
+// The pass returns such an N only where both margins exceed kSumTol, far
+// above the FP32 sum's own error. asym: N = floor(S) from the central
+// asymptotic branch, with S (within s_tol of the FP64 value) and its error
+// term Del. Wf: that pass's W = Phi^-1(U) in FP32. Stored in FP32: the
+// pass's error bounds are far above FP32 rounding of these values.
+constexpr float kSumTol = 3e-5f;
 struct PoisMargin {
-  bool valid;
-  double up, lo, p_hi, p_lo;
-  bool asym;
-  double S, Del;
-  // error allowances of the margins and of S (FP64 pass: rounding; FP32
-  // first pass: its error bounds), and that pass's W in FP32
-  double slack = 1e-12, s_tol = 1e-9;
-  bool fp32 = false;
-  float Wf = 0.0f;
+  bool valid, asym;
+  float up, lo, p_hi, p_lo;
+  float S, Del, s_tol;
+  float Wf;
 };
 
 // DEFER: instead of running the rare branches (the tail Newton iteration for
@@ -216,12 +215,7 @@ struct PoisMargin {
 // *rare and return early; the caller then runs the full inversion for those
 // cases (poissinv), so the result is the same.
 template <bool DEFER = false>
-__device__ __forceinline__ double poissinv_w(double lam, double U, double W, bool *rare = nullptr,
-                                             PoisMargin *mg = nullptr) {
-  if (mg) {
-    mg->valid = false;
-    mg->asym = false;
-  }
+__device__ __forceinline__ double poissinv_w(double lam, double U, double W, bool *rare = nullptr) {
   if (U <= 0.0) return 0.0;
   if (U >= 1.0) return __builtin_nan("");
   const double V = 1.0 - U;
@@ -257,11 +251,6 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
       S = S + Del;
     }
     X = floor(S);
-    if (mg && fabs(W) < 3.0 && X >= 10.0 && !(S < X + 2.0 * Del)) {
-      mg->asym = true;
-      mg->S = S;
-      mg->Del = Del;
-    }
     if (S > 10.0 && S < X + 2.0 * Del) {
       if constexpr (DEFER) {
         *rare = true;
@@ -343,30 +332,12 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
     double Tt = exp(0.5 * lam);
     Del = 0.0;
     if (U > 0.5) Del = Tt * (1e-13 * Tt);
-    const double T0 = Tt;
     double Sm = 1.0 - Tt * (U * Tt) + Del;
-    // Sm_n = (n!/lam^n) e^lam (F(n) - U): Pr tracks n!/lam^n
-    double Smp = Sm, Pr = 1.0;
     while (Sm < 0.0) {
       X += 1.0;
       Tt = X * ilam;
       Del = Tt * Del;
-      Smp = Sm;
-      Pr *= Tt;
       Sm = Tt * Sm + 1.0;
-    }
-    if (mg && !(Sm < 2.0 * Del)) {
-      const double ph = 1.0 / (Pr * T0 * T0);  // p(N)
-      mg->valid = true;
-      mg->p_hi = ph;
-      mg->up = Sm * ph;
-      if (X > 0.0) {
-        mg->p_lo = ph * X * ilam;  // p(N-1)
-        mg->lo = -Smp * mg->p_lo;
-      } else {
-        mg->p_lo = 0.0;
-        mg->lo = U;
-      }
     }
     if (Sm < 2.0 * Del) {
       Del = 1e13 * Del;
@@ -390,20 +361,6 @@ __device__ __forceinline__ double poissinv_w(double lam, double U, double W, boo
 
 __device__ static __noinline__ double poissinv(double lam, double U) {
   return poissinv_w(lam, U, (lam > 4.0 && U > 0.0 && U < 1.0) ? giles_w(U) : 0.0);
-}
-
-// The count Giles' method returns from its central asymptotic branch alone
-// (lam > 4, |W| < 3, S clear of the correction band, floor(S) >= 10), for
-// 0 < U < 1; false when any other branch would decide.
-__device__ __forceinline__ bool giles_asym_only(double lam, double W, int &N) {
-  if (!(lam > 4.0 && fabs(W) < 3.0)) return false;
-  double S, Del;
-  giles_asym_S(lam, 1.0 / lam, W, S, Del);
-  const double X = floor(S);
-  if (S > 10.0 && S < X + 2.0 * Del) return false;
-  if (X < 10.0) return false;
-  N = static_cast<int>(X);
-  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -470,16 +427,14 @@ __device__ __forceinline__ int pois_sum32(double lam, double U, PoisMargin *mg) 
       run = F < uf;
     }
   }
-  constexpr float tol = 3e-5f;
   const float up = F - uf, lo = uf - Fp;
-  if (run || !(up > tol) || !(n == 0 || lo > tol)) return -1;
+  if (run || !(up > kSumTol) || !(n == 0 || lo > kSumTol)) return -1;
   if (mg) {
     mg->valid = true;
     mg->up = up;
-    mg->lo = n > 0 ? static_cast<double>(lo) : U;
+    mg->lo = n > 0 ? lo : uf;
     mg->p_hi = p;
     mg->p_lo = n > 0 ? pp : 0.0f;
-    mg->slack = tol;
   }
   return n;
 }
@@ -491,7 +446,6 @@ __device__ __forceinline__ int poisson_first32(double lam, double U, float W, Po
   if (mg) {
     mg->valid = false;
     mg->asym = false;
-    mg->fp32 = true;
     mg->Wf = W;
   }
   if (U <= 0.0) return 0;

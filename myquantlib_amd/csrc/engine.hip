@@ -1236,6 +1236,33 @@ struct HostBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
+// Weights of the level means 0..L in a multilevel estimate. Giles' MLMC: 1.
+// ML2R (multilevel Richardson-Romberg, Lemaire & Pages 2017) on refiners
+// n_l = 2^l and weak order alpha: the Richardson weights w_l = L_l(0), the
+// Lagrange basis at 0 over the nodes x_l = n_l^-alpha, i.e.
+// w_l = prod_{j != l} x_j / (x_j - x_l), which solve sum_l w_l = 1,
+// sum_l w_l x_l^k = 0 (k = 1..L); level l's mean E[P_l - P_{l-1}] then
+// carries W_l = sum_{j >= l} w_j (W_0 = 1).
+std::vector<double> mlmc_level_weights(size_t L, double alpha, bool ml2r) {
+  std::vector<double> W(L + 1, 1.0);
+  if (!ml2r) return W;
+  std::vector<double> x(L + 1), w(L + 1);
+  for (size_t l = 0; l <= L; ++l) x[l] = std::pow(2.0, -alpha * static_cast<double>(l));
+  for (size_t l = 0; l <= L; ++l) {
+    double p = 1.0;
+    for (size_t j = 0; j <= L; ++j)
+      if (j != l) p *= x[j] / (x[j] - x[l]);
+    w[l] = p;
+  }
+  double t = 0.0;
+  for (size_t l = L + 1; l-- > 0;) {
+    t += w[l];
+    W[l] = t;
+  }
+  W[0] = 1.0;  // sum of all w, exactly 1 up to rounding
+  return W;
+}
+
 // One level's launch sequence -- dN upload, level kernel, block-sum download
 // -- captured once as a HIP graph and replayed on every visit of the level
 // by the Giles loop (the per-visit path count goes through pinned memory).
@@ -1272,9 +1299,9 @@ struct MlmcRun {
     job_stride = (sizeof(Job<R>) + 255) / 256 * 256 + 256;
     c->mlmc_job.reserve(L * job_stride);
     h_dn.reserve(L * sizeof(uint64_t));
-    h_sums.reserve(L * 2 * nb * sizeof(double));
+    h_sums.reserve(L * MLMC_SUMS * nb * sizeof(double));
     d_dn.reserve(L * sizeof(uint64_t));
-    d_sums.reserve(L * 2 * nb * sizeof(double));
+    d_sums.reserve(L * MLMC_SUMS * nb * sizeof(double));
     graphs.resize(L);
   }
 
@@ -1321,7 +1348,7 @@ struct MlmcRun {
     HIPCHECK(hipStreamSynchronize(c->stream));
     const Job<R> *dj = reinterpret_cast<const Job<R> *>(slot);
     uint64_t *st = c->mlmc_streams.as<uint64_t>();
-    double *bs = d_sums.as<double>() + l * 2 * nb;
+    double *bs = d_sums.as<double>() + l * MLMC_SUMS * nb;
     const uint64_t *dn = d_dn.as<uint64_t>() + l;
     const int cp = coupled ? 1 : 0;
     auto &G = graphs[l];
@@ -1330,24 +1357,34 @@ struct MlmcRun {
     hipError_t e = hipMemcpyAsync(d_dn.as<uint64_t>() + l, h_dn.as<uint64_t>() + l, sizeof(uint64_t),
                                   hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-      if (nf == 0)
-        hipLaunchKernelGGL((mlmc_level_kernel<R, 0>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs);
-      else if (nf == 1)
-        hipLaunchKernelGGL((mlmc_level_kernel<R, 1>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs);
-      else
-        hipLaunchKernelGGL((mlmc_level_kernel<R, 2>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs);
+      const bool gk = mode != MYQL_GREEK_NONE;
+#define MYQL_MLMC_LAUNCH(NF_, GK_)                                                                              \
+  hipLaunchKernelGGL((mlmc_level_kernel<R, NF_, GK_>), dim3(nb), dim3(BLOCK), smem, c->stream, dj, st, dn, cp, bs)
+      if (nf == 0) {
+        if (gk) MYQL_MLMC_LAUNCH(0, true);
+        else MYQL_MLMC_LAUNCH(0, false);
+      } else if (nf == 1) {
+        if (gk) MYQL_MLMC_LAUNCH(1, true);
+        else MYQL_MLMC_LAUNCH(1, false);
+      } else {
+        if (gk) MYQL_MLMC_LAUNCH(2, true);
+        else MYQL_MLMC_LAUNCH(2, false);
+      }
+#undef MYQL_MLMC_LAUNCH
       e = hipGetLastError();
     }
     if (e == hipSuccess)
-      e = hipMemcpyAsync(h_sums.as<double>() + l * 2 * nb, bs, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
-                         c->stream);
+      e = hipMemcpyAsync(h_sums.as<double>() + l * MLMC_SUMS * nb, bs, MLMC_SUMS * nb * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream);
     const hipError_t ce = hipStreamEndCapture(c->stream, &G->g);
     HIPCHECK(e);
     HIPCHECK(ce);
     HIPCHECK(hipGraphInstantiate(&G->exec, G->g, nullptr, nullptr, 0));
   }
 
-  void level(size_t l, uint64_t dN, double &sum, double &sq) {
+  // adds the level's new paths into acc (MLMC_SUMS sums: price, delta and
+  // gamma differences and their squares)
+  void level(size_t l, uint64_t dN, double *acc) {
     if (!graphs[l]) build_level(l);
     h_dn.as<uint64_t>()[l] = dN;
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
@@ -1357,11 +1394,9 @@ struct MlmcRun {
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     device_ms += ms;
-    const double *h = h_sums.as<double>() + l * 2 * nb;
-    for (unsigned b = 0; b < nb; ++b) {
-      sum += h[2 * b];
-      sq += h[2 * b + 1];
-    }
+    const double *h = h_sums.as<double>() + l * MLMC_SUMS * nb;
+    for (unsigned b = 0; b < nb; ++b)
+      for (int a = 0; a < MLMC_SUMS; ++a) acc[a] += h[MLMC_SUMS * b + a];
   }
 
   void run(myql_mlmc_result &res) {
@@ -1372,29 +1407,42 @@ struct MlmcRun {
     prepare();
     const size_t Lmax = cfg.L_max;
     size_t L = cfg.L_min;
+    const bool ml2r = cfg.estimator == MYQL_MLMC_ML2R;
     std::vector<uint64_t> N(Lmax + 1, 0);
-    std::vector<double> sum(Lmax + 1, 0.0), sq(Lmax + 1, 0.0), V(Lmax + 1, 0.0), cost(Lmax + 1, 0.0);
+    std::vector<std::array<double, MLMC_SUMS>> acc(Lmax + 1);
+    for (auto &a : acc) a.fill(0.0);
+    std::vector<double> V(Lmax + 1, 0.0), cost(Lmax + 1, 0.0);
     for (size_t l = 0; l <= Lmax; ++l) cost[l] = static_cast<double>(cfg.base_steps * (1ull << l));
+    auto mean = [&](size_t l, int a) { return acc[l][a] / static_cast<double>(N[l]); };
+    auto var = [&](size_t l, int a) {
+      const double m = mean(l, a);
+      return std::max(1e-18, acc[l][a + 1] / static_cast<double>(N[l]) - m * m);
+    };
     double alpha = cfg.alpha, beta = cfg.beta;
     const double eps = cfg.epsilon;
+    std::vector<double> W = mlmc_level_weights(L, cfg.alpha, ml2r);
+    // sample sizes of the weighted sum sum_l W_l mean_l at variance eps^2/2
+    // for the least cost (W = 1: Giles' N_l, MultiLevelMonteCarloPricer.hpp:
+    // 88-95)
+    auto optimal = [&](size_t l, size_t Lc) -> uint64_t {
+      if (!(V[l] > 1e-18)) return 0;
+      double sumVC = 0.0;
+      for (size_t k = 0; k <= Lc; ++k) sumVC += std::fabs(W[k]) * std::sqrt(V[k] * cost[k]);
+      return static_cast<uint64_t>(
+          std::ceil(2.0 * std::pow(eps, -2.0) * std::fabs(W[l]) * std::sqrt(V[l] / cost[l]) * sumVC));
+    };
     bool converged = false;
     while (!converged) {
+      W = mlmc_level_weights(L, cfg.alpha, ml2r);
       for (size_t l = 0; l <= L; ++l) {
-        double sumVC = 0.0;
-        for (size_t k = 0; k <= L; ++k) sumVC += std::sqrt(V[k] * cost[k]);
-        uint64_t opt = 0;
-        if (V[l] > 1e-18)
-          opt = static_cast<uint64_t>(std::ceil(2.0 * std::pow(eps, -2.0) * std::sqrt(V[l] / cost[l]) * sumVC));
+        const uint64_t opt = optimal(l, L);
         const uint64_t pilot = std::max<uint64_t>(20, cfg.N0 / (1ull << l));
         const uint64_t target = std::max(pilot, opt);
         if (N[l] < target) {
           const uint64_t dN = target - N[l];
-          level(l, dN, sum[l], sq[l]);
+          level(l, dN, acc[l].data());
           N[l] += dN;
-          if (N[l] > 1) {
-            const double mean = sum[l] / N[l];
-            V[l] = std::max(1e-18, sq[l] / N[l] - mean * mean);
-          }
+          if (N[l] > 1) V[l] = var(l, 0);
         }
       }
       if (cfg.adaptive_rates && L >= 2) {
@@ -1415,48 +1463,61 @@ struct MlmcRun {
           return -((cnt * sly - sl * sy) / (cnt * sl2 - sl * sl)) / std::log(2.0);
         };
         std::vector<double> am(L + 1);
-        for (size_t l = 0; l <= L; ++l) am[l] = std::fabs(sum[l] / N[l]);
+        for (size_t l = 0; l <= L; ++l) am[l] = std::fabs(mean(l, 0));
         const double ea = order(am), eb = order(V);
         if (ea > 0.3) alpha = std::min(2.0, ea);
         if (eb > 0.3) beta = std::min(2.0, eb);
       }
-      double sumVC = 0.0;
-      for (size_t l = 0; l <= L; ++l) sumVC += std::sqrt(V[l] * cost[l]);
       bool more = false;
-      for (size_t l = 0; l <= L; ++l) {
-        const uint64_t opt =
-            static_cast<uint64_t>(std::ceil(2.0 * std::pow(eps, -2.0) * std::sqrt(V[l] / cost[l]) * sumVC));
-        if (opt > N[l]) more = true;
-      }
+      for (size_t l = 0; l <= L; ++l)
+        if (optimal(l, L) > N[l]) more = true;
       if (more) continue;
-      double eff = std::fabs(sum[L] / N[L]);
-      if (L > 0) eff = std::max(eff, 0.5 * std::fabs(sum[L - 1] / N[L - 1]));
-      const double bias = eff / (std::pow(2.0, alpha) - 1.0);
-      if (bias > eps / 1.4142135623730951) {
-        if (L < Lmax) ++L;
-        else converged = true;
+      double bias;
+      if (!ml2r) {
+        // Giles' bias estimate (MultiLevelMonteCarloPricer.hpp:190-205)
+        double eff = std::fabs(mean(L, 0));
+        if (L > 0) eff = std::max(eff, 0.5 * std::fabs(mean(L - 1, 0)));
+        bias = eff / (std::pow(2.0, alpha) - 1.0);
+      } else if (L == 0) {
+        bias = __builtin_inf();  // one depth: nothing to compare
       } else {
-        converged = true;
+        // the change from depth L - 1 to depth L, which the deeper
+        // extrapolation removes: an estimate of the shallower one's bias,
+        // so an upper estimate of this one's
+        const std::vector<double> Wp = mlmc_level_weights(L - 1, cfg.alpha, true);
+        double d = W[L] * mean(L, 0);
+        for (size_t l = 0; l < L; ++l) d += (W[l] - Wp[l]) * mean(l, 0);
+        bias = std::fabs(d);
       }
+      if (bias > eps / 1.4142135623730951 && L < Lmax) ++L;
+      else converged = true;
     }
+    W = mlmc_level_weights(L, cfg.alpha, ml2r);
     const double df = std::exp(-r * in.maturity);
-    res.price = 0.0;
-    double var_total = 0.0;
+    double est[3] = {0.0, 0.0, 0.0}, ev[3] = {0.0, 0.0, 0.0};
     res.total_steps = 0;
     res.alpha_estimated = alpha;
     res.beta_estimated = beta;
     res.n_levels = static_cast<int32_t>(L + 1);
+    const bool gk = mode != MYQL_GREEK_NONE;
     for (size_t l = 0; l <= L; ++l) {
       const uint64_t spp = (l == 0) ? cfg.base_steps : (cfg.base_steps * (1ull << l) * 3 / 2);
       res.total_steps += N[l] * spp;
-      const double mean = sum[l] / N[l];
       if (res.paths_per_level) res.paths_per_level[l] = N[l];
-      if (res.mean_diffs) res.mean_diffs[l] = mean * df;
+      if (res.mean_diffs) res.mean_diffs[l] = mean(l, 0) * df;
       if (res.var_diffs) res.var_diffs[l] = V[l] * df * df;
-      res.price += mean * df;
-      var_total += V[l] / N[l];
+      if (res.level_weights) res.level_weights[l] = W[l];
+      for (int a = 0; a < (gk ? 3 : 1); ++a) {
+        est[a] += W[l] * mean(l, 2 * a);
+        ev[a] += W[l] * W[l] * (a == 0 ? V[l] : var(l, 2 * a)) / static_cast<double>(N[l]);
+      }
     }
-    res.price_std_err = std::sqrt(var_total) * df;
+    res.price = est[0] * df;
+    res.price_std_err = std::sqrt(ev[0]) * df;
+    res.delta = est[1] * df;
+    res.delta_std_err = std::sqrt(ev[1]) * df;
+    res.gamma = est[2] * df;
+    res.gamma_std_err = std::sqrt(ev[2]) * df;
     res.device_ms = device_ms;
   }
 };
@@ -1470,6 +1531,10 @@ void mlmc_validate(const myql_model *m, const int32_t *schemes, const myql_instr
   if (cfg->L_max > 30 || cfg->L_min > cfg->L_max) throw Fail(MYQL_E_INVALID_ARG, "need L_min <= L_max <= 30");
   if (cfg->base_steps < 1) throw Fail(MYQL_E_INVALID_ARG, "base_steps must be >= 1");
   if (!(cfg->epsilon > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "epsilon must be positive");
+  if (cfg->estimator != MYQL_MLMC_GILES && cfg->estimator != MYQL_MLMC_ML2R)
+    throw Fail(MYQL_E_INVALID_ARG, "estimator must be MYQL_MLMC_GILES or MYQL_MLMC_ML2R");
+  if (cfg->estimator == MYQL_MLMC_ML2R && !(cfg->alpha > 0.0 && cfg->alpha <= 8.0))
+    throw Fail(MYQL_E_INVALID_ARG, "ML2R needs the weak order alpha in (0, 8]");
 }
 
 // ---------------------------------------------------------------------------

@@ -369,8 +369,8 @@ __device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, u
   }
 }
 
-// Count of a bumped slot (intensity lamk) from slot 0's inversion (intensity
-// lam0 = 1/il0, count n0 not deferred, margins mg) and the shared normal
+// Count of a bumped slot (intensity lamk) from slot 0's FP32 first pass
+// (intensity lam0 = 1/il0, count n0, margins mg) and the shared normal
 // quantile W; -1 when undecided, and the slot then takes the full inversion.
 //  - Equal intensities: n0 (same inputs, same arithmetic).
 //  - Slot 0 decided by the summation: dF(n)/dlam = -p(n; lam) < 0, so raising
@@ -389,14 +389,12 @@ __device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, u
 //    (|W| < 3, lam > 4), so the slot's S lies within 2d of S0; when that
 //    interval stays clear of the integers and of the correction band
 //    [X, X + 2 Del) (Del scaling as 1/lam), floor(S) is n0. Else the slot
-//    reruns the branch with the shared W.
+//    reruns the branch in FP32 with the shared W.
 // Giles' method returns the exact quantile except where U lies within
-// rounding of a CDF value; the slack (1e-12 after the FP64 method, the error
-// bounds after the FP32 first pass, whose S also carries its tolerance)
-// keeps those cases out, so a count decided here equals the method's. After
-// the FP32 pass the rerun of the asymptotic branch is in FP32 too.
-__device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, double il0, double W,
-                                            const PoisMargin &mg) {
+// rounding of a CDF value; the first pass's error bounds (kSumTol on the
+// margins, s_tol on S) keep those cases out, so a count decided here equals
+// the method's.
+__device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, double il0, const PoisMargin &mg) {
   const double dl = fabs(lamk - lam0);
   if (dl == 0.0) return n0;
   if (mg.valid) {
@@ -405,43 +403,40 @@ __device__ __forceinline__ int bumped_count(double lam0, double lamk, int n0, do
     if (!(x <= 0.5)) return -1;
     const double g = dl * (1.0 + 2.0 * x);
     const bool rise = lamk > lam0;
-    const double sl = mg.slack;
+    const double sl = kSumTol;
+    const double up = mg.up, lo = mg.lo, p_hi = mg.p_hi, p_lo = mg.p_lo;
     if (rise) {
-      if (!(n0 == 0 || mg.lo > sl)) return -1;
-      if (mg.up > g * mg.p_hi + sl) return n0;
+      if (!(n0 == 0 || lo > sl)) return -1;
+      if (up > g * p_hi + sl) return n0;
     } else {
-      if (!(mg.up > sl)) return -1;
-      if (n0 == 0 || mg.lo > g * mg.p_lo + sl) return n0;
+      if (!(up > sl)) return -1;
+      if (n0 == 0 || lo > g * p_lo + sl) return n0;
     }
     const double n = static_cast<double>(rise ? n0 : n0 - 1);
-    const double p0 = rise ? mg.p_hi : mg.p_lo;
+    const double p0 = rise ? p_hi : p_lo;
     const double q = 2.0 * n * il0, c = n * il0 - 1.0;
     const double B = dl * dl * dl * (1.0 / 6.0) * p0 * (1.0 + 2.0 * x) * ((q + 1.0) * (q + 1.0) + 2.0 * q * il0) +
                      sl;
     if (rise) {
-      const double m = mg.up - p0 * dl * (1.0 + 0.5 * c * dl);  // F(n0; lamk) - U
+      const double m = up - p0 * dl * (1.0 + 0.5 * c * dl);  // F(n0; lamk) - U
       if (m > B) return n0;
       const double n1 = static_cast<double>(n0 + 1);
-      if (m < -B && mg.p_hi * lam0 * (1.0 - dl) + (m - B) * n1 > sl * n1) return n0 + 1;
+      if (m < -B && p_hi * lam0 * (1.0 - dl) + (m - B) * n1 > sl * n1) return n0 + 1;
     } else {
-      const double m = mg.lo - p0 * dl * (1.0 - 0.5 * c * dl);  // U - F(n0 - 1; lamk)
+      const double m = lo - p0 * dl * (1.0 - 0.5 * c * dl);  // U - F(n0 - 1; lamk)
       if (m > B) return n0;
-      if (m < -B && mg.p_lo * (1.0 - n * dl * il0) + (m - B) > sl) return n0 - 1;
+      if (m < -B && p_lo * (1.0 - n * dl * il0) + (m - B) > sl) return n0 - 1;
     }
     return -1;
   }
   if (mg.asym && dl <= 0.5 * lam0 && lam0 - dl > 4.0) {
-    const double f0 = mg.S - static_cast<double>(n0);
-    const double e = 2.0 * dl + mg.s_tol;
-    if (f0 + e < 1.0 && f0 - e >= 2.0 * mg.Del * (1.0 + 2.0 * dl * il0)) return n0;
+    const double f0 = static_cast<double>(mg.S) - static_cast<double>(n0);
+    const double e = 2.0 * dl + static_cast<double>(mg.s_tol);
+    if (f0 + e < 1.0 && f0 - e >= 2.0 * static_cast<double>(mg.Del) * (1.0 + 2.0 * dl * il0)) return n0;
   }
-  if (mg.fp32) {
-    if (!(lamk > 4.0 && fabsf(mg.Wf) < 2.999f)) return -1;
-    float S, Del, tol;
-    return giles_asym32(lamk, mg.Wf, S, Del, tol);
-  }
-  int n = n0;
-  return giles_asym_only(lamk, W, n) ? n : -1;
+  if (!(lamk > 4.0 && fabsf(mg.Wf) < 2.999f)) return -1;
+  float S, Del, tol;
+  return giles_asym32(lamk, mg.Wf, S, Del, tol);
 }
 
 // Giles' count for a single slot with the rare branches packed across the
@@ -490,7 +485,7 @@ __device__ __forceinline__ void poisson_counts_lockstep(const double (&lams)[NV]
     const double il0 = 1.0 / lams[0];
 #pragma unroll
     for (int k = 1; k < NV; ++k) {
-      const int n = r0 ? -1 : bumped_count(lams[0], lams[k], n0, il0, W, mg);
+      const int n = r0 ? -1 : bumped_count(lams[0], lams[k], n0, il0, mg);
       if (n >= 0) Ns[k] = n;
       else pend |= 1u << k;
     }

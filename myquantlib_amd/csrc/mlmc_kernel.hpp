@@ -159,68 +159,87 @@ __device__ void sim_coupled(const Job<R> *__restrict__ J, Rng &g, R *__restrict_
   finalize_stage(J, tc, xc, C->dt, C->total_time, stage, 1, tid);
 }
 
-// Price dN new paths of one level; per-block FP64 (sum, sum of squares).
+// Price dN new paths of one level; per block the FP64 (sum, sum of squares)
+// of the level's price difference and, with GK (Greek mode Essential or
+// Full), of its delta and gamma differences: each path's spot-up and
+// spot-down payoffs (the reference's calculate_to_buffer output,
+// MultiLevelMonteCarloPricer.hpp:280,321-324) give the central differences
+// (up - dn) / 2h and (up - 2 base + dn) / h^2 of its fine and its coarse
+// path; block_sums holds MLMC_SUMS doubles per block.
 // Compiled for four waves per SIMD: 128 VGPRs with 5 spilled for one factor
 // (133 and three waves otherwise), 4 spilled for two (194 and two waves); at
 // 262,144 streams the Heston up-and-out MLMC run takes 697 ms against 764 ms
 // for the three-wave build at its best, 196,608 streams.
-template <typename R, int NF>
+constexpr int MLMC_SUMS = 6;
+
+template <typename R, int NF, bool GK>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) mlmc_level_kernel(const Job<R> *__restrict__ J,
                                                            uint64_t *__restrict__ streams,
                                                            const uint64_t *__restrict__ dNp, int coupled,
                                                            double *__restrict__ block_sums) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double s_red[2 * (BLOCK / 64)];
+  __shared__ double s_red[MLMC_SUMS * (BLOCK / 64)];
   R *stage = reinterpret_cast<R *>(smem);
   const int tid = threadIdx.x;
   const uint64_t S = J->S;
   const uint64_t dN = *dNp;  // new paths of this level (device memory: the launch is a replayed graph)
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * BLOCK + tid;
-  double sy = 0.0, sy2 = 0.0;
+  double acc[MLMC_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (s < S) {
     Rng g = {streams[4 * s], streams[4 * s + 1], streams[4 * s + 2], streams[4 * s + 3]};
     const uint64_t q = dN / S, rem = dN % S;
     const uint64_t cnt = q + (s < rem ? 1 : 0);
     const double K = J->strikes[0];
     for (uint64_t i = 0; i < cnt; ++i) {
-      double y, p1, d1 = 0, d2 = 0;
+      double y, yd = 0.0, yg = 0.0;
       if (!coupled) {
         if (J->pathdep) sim_group<R, NF, 1, true>(J, 0, g, stage, tid);
         else sim_group<R, NF, 1, false>(J, 0, g, stage, tid);
-        eval3(J, stage, V_BASE, tid, K, false, y, d1, d2);
+        double pu = 0.0, pd = 0.0;
+        eval3(J, stage, V_BASE, tid, K, GK, y, pu, pd);
+        if constexpr (GK) {
+          yd = (pu - pd) * J->inv2h;
+          yg = (pu - y * 2.0 + pd) * J->invh2;
+        }
       } else {
         sim_coupled<R, NF>(J, g, stage, tid);
-        double pf;
-        eval3(J, stage, 0, tid, K, false, pf, d1, d2);
-        eval3(J, stage, 1, tid, K, false, p1, d1, d2);
-        y = pf - p1;
+        double pf, fu = 0.0, fd = 0.0, pc, cu = 0.0, cd = 0.0;
+        eval3(J, stage, 0, tid, K, GK, pf, fu, fd);
+        eval3(J, stage, 1, tid, K, GK, pc, cu, cd);
+        y = pf - pc;
+        if constexpr (GK) {
+          yd = (fu - fd) * J->inv2h - (cu - cd) * J->inv2h;
+          yg = (fu - pf * 2.0 + fd) * J->invh2 - (cu - pc * 2.0 + cd) * J->invh2;
+        }
       }
-      sy += y;
-      sy2 += y * y;
+      acc[0] += y;
+      acc[1] += y * y;
+      if constexpr (GK) {
+        acc[2] += yd;
+        acc[3] += yd * yd;
+        acc[4] += yg;
+        acc[5] += yg * yg;
+      }
     }
     streams[4 * s] = g.s0;
     streams[4 * s + 1] = g.s1;
     streams[4 * s + 2] = g.s2;
     streams[4 * s + 3] = g.s3;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    sy += __shfl_xor(sy, off, 64);
-    sy2 += __shfl_xor(sy2, off, 64);
-  }
+  constexpr int NS = GK ? MLMC_SUMS : 2;
+#pragma unroll
+  for (int a = 0; a < NS; ++a)
+    for (int off = 32; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, 64);
   const int lane = tid & 63, wave = tid >> 6;
   if (lane == 0) {
-    s_red[2 * wave] = sy;
-    s_red[2 * wave + 1] = sy2;
+#pragma unroll
+    for (int a = 0; a < MLMC_SUMS; ++a) s_red[MLMC_SUMS * wave + a] = acc[a];
   }
   __syncthreads();
-  if (tid == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < BLOCK / 64; ++w) {
-      a += s_red[2 * w];
-      b += s_red[2 * w + 1];
-    }
-    block_sums[2 * blockIdx.x] = a;
-    block_sums[2 * blockIdx.x + 1] = b;
+  if (tid < MLMC_SUMS) {
+    double t = 0.0;
+    for (int w = 0; w < BLOCK / 64; ++w) t += s_red[MLMC_SUMS * w + tid];
+    block_sums[MLMC_SUMS * blockIdx.x + tid] = t;
   }
 }
 

@@ -119,9 +119,9 @@ def mlmc_config(epsilon: float = 0.005, L_min: int = 2, L_max: int = 15, N0: int
                 base_steps: int = 1, alpha: float = 2.0, beta: float = 1.0,
                 adaptive_rates: bool = True, seed: int = 42, fd_bump: float = 1e-4,
                 precision: int = capi.FP64, stream_layout: int = capi.STREAMS_JUMP,
-                num_streams: int = 0, flags: int = 0) -> MLMCConfig:
+                num_streams: int = 0, flags: int = 0, estimator: int = capi.MLMC_GILES) -> MLMCConfig:
     return MLMCConfig(epsilon, L_min, L_max, N0, base_steps, alpha, beta, int(adaptive_rates),
-                      seed, fd_bump, precision, stream_layout, num_streams, flags)
+                      seed, fd_bump, precision, stream_layout, num_streams, flags, estimator)
 
 
 class ResultBuffers:
@@ -164,10 +164,12 @@ class MLMCBuffers:
         self.ppl = np.zeros(L_max + 1, dtype=np.uint64)
         self.mean = np.full(L_max + 1, np.nan)
         self.var = np.full(L_max + 1, np.nan)
+        self.weights = np.full(L_max + 1, np.nan)
         r = MLMCResult()
         r.paths_per_level = self.ppl.ctypes.data_as(C.POINTER(C.c_uint64))
         r.mean_diffs = self.mean.ctypes.data_as(C.POINTER(C.c_double))
         r.var_diffs = self.var.ctypes.data_as(C.POINTER(C.c_double))
+        r.level_weights = self.weights.ctypes.data_as(C.POINTER(C.c_double))
         self.struct = r
 
     def as_dict(self) -> dict:
@@ -176,6 +178,8 @@ class MLMCBuffers:
         return dict(price=r.price, price_std_err=r.price_std_err, total_steps=int(r.total_steps),
                     n_levels=n, paths_per_level=self.ppl[:n].copy(),
                     mean_diffs=self.mean[:n].copy(), var_diffs=self.var[:n].copy(),
+                    level_weights=self.weights[:n].copy(),
+                    delta=r.delta, delta_std_err=r.delta_std_err, gamma=r.gamma, gamma_std_err=r.gamma_std_err,
                     alpha_estimated=r.alpha_estimated, beta_estimated=r.beta_estimated,
                     device_ms=r.device_ms)
 

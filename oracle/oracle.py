@@ -90,7 +90,18 @@ class Oracle:
                                               C.POINTER(capi.Instrument), C.c_int32,
                                               C.POINTER(capi.MLMCConfig), C.c_double, C.c_double,
                                               C.c_double, C.POINTER(capi.MLMCResult), C.c_int]
+        lib.oracle_mlmc_level_weights.restype = C.c_int
+        lib.oracle_mlmc_level_weights.argtypes = [C.c_int64, C.c_double, C.c_int32, dp]
         self.lib = lib
+
+    def mlmc_level_weights(self, L: int, alpha: float, ml2r: bool = True) -> np.ndarray:
+        """Weights W_0..W_L of the level means (ML2R: from the Vandermonde
+        system of the Richardson weights, solved by elimination)."""
+        W = np.zeros(L + 1)
+        rc = self.lib.oracle_mlmc_level_weights(L, alpha, int(ml2r), W.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise EngineError(rc, "bad arguments")
+        return W
 
     def op_count(self, reset: bool = False) -> int:
         """Floating-point ops counted since the last reset (ops build only)."""

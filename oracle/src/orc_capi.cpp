@@ -560,12 +560,62 @@ int oracle_debug_coupled_path(const myql_model *model, const int32_t *schemes, c
   }
 }
 
+// Weights of the level means 0..L (the multilevel estimator's extension
+// beyond the reference, which sums the means with weight 1,
+// MultiLevelMonteCarloPricer.hpp:215-235). ML2R (Lemaire & Pages 2017):
+// the Richardson weights w solve the Vandermonde system
+// sum_j w_j x_j^k = [k == 0], k = 0..L, x_j = 2^(-alpha j), here by Gaussian
+// elimination with partial pivoting; level l carries W_l = sum_{j >= l} w_j.
+static std::vector<double> level_weights(size_t L, double alpha, bool ml2r) {
+  std::vector<double> W(L + 1, 1.0);
+  if (!ml2r) return W;
+  const size_t n = L + 1;
+  std::vector<double> A(n * n), b(n, 0.0);
+  for (size_t k = 0; k < n; ++k)
+    for (size_t j = 0; j < n; ++j) A[k * n + j] = std::pow(2.0, -alpha * static_cast<double>(j * k));
+  b[0] = 1.0;
+  for (size_t c = 0; c < n; ++c) {
+    size_t piv = c;
+    for (size_t r = c + 1; r < n; ++r)
+      if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+    for (size_t j = 0; j < n; ++j) std::swap(A[c * n + j], A[piv * n + j]);
+    std::swap(b[c], b[piv]);
+    for (size_t r = c + 1; r < n; ++r) {
+      const double f = A[r * n + c] / A[c * n + c];
+      for (size_t j = c; j < n; ++j) A[r * n + j] -= f * A[c * n + j];
+      b[r] -= f * b[c];
+    }
+  }
+  std::vector<double> w(n);
+  for (size_t c = n; c-- > 0;) {
+    double t = b[c];
+    for (size_t j = c + 1; j < n; ++j) t -= A[c * n + j] * w[j];
+    w[c] = t / A[c * n + c];
+  }
+  double t = 0.0;
+  for (size_t l = n; l-- > 0;) {
+    t += w[l];
+    W[l] = t;
+  }
+  W[0] = 1.0;
+  return W;
+}
+
+int oracle_mlmc_level_weights(int64_t L, double alpha, int32_t ml2r, double *W) {
+  if (L < 0 || L > 30 || !W) return MYQL_E_INVALID_ARG;
+  const std::vector<double> w = level_weights(static_cast<size_t>(L), alpha, ml2r != 0);
+  for (int64_t l = 0; l <= L; ++l) W[l] = w[l];
+  return MYQL_OK;
+}
+
 int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const myql_instrument *instr,
                           int32_t mode, const myql_mlmc_config *cfg, double S0, double r, double q,
                           myql_mlmc_result *res, int nthreads) {
   try {
     validate(model, schemes, instr);
     if (!cfg || !res) throw Error(MYQL_E_INVALID_ARG, "null argument");
+    if (cfg->estimator != MYQL_MLMC_GILES && cfg->estimator != MYQL_MLMC_ML2R)
+      throw Error(MYQL_E_INVALID_ARG, "bad estimator");
     const myql_model &M0 = *model;
     Instr I;
     I.in = *instr;
@@ -576,15 +626,31 @@ int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const
     const double h = S0 * cfg->fd_bump;
     const TrackerCfg tc = tracker_config(I, mode, S0, h);
     const size_t Lmax = cfg->L_max;
+    const bool greeks = mode != MYQL_GREEK_NONE;
+    const bool ml2r = cfg->estimator == MYQL_MLMC_ML2R;
     uint64_t S = cfg->num_streams ? cfg->num_streams : 65536;
     std::vector<Xoshiro> pool(S);
     for (uint64_t s = 0; s < S; ++s) pool[s] = stream_state(cfg->seed, cfg->stream_layout, s);
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
     TableCache cache;
     cache.reference_rows = (cfg->flags & MYQL_FLAG_REFERENCE_NCI_ROWS) != 0;
+    constexpr int NS = 6;  // price, delta and gamma differences, each with its square
 
-    auto evaluate_level = [&](size_t l, uint64_t dN, double &sum, double &sq) {
-      std::vector<double> ps(S, 0.0), pq(S, 0.0);
+    // per path: the level's price difference and, with Greeks, the
+    // differences of the fine and coarse central differences of the
+    // spot-bumped payoffs (calculate_to_buffer's up / dn buffers)
+    auto add = [&](double *a, double p, double u, double d) {
+      a[0] += p;
+      a[1] += p * p;
+      if (greeks) {
+        a[2] += u;
+        a[3] += u * u;
+        a[4] += d;
+        a[5] += d * d;
+      }
+    };
+    auto evaluate_level = [&](size_t l, uint64_t dN, double *acc) {
+      std::vector<double> part(S * NS, 0.0);
       if (l == 0) {
         const uint64_t cs = cfg->base_steps;
         const double dtc = T / static_cast<double>(cs);
@@ -600,8 +666,7 @@ int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const
             start_path(st, st0, tc, S0);
             multi_step(st, st0, tc, pool[s], static_cast<unsigned>(cs));
             to_buffer(I, mode, st0, S0, h, &p0, &d1, &d2);
-            ps[s] += p0;
-            pq[s] += p0 * p0;
+            add(&part[s * NS], p0, (d1 - d2) / (2.0 * h), (d1 - 2.0 * p0 + d2) / (h * h));
           }
         }
       } else {
@@ -614,50 +679,57 @@ int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const
         for (int64_t s = 0; s < static_cast<int64_t>(S); ++s) {
           uint64_t b0, cnt;
           partition(dN, S, s, b0, cnt);
-          double pf, pc, d1, d2;
+          double pf, pc, fu, fd, cu, cd;
           for (uint64_t i = 0; i < cnt; ++i) {
             PathState sf, sc;
             start_path(F, sf, tc, S0);
             start_path(C, sc, tc, S0);
             multi_step_coupled(F, C, sf, sc, tc, pool[s], static_cast<unsigned>(cs));
-            to_buffer(I, mode, sf, S0, h, &pf, &d1, &d2);
-            to_buffer(I, mode, sc, S0, h, &pc, &d1, &d2);
-            const double dd = pf - pc;
-            ps[s] += dd;
-            pq[s] += dd * dd;
+            to_buffer(I, mode, sf, S0, h, &pf, &fu, &fd);
+            to_buffer(I, mode, sc, S0, h, &pc, &cu, &cd);
+            add(&part[s * NS], pf - pc, (fu - fd) / (2.0 * h) - (cu - cd) / (2.0 * h),
+                (fu - 2.0 * pf + fd) / (h * h) - (cu - 2.0 * pc + cd) / (h * h));
           }
         }
       }
-      for (uint64_t s = 0; s < S; ++s) {
-        sum += ps[s];
-        sq += pq[s];
-      }
+      for (uint64_t s = 0; s < S; ++s)
+        for (int a = 0; a < NS; ++a) acc[a] += part[s * NS + a];
     };
 
     size_t L = cfg->L_min;
     std::vector<uint64_t> N(Lmax + 1, 0);
-    std::vector<double> sum(Lmax + 1, 0.0), sq(Lmax + 1, 0.0), V(Lmax + 1, 0.0), cost(Lmax + 1, 0.0);
+    std::vector<std::array<double, NS>> acc(Lmax + 1);
+    for (auto &a : acc) a.fill(0.0);
+    std::vector<double> V(Lmax + 1, 0.0), cost(Lmax + 1, 0.0);
     for (size_t l = 0; l <= Lmax; ++l) cost[l] = static_cast<double>(cfg->base_steps * (1ull << l));
+    auto mean = [&](size_t l, int a) { return acc[l][a] / static_cast<double>(N[l]); };
+    auto var = [&](size_t l, int a) {
+      const double m = mean(l, a);
+      return std::max(1e-18, acc[l][a + 1] / static_cast<double>(N[l]) - m * m);
+    };
     double alpha = cfg->alpha, beta = cfg->beta;
     const double eps = cfg->epsilon;
+    std::vector<double> W = level_weights(L, cfg->alpha, ml2r);
+    // least-cost sample sizes of sum_l W_l mean_l at variance eps^2 / 2
+    auto optimal = [&](size_t l, size_t Lc) -> uint64_t {
+      if (!(V[l] > 1e-18)) return 0;
+      double sumVC = 0.0;
+      for (size_t k = 0; k <= Lc; ++k) sumVC += std::fabs(W[k]) * std::sqrt(V[k] * cost[k]);
+      return static_cast<uint64_t>(
+          std::ceil(2.0 * std::pow(eps, -2.0) * std::fabs(W[l]) * std::sqrt(V[l] / cost[l]) * sumVC));
+    };
     bool converged = false;
     while (!converged) {
+      W = level_weights(L, cfg->alpha, ml2r);
       for (size_t l = 0; l <= L; ++l) {
-        double sumVC = 0.0;
-        for (size_t k = 0; k <= L; ++k) sumVC += std::sqrt(V[k] * cost[k]);
-        uint64_t opt = 0;
-        if (V[l] > 1e-18)
-          opt = static_cast<uint64_t>(std::ceil(2.0 * std::pow(eps, -2.0) * std::sqrt(V[l] / cost[l]) * sumVC));
+        const uint64_t opt = optimal(l, L);
         const uint64_t pilot = std::max<uint64_t>(20, cfg->N0 / (1ull << l));
         const uint64_t target = std::max(pilot, opt);
         if (N[l] < target) {
           const uint64_t dN = target - N[l];
-          evaluate_level(l, dN, sum[l], sq[l]);
+          evaluate_level(l, dN, acc[l].data());
           N[l] += dN;
-          if (N[l] > 1) {
-            const double mean = sum[l] / N[l];
-            V[l] = std::max(1e-18, sq[l] / N[l] - mean * mean);
-          }
+          if (N[l] > 1) V[l] = var(l, 0);
         }
       }
       if (cfg->adaptive_rates && L >= 2) {
@@ -681,33 +753,36 @@ int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const
           return -slope / std::log(2.0);
         };
         std::vector<double> am(L + 1);
-        for (size_t l = 0; l <= L; ++l) am[l] = std::fabs(sum[l] / N[l]);
+        for (size_t l = 0; l <= L; ++l) am[l] = std::fabs(mean(l, 0));
         const double ea = order(am), eb = order(V);
         if (ea > 0.3) alpha = std::min(2.0, ea);
         if (eb > 0.3) beta = std::min(2.0, eb);
       }
-      double sumVC = 0.0;
-      for (size_t l = 0; l <= L; ++l) sumVC += std::sqrt(V[l] * cost[l]);
       bool more = false;
-      for (size_t l = 0; l <= L; ++l) {
-        const uint64_t opt =
-            static_cast<uint64_t>(std::ceil(2.0 * std::pow(eps, -2.0) * std::sqrt(V[l] / cost[l]) * sumVC));
-        if (opt > N[l]) more = true;
-      }
+      for (size_t l = 0; l <= L; ++l)
+        if (optimal(l, L) > N[l]) more = true;
       if (more) continue;
-      double eff = std::fabs(sum[L] / N[L]);
-      if (L > 0) eff = std::max(eff, 0.5 * std::fabs(sum[L - 1] / N[L - 1]));
-      const double bias = eff / (std::pow(2.0, alpha) - 1.0);
-      if (bias > eps / 1.4142135623730951) {
-        if (L < Lmax) ++L;
-        else converged = true;
+      double bias;
+      if (!ml2r) {
+        double eff = std::fabs(mean(L, 0));
+        if (L > 0) eff = std::max(eff, 0.5 * std::fabs(mean(L - 1, 0)));
+        bias = eff / (std::pow(2.0, alpha) - 1.0);
+      } else if (L == 0) {
+        bias = HUGE_VAL;
       } else {
-        converged = true;
+        // |estimate at depth L - estimate at depth L - 1|
+        const std::vector<double> Wp = level_weights(L - 1, cfg->alpha, true);
+        double a = 0.0, b = 0.0;
+        for (size_t l = 0; l <= L; ++l) a += W[l] * mean(l, 0);
+        for (size_t l = 0; l < L; ++l) b += Wp[l] * mean(l, 0);
+        bias = std::fabs(a - b);
       }
+      if (bias > eps / 1.4142135623730951 && L < Lmax) ++L;
+      else converged = true;
     }
+    W = level_weights(L, cfg->alpha, ml2r);
     const double df = std::exp(-r * T);
-    res->price = 0.0;
-    double var_total = 0.0;
+    double est[3] = {0.0, 0.0, 0.0}, ev[3] = {0.0, 0.0, 0.0};
     res->total_steps = 0;
     res->alpha_estimated = alpha;
     res->beta_estimated = beta;
@@ -715,14 +790,21 @@ int oracle_mlmc_calculate(const myql_model *model, const int32_t *schemes, const
     for (size_t l = 0; l <= L; ++l) {
       const uint64_t spp = (l == 0) ? cfg->base_steps : (cfg->base_steps * (1ull << l) * 3 / 2);
       res->total_steps += N[l] * spp;
-      const double mean = sum[l] / N[l];
       if (res->paths_per_level) res->paths_per_level[l] = N[l];
-      if (res->mean_diffs) res->mean_diffs[l] = mean * df;
+      if (res->mean_diffs) res->mean_diffs[l] = mean(l, 0) * df;
       if (res->var_diffs) res->var_diffs[l] = V[l] * df * df;
-      res->price += mean * df;
-      var_total += V[l] / N[l];
+      if (res->level_weights) res->level_weights[l] = W[l];
+      for (int a = 0; a < (greeks ? 3 : 1); ++a) {
+        est[a] += W[l] * mean(l, 2 * a);
+        ev[a] += W[l] * W[l] * (a == 0 ? V[l] : var(l, 2 * a)) / static_cast<double>(N[l]);
+      }
     }
-    res->price_std_err = std::sqrt(var_total) * df;
+    res->price = est[0] * df;
+    res->price_std_err = std::sqrt(ev[0]) * df;
+    res->delta = est[1] * df;
+    res->delta_std_err = std::sqrt(ev[1]) * df;
+    res->gamma = est[2] * df;
+    res->gamma_std_err = std::sqrt(ev[2]) * df;
     res->device_ms = 0.0;
     return MYQL_OK;
   } catch (const Error &e) {
