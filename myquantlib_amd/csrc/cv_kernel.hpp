@@ -168,7 +168,11 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
   // consecutive entries
   for (uint64_t i = 0; i < count; ++i) {
     const uint64_t p = i * S + s;
-    R x = static_cast<R>(J->logS0);
+    // the log-return or the log-price, as the pricing kernels carry it
+    // (recorded as the log-price)
+    constexpr bool LR = log_return_paths<R, false>();
+    const double xrec0 = LR ? J->logS0 : 0.0;
+    R x = LR ? R(0) : static_cast<R>(J->logS0);
     R v[NFX];
 #pragma unroll
     for (int f = 0; f < NFX; ++f) v[f] = P->v0[f];
@@ -180,7 +184,7 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
     if constexpr (NF == 0) {
       // zero-factor European: one step to maturity (ASVJstepper.hpp:145-161),
       // one control point at t = 0
-      put(0, 0, x);
+      put(0, 0, xrec0 + static_cast<double>(x));
       put(0, 1, 0.0);
       put(0, 2, 0.0);
       const R G = normcdfinv<R>(uniform(g));
@@ -194,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
             put(j - 1, 4, V[0]);
             put(j - 1, 5, V[1]);
           }
-          put(j, 0, x);
+          put(j, 0, xrec0 + static_cast<double>(x));
           put(j, 1, v[0]);
           put(j, 2, NF > 1 ? static_cast<double>(v[NFX - 1]) : 0.0);
           A = 0.0;
@@ -244,8 +248,9 @@ __global__ void __launch_bounds__(BLOCK) cv_paths_kernel(const Job<R> *__restric
     const R xm = x;
     R xs[1] = {x};
     euro_jump<R, 1>(J, PP, xs, g);
-    set.ST[p] = static_cast<double>(dexp<R>(xs[0]));
-    if (set.Sm) set.Sm[p] = static_cast<double>(dexp<R>(xm));
+    const R S0r = LR ? static_cast<R>(J->S0) : R(1);
+    set.ST[p] = static_cast<double>(S0r * dexp<R>(xs[0]));
+    if (set.Sm) set.Sm[p] = static_cast<double>(S0r * dexp<R>(xm));
   }
 }
 

@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(BLOCK) debug_poisson_counts_kernel(const doubl
   const double uu = live ? u[i] : 0.5;
   int Ns[NV];
   if constexpr (NV == 1) Ns[0] = poisson_count_packed(l[0], uu);
-  else poisson_counts_lockstep<NV>(l, uu, Ns);
+  else poisson_counts_lockstep<NV>([&](int k) { return l[k]; }, uu, Ns);
   if (live)
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[i * NV + k] = Ns[k];

@@ -67,6 +67,9 @@ template <typename R, int NV, int NFX, bool PD>
 constexpr bool t_slots_fp64() { return sizeof(R) == 4 && !PD && NV == 3 + 2 * NFX; }
 template <int NFX>
 constexpr int t_slot0() { return 1 + 2 * NFX; }
+// whether a kernel's paths carry the log-return instead of the log-price
+template <typename R, bool PD>
+constexpr bool log_return_paths() { return sizeof(R) == 4 && !PD; }
 
 template <typename R>
 struct Trk {
@@ -330,8 +333,8 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 // lanes one item each, and read back by their owners. The wave sits through
 // the rare code once per 64 items instead of once per slot. Must be entered
 // by every active lane of the wave (a wave-uniform condition).
-template <int NV>
-__device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, unsigned rare, int (&Ns)[NV]) {
+template <int NV, typename Lam>
+__device__ __forceinline__ void giles_rare(const Lam &lams, double U, unsigned rare, int (&Ns)[NV]) {
   constexpr int W64 = 64;
   __shared__ double s_lam[BLOCK / W64][W64];
   __shared__ double s_u[BLOCK / W64][W64];
@@ -353,7 +356,7 @@ __device__ __forceinline__ void giles_rare(const double (&lams)[NV], double U, u
     for (int k = 0; k < NV; ++k) {
       const int g = idx[k] - base;
       if (((rare >> k) & 1u) && g >= 0 && g < n_active) {
-        s_lam[wave][g] = lams[k];
+        s_lam[wave][g] = lams(k);
         s_u[wave][g] = U;
       }
     }
@@ -447,8 +450,7 @@ __device__ __forceinline__ int poisson_count_packed(double lam, double u) {
   int Ns[1] = {poisson_first32(lam, u, W, nullptr)};
   const bool rare = Ns[0] < 0;
   if (__any(rare)) {
-    const double lams[1] = {lam};
-    giles_rare<1>(lams, u, rare ? 1u : 0u, Ns);
+    giles_rare<1>([&](int) { return lam; }, u, rare ? 1u : 0u, Ns);
   }
   return Ns[0];
 }
@@ -467,25 +469,28 @@ __device__ __forceinline__ int poisson_count_packed(double lam, double u) {
 //    branches) is finished by the FP64 method on wave-packed items
 //    (giles_rare).
 // Every active lane of the wave must reach the call together.
-template <int NV>
-__device__ __forceinline__ void poisson_counts_lockstep(const double (&lams)[NV], double u1, int (&Ns)[NV]) {
+// lams(k): slot k's intensity, evaluated where needed rather than kept in
+// registers for all slots at once.
+template <int NV, typename Lam>
+__device__ __forceinline__ void poisson_counts_lockstep(const Lam &lams, double u1, int (&Ns)[NV]) {
   bool big = false;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    big = big || lams[k] > 4.0;
+    big = big || lams(k) > 4.0;
     Ns[k] = 0;
   }
   if (u1 > 0.0) {  // u1 == 0: every count is 0
     const float W = big ? giles_w32(u1) : 0.0f;
     PoisMargin mg;
-    const int n0 = poisson_first32(lams[0], u1, W, &mg);
+    const double lam0 = lams(0);
+    const int n0 = poisson_first32(lam0, u1, W, &mg);
     const bool r0 = n0 < 0;
     unsigned pend = r0 ? 1u : 0u;
     Ns[0] = r0 ? 0 : n0;
-    const double il0 = 1.0 / lams[0];
+    const double il0 = 1.0 / lam0;
 #pragma unroll
     for (int k = 1; k < NV; ++k) {
-      const int n = r0 ? -1 : bumped_count(lams[0], lams[k], n0, il0, mg);
+      const int n = r0 ? -1 : bumped_count(lam0, lams(k), n0, il0, mg);
       if (n >= 0) Ns[k] = n;
       else pend |= 1u << k;
     }
@@ -558,6 +563,29 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
   } else if (sch == 2) {  // NCI: Poisson mixture, tabulated chi2 inverse
     const double u1 = uniform(g);
     const double u2 = uniform(g);
+    int Ns[NV];
+    if constexpr (NV == 1) {
+      Ns[0] = poisson_count_packed(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1);
+    } else {
+      auto lam = [&](int k) {
+        return P[k]->Kd[f][0] * (is_t(k) ? vt[k - KT][f] : static_cast<double>(v[k][f]));
+      };
+      if constexpr (sizeof(R) == 4) {
+        // FP32 kernels form each intensity where it is read: five FP64
+        // intensities kept live across the inversion pushed the four-wave
+        // build's spills from 15 to 29 VGPRs
+        poisson_counts_lockstep<NV>(lam, u1, Ns);
+      } else {
+        // the FP64 kernel (three waves, no spills) runs faster with them
+        // formed once
+        double lams[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lams[k] = lam(k);
+        poisson_counts_lockstep<NV>([&](int k) { return lams[k]; }, u1, Ns);
+      }
+    }
+    // grid interval and Hermite weights of u2, formed after the counts so
+    // they are not live across the Poisson inversion
     const double pos = u2 * J->nci_inv_dx;
     int i = static_cast<int>(pos);
     if (i >= J->nci_grid - 1) i = J->nci_grid - 2;
@@ -566,16 +594,6 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     const R h00 = R(2) * t3 - R(3) * t2 + R(1), h10 = t3 - R(2) * t2 + t;
     const R h01 = R(-2) * t3 + R(3) * t2, h11 = t3 - t2;
     const R *__restrict__ tab = static_cast<const R *>(J->nci[f]);
-    int Ns[NV];
-    if constexpr (NV == 1) {
-      Ns[0] = poisson_count_packed(P[0]->Kd[f][0] * static_cast<double>(v[0][f]), u1);
-    } else {
-      double lams[NV];
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-        lams[k] = P[k]->Kd[f][0] * (is_t(k) ? vt[k - KT][f] : static_cast<double>(v[k][f]));
-      poisson_counts_lockstep<NV>(lams, u1, Ns);
-    }
     // the table (chi2 quantiles of d + 2N degrees of freedom) and u2 are
     // common to all slots, so slots sharing slot 0's count share its
     // interpolated quantile and differ only in their scale factor
@@ -591,9 +609,20 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     };
     const bool main0 = in_tab(Ns[0], 0) && u2 < 0.999;
     R y0 = R(0);
+    // The maturity-bumped slots of FP32 kernels read the FP64 rows (below).
+    // There slot 0's row is read from the FP64 table alone -- one gather
+    // per step instead of an FP32 and an FP64 one -- and the FP32 slots take
+    // its value rounded to FP32.
+    double td = 0.0, y0d = 0.0;
+    if constexpr (TD) td = pos - static_cast<double>(i);
     if (main0) {
-      const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
-      y0 = h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3];
+      if constexpr (TD) {
+        y0d = nci_row64(J->nci64[f], J->nci_grid, Ns[0], i, td);
+        y0 = static_cast<R>(y0d);
+      } else {
+        const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
+        y0 = h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3];
+      }
     }
     // likewise slot 0's upper-tail value, or its exact quantile when its
     // row is inverted exactly: computed once, taken by the slots on the same
@@ -625,11 +654,6 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     // shifts the next step's Poisson intensity, and where the uniform lies
     // near a CDF value the count, which would part those paths from the FP64
     // ones and from each other.
-    double td = 0.0, y0d = 0.0;
-    if constexpr (TD) {
-      td = pos - static_cast<double>(i);
-      if (main0) y0d = nci_row64(J->nci64[f], J->nci_grid, Ns[0], i, td);
-    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const int N = Ns[k];
@@ -682,12 +706,14 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
   }
   R x[NV], v[NV][NFX];
   Trk<R> tr[PD ? NV : 1];
-  // Europeans carry the log-return x - log S0 (no tracker compares it with a
-  // log-space level) and stage S0 e^x: in FP32 a log-price near log S0 ~ 4.6
+  // FP32 Europeans carry the log-return x - log S0 (no tracker compares it
+  // with a log-space level) and stage S0 e^x: a log-price near log S0 ~ 4.6
   // would round every path's start by up to 2.4e-7 -- one systematic shift of
   // all terminal spots -- and each step's sum by as much again, where the
-  // log-return's rounding scales with its own size.
-  const R x0 = PD ? static_cast<R>(J->logS0) : R(0);
+  // log-return's rounding scales with its own size. FP64 paths keep the
+  // log-price, the reference's arithmetic (ASVJstepper.hpp:134-229).
+  constexpr bool LR = log_return_paths<R, PD>();
+  const R x0 = LR ? R(0) : static_cast<R>(J->logS0);
   // FP64 state of the maturity-bumped slots (t_slots_fp64)
   constexpr bool TD = t_slots_fp64<R, NV, NFX, PD>();
   constexpr int KT = t_slot0<NFX>();
@@ -784,8 +810,10 @@ __device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfir
     }
     if constexpr (PD) {
       finalize_stage<R, BLK>(J, tr[k], x[k], P[k]->dt, P[k]->total_time, stage, vid[k], tid);
-    } else {
+    } else if constexpr (LR) {
       stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = static_cast<R>(J->S0) * dexp<R>(x[k]);
+    } else {
+      stage[(F_SPOT * MAXV + vid[k]) * BLK + tid] = dexp<R>(x[k]);
     }
   }
 }
