@@ -31,9 +31,10 @@ struct MonteCarloConfig {
   // Engine options (no counterpart in the reference; defaults give FP64
   // paths and the engine's default stream layout).
   int precision = MYQL_FP64;              // MYQL_FP64 / MYQL_FP32 path arithmetic
-  int stream_layout = MYQL_STREAMS_JUMP;  // MYQL_STREAMS_JUMP / SEEDED
+  int stream_layout = MYQL_STREAMS_JUMP;  // MYQL_STREAMS_JUMP / SEEDED / SOBOL (randomized QMC)
   size_t num_streams = 0;                 // 0: library default
   unsigned flags = 0;                     // MYQL_FLAG_*
+  unsigned qmc_replicates = 0;            // SOBOL: independent digital shifts (0: 16)
 };
 
 namespace myql::detail {
@@ -50,6 +51,7 @@ inline myql_mc_config to_capi(const MonteCarloConfig &c) {
   o.stream_layout = c.stream_layout;
   o.num_streams = c.num_streams;
   o.flags = c.flags;
+  o.qmc_replicates = c.qmc_replicates;
   return o;
 }
 }  // namespace myql::detail

@@ -92,8 +92,24 @@ enum { MYQL_FP64 = 0, MYQL_FP32 = 1 };
  *         per-OpenMP-thread seeding (MonteCarloPricer.hpp:191); with S equal
  *         to OMP_NUM_THREADS the draws equal the reference's.
  * In both layouts stream s prices the contiguous block of paths that an
- * OpenMP static schedule over S threads would give thread s. */
-enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1 };
+ * OpenMP static schedule over S threads would give thread s.
+ *  SOBOL  randomized quasi-Monte Carlo (no counterpart in the reference,
+ *         whose README.md:332 lists quasi-Monte Carlo as future work): the
+ *         d-th uniform a path draws is coordinate d of a Sobol' point
+ *         (direction integers from primitive polynomials in order of
+ *         degree, Bratley & Fox), XORed with a random 32-bit digital shift
+ *         per coordinate, u = (x + 1/2) 2^-32. The call is priced as
+ *         qmc_replicates independent shifts (default 16) of num_paths /
+ *         qmc_replicates points each (a power of two keeps each replicate a
+ *         complete net); every value is the mean over the replicates and its
+ *         standard error is their standard deviation / sqrt(replicates).
+ *         Paths must draw a bounded number of uniforms: SchemeNV or
+ *         SchemeNCI factors (or none), no jumps or Merton jumps, and at most
+ *         1,024 coordinates per path -- per step 1 for the stock, 1 per NV
+ *         and 2 per NCI factor and 2 for a path-dependent instrument's
+ *         Merton jump, plus 2 for a European's terminal jump -- else
+ *         MYQL_E_UNSUPPORTED. */
+enum { MYQL_STREAMS_JUMP = 0, MYQL_STREAMS_SEEDED = 1, MYQL_STREAMS_SOBOL = 2 };
 /* Config flags.
  *  REFERENCE_NCI_ROWS  use the reference's SchemeNCI table semantics
  *      (VolSchemes.hpp:245-255, 342-360): n_max from v_max = max(v0, 3 theta),
@@ -165,6 +181,7 @@ typedef struct {
   int32_t stream_layout; /* MYQL_STREAMS_* */
   uint64_t num_streams;  /* S; 0 = library default (65536, at most num_paths) */
   uint32_t flags;        /* MYQL_FLAG_* */
+  uint32_t qmc_replicates; /* MYQL_STREAMS_SOBOL: independent shifts; 0 = 16 */
 } myql_mc_config;
 
 /* ---- MonteCarloResult<Mode, T> (PricingTypes.hpp:46-97) ----------------
@@ -423,6 +440,14 @@ int myql_debug_poissinv(myql_ctx *ctx, const double *lambda, const double *u,
  * (lambda[i * n_slots + k], u[i]). No counterpart in the reference. */
 int myql_debug_poisson_counts(myql_ctx *ctx, const double *lambda, int32_t n_slots, const double *u, int64_t n,
                               int32_t *out);
+/* Debug: the randomized quasi-Monte Carlo coordinates the path kernels
+ * draw (MYQL_STREAMS_SOBOL) for replicate `replicate` of seed `seed`:
+ * out[i * dims + d] is coordinate d (as the 32-bit integer x of
+ * u = (x + 1/2) 2^-32) of point i < n_points when the points are spread
+ * over n_streams lanes (a multiple of 64) as a pricing call spreads them.
+ * dims <= 1,024, n_points <= 2^24. No counterpart in the reference. */
+int myql_debug_qmc_points(myql_ctx *ctx, uint64_t seed, uint64_t replicate, int32_t dims, uint64_t n_points,
+                          uint64_t n_streams, uint32_t *out);
 /* Device chi-squared quantile (inverse regularised incomplete gamma). */
 int myql_debug_chi2_quantile(myql_ctx *ctx, const double *dof, const double *p,
                              int64_t n, double *out);

@@ -23,7 +23,7 @@ BARRIER_UP, BARRIER_DOWN = range(2)
 BARRIER_IN, BARRIER_OUT = range(2)
 REBATE_NONE, REBATE_AT_HIT, REBATE_AT_MATURITY = range(3)
 FP64, FP32 = range(2)
-STREAMS_JUMP, STREAMS_SEEDED = range(2)
+STREAMS_JUMP, STREAMS_SEEDED, STREAMS_SOBOL = range(3)
 FLAG_REFERENCE_NCI_ROWS = 1
 FLAG_AUTOTUNE = 2
 MLMC_GILES, MLMC_ML2R = range(2)
@@ -54,7 +54,7 @@ class MCConfig(C.Structure):
     _fields_ = [("num_paths", C.c_uint64), ("time_steps", C.c_uint64), ("seed", C.c_uint64),
                 ("fd_bump", C.c_double), ("vol_bump", C.c_double), ("t_bump", C.c_double),
                 ("r_bump", C.c_double), ("precision", C.c_int32), ("stream_layout", C.c_int32),
-                ("num_streams", C.c_uint64), ("flags", C.c_uint32)]
+                ("num_streams", C.c_uint64), ("flags", C.c_uint32), ("qmc_replicates", C.c_uint32)]
 
 
 _PD = C.POINTER(C.c_double)
@@ -139,7 +139,7 @@ EXPORTED = [
     "myql_create", "myql_destroy", "myql_last_error", "myql_tuning_report", "myql_mc_calculate",
     "myql_mc_calculate_noalloc", "myql_mc_calculate_batch", "myql_mlmc_calculate", "myql_fourier_calculate_batch", "myql_cv_train",
     "myql_mc_calculate_batch_cv", "myql_debug_stream_draws", "myql_debug_normcdfinv",
-    "myql_debug_poissinv", "myql_debug_poisson_counts", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
+    "myql_debug_poissinv", "myql_debug_poisson_counts", "myql_debug_qmc_points", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
     "myql_debug_cv_input_grad",
     "myql_debug_cv_param_grad",
 ]
@@ -185,6 +185,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_normcdfinv.argtypes = [vp, _PD, C.c_int64, _PD, C.POINTER(C.c_float)]
     lib.myql_debug_poissinv.restype = C.c_int
     lib.myql_debug_poissinv.argtypes = [vp, _PD, _PD, C.c_int64, _PD]
+    lib.myql_debug_qmc_points.restype = C.c_int
+    lib.myql_debug_qmc_points.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_int32, C.c_uint64, C.c_uint64,
+                                          C.POINTER(C.c_uint32)]
     lib.myql_debug_poisson_counts.restype = C.c_int
     lib.myql_debug_poisson_counts.argtypes = [vp, _PD, C.c_int32, _PD, C.c_int64, C.POINTER(C.c_int32)]
     lib.myql_debug_chi2_quantile.restype = C.c_int

@@ -50,6 +50,57 @@ __device__ __forceinline__ double uniform(Rng &g) {
   }
 }
 
+// Randomized quasi-Monte Carlo draws (Sobol' points, digitally shifted):
+// the d-th uniform of a path is coordinate d of the path's point,
+// hi[d] ^ lo[64 d], where hi holds the wavefront's shifted upper-bit part of
+// every coordinate (in LDS, updated once per wave step) and lo points at
+// the lane's column of the lower-6-bit table (see mc_paths_body); the
+// copies a replay takes restart at the same coordinate, as a copied stream
+// state would. u = (x + 1/2) 2^-32 lies in (0, 1).
+constexpr int QMC_DMAX = 1024;  // coordinates per quasi-Monte Carlo point
+struct Qrng {
+  const uint32_t *hi;
+  const uint32_t *lo;
+  int d;
+};
+
+// next coordinate as a 32-bit integer
+__device__ __forceinline__ uint32_t qmc_next(Qrng &q) {
+  // the host sizes the coordinates for the paths' draws; the clamp only
+  // keeps every read inside the tables
+  const int d = q.d < QMC_DMAX ? q.d : QMC_DMAX - 1;
+  const uint32_t x = q.hi[d] ^ q.lo[64 * d];
+  ++q.d;
+  return x;
+}
+
+__device__ __forceinline__ double uniform(Qrng &q) { return (static_cast<double>(qmc_next(q)) + 0.5) * 0x1p-32; }
+
+// The wavefront's upper-bit part of every coordinate (hi[d], d < dims) for
+// wave step h: shift[d] ^ the XOR of V[k][d] over the set bits k >= 6 of
+// gray(h) << 6. Each lane fills the coordinates d = lane (mod 64).
+__device__ __forceinline__ void qmc_hi_init(uint32_t *hi, const uint32_t *__restrict__ V,
+                                            const uint32_t *__restrict__ shift, int dims, uint64_t h, int lane) {
+  const uint64_t gc = h ^ (h >> 1);
+  for (int d = lane; d < dims; d += 64) {
+    uint32_t x = shift[d];
+    uint64_t b = gc;
+    for (int k = 6; b != 0 && k < 32; ++k, b >>= 1)
+      if (b & 1) x ^= V[k * QMC_DMAX + d];
+    hi[d] = x;
+  }
+}
+
+// From wave step h - 1 to h (h >= 1): gray(h) differs from gray(h - 1) in
+// bit ctz(h) only.
+__device__ __forceinline__ void qmc_hi_step(uint32_t *hi, const uint32_t *__restrict__ V, int dims, uint64_t h,
+                                            int lane) {
+  // points stay below 2^31 (host-checked), so k <= 30; the clamp only keeps
+  // the read inside the table
+  const int k = min(6 + __builtin_ctzll(h), 31);
+  for (int d = lane; d < dims; d += 64) hi[d] ^= V[k * QMC_DMAX + d];
+}
+
 __device__ __forceinline__ uint64_t splitmix(uint64_t &x) {
   x += 0x9E3779B97F4A7C15ull;
   uint64_t z = x;
@@ -494,8 +545,8 @@ __device__ __forceinline__ int jump_count(double mean, double exp_mean, double u
 }
 
 // Exponential(rate) by inversion (replaces Boost's ziggurat: one draw each).
-template <typename R>
-__device__ __forceinline__ R exponential(R rate, Rng &g) {
+template <typename R, typename Gen>
+__device__ __forceinline__ R exponential(R rate, Gen &g) {
   return static_cast<R>(-log1p(-uniform(g))) / rate;
 }
 

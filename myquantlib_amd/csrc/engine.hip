@@ -314,6 +314,37 @@ __global__ void __launch_bounds__(BLOCK) debug_poisson_counts_kernel(const doubl
     for (int k = 0; k < NV; ++k) out[i * NV + k] = Ns[k];
 }
 
+// The quasi-Monte Carlo coordinates the path kernels draw: lane l of wave w
+// (S / 64 waves) at iteration it < n_it reads point 64 (w n_it + it) + l
+// through the same hi-table walk (qmc_hi_init, qmc_hi_step) and Qrng
+// reads; out[i * dims + d] = coordinate d of point i < n_points.
+__global__ void __launch_bounds__(BLOCK) debug_qmc_kernel(const uint32_t *__restrict__ V,
+                                                          const uint32_t *__restrict__ lo,
+                                                          const uint32_t *__restrict__ shift, int dims,
+                                                          uint64_t n_waves, uint64_t n_it, uint64_t n_points,
+                                                          uint32_t *__restrict__ out) {
+  __shared__ uint32_t s_qhi[(BLOCK / 64) * QMC_DMAX];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (BLOCK / 64) + wave;
+  if (w >= n_waves) return;  // whole waves exit together
+  uint32_t *hi = s_qhi + wave * QMC_DMAX;
+  const uint64_t h0 = w * n_it;
+  qmc_hi_init(hi, V, shift, dims, h0, lane);
+  wave_sync();
+  for (uint64_t it = 0; it < n_it; ++it) {
+    if (it > 0) {
+      qmc_hi_step(hi, V, dims, h0 + it, lane);
+      wave_sync();
+    }
+    const uint64_t i = 64 * (h0 + it) + static_cast<uint64_t>(lane);
+    if (i < n_points) {
+      Qrng q{hi, lo + lane, 0};
+      for (int d = 0; d < dims; ++d) out[i * static_cast<uint64_t>(dims) + d] = qmc_next(q);
+    }
+    wave_sync();
+  }
+}
+
 __global__ void debug_chi2_kernel(const double *__restrict__ dof, const double *__restrict__ p, int64_t n,
                                   double *__restrict__ out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -364,6 +395,10 @@ struct myql_ctx {
   // by later calls) and launch arrays
   std::vector<std::unique_ptr<CvRecord>> cv_recs;
   DevBuf cv_jobs, cv_sets, cv_tasks, cv_part, cv_sums, cv_coef;
+  // randomized QMC: Sobol' direction integers [32][QMC_DMAX] and the
+  // lower-6-bit table [QMC_DMAX][64] (built at the first such call), and
+  // the digital shifts of a call's replicates
+  DevBuf qmc_V, qmc_lo, qmc_shift;
 };
 
 // Device memory of myql_mc_calculate_noalloc: stream states, job blocks,
@@ -577,6 +612,127 @@ double expected_avg_variance(const myql_model &m, double T) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Randomized quasi-Monte Carlo (MYQL_STREAMS_SOBOL)
+// ---------------------------------------------------------------------------
+uint64_t qmc_reps(const myql_mc_config &cfg) { return cfg.qmc_replicates ? cfg.qmc_replicates : 16; }
+
+// Uniforms a path draws, at most: per step the stock's normal, one per NV
+// factor and two per NCI factor (the Poisson and chi-squared uniforms), two
+// per step for a path-dependent instrument's Merton jump (count, size), and
+// two for a European's terminal Merton jump; a zero-factor European takes
+// one step to maturity.
+uint64_t qmc_dims(const myql_model &m, const int32_t *schemes, const myql_instrument &in, uint64_t steps) {
+  const bool pd = in.kind != MYQL_INSTR_EUROPEAN;
+  const bool jumps = m.jump_kind == MYQL_JUMP_MERTON;
+  uint64_t per_step = 1;
+  for (int f = 0; f < m.n_factors; ++f) per_step += schemes[f] == MYQL_SCHEME_NCI ? 2 : 1;
+  if (pd && jumps) per_step += 2;
+  const uint64_t n = (m.n_factors == 0 && !pd) ? 1 : steps;
+  return n * per_step + (!pd && jumps ? 2 : 0);
+}
+
+uint64_t splitmix64_host(uint64_t &x) {
+  x += 0x9E3779B97F4A7C15ull;
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// x has order 2^s - 1 modulo p (bit i = coefficient of x^i, degree s).
+bool gf2_primitive(uint32_t p, int s) {
+  auto mulmod = [&](uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    while (b) {
+      if (b & 1) r ^= a;
+      b >>= 1;
+      a <<= 1;
+      if ((a >> s) & 1) a ^= p;
+    }
+    return r;
+  };
+  auto xpow = [&](uint32_t e) {
+    uint32_t r = 1, base = 2;
+    while (e) {
+      if (e & 1) r = mulmod(r, base);
+      base = mulmod(base, base);
+      e >>= 1;
+    }
+    return r;
+  };
+  if (!((p >> s) & 1) || !(p & 1)) return false;
+  const uint32_t order = (1u << s) - 1;
+  if (xpow(order) != 1) return false;
+  uint32_t n = order;
+  for (uint32_t f = 2; f * f <= n; ++f) {
+    if (n % f) continue;
+    if (xpow(order / f) == 1) return false;
+    while (n % f == 0) n /= f;
+  }
+  return n == 1 || xpow(order / n) != 1;
+}
+
+// Sobol' direction integers V[k][d] (k = 0..31 for bits 1..32 of the
+// point index, d < QMC_DMAX), Bratley & Fox: coordinate 0 the van der
+// Corput sequence; coordinate d >= 1 the d-th primitive polynomial over
+// GF(2) in order of degree and encoding, odd initial direction integers
+// m_k < 2^k from splitmix64 seeded with d, and the recurrence
+// m_k = 2 a_1 m_(k-1) ^ ... ^ 2^(s-1) a_(s-1) m_(k-s+1) ^ 2^s m_(k-s) ^ m_(k-s).
+std::vector<uint32_t> sobol_directions() {
+  std::vector<uint32_t> V(32 * QMC_DMAX);
+  for (int k = 1; k <= 32; ++k) V[(k - 1) * QMC_DMAX] = 1u << (32 - k);
+  int d = 1;
+  for (int s = 1; d < QMC_DMAX; ++s) {
+    for (uint32_t mid = 0; mid < (1u << (s - 1)) && d < QMC_DMAX; ++mid) {
+      const uint32_t p = (1u << s) | (mid << 1) | 1u;
+      if (!gf2_primitive(p, s)) continue;
+      uint64_t m[33] = {};
+      uint64_t st = static_cast<uint64_t>(d);
+      for (int k = 1; k <= s && k <= 32; ++k) {
+        const uint64_t z = splitmix64_host(st);
+        m[k] = k > 1 ? 2 * (z % (1ull << (k - 1))) + 1 : 1;
+      }
+      for (int k = s + 1; k <= 32; ++k) {
+        uint64_t v = m[k - s] ^ (m[k - s] << s);
+        for (int i = 1; i < s; ++i)
+          if ((p >> (s - i)) & 1) v ^= m[k - i] << i;
+        m[k] = v;
+      }
+      for (int k = 1; k <= 32; ++k) V[(k - 1) * QMC_DMAX + d] = static_cast<uint32_t>((m[k] << (32 - k)) & 0xFFFFFFFFull);
+      ++d;
+    }
+  }
+  return V;
+}
+
+// Direction and lower-bit tables on the context's device (built once).
+void qmc_tables(myql_ctx *c) {
+  if (c->qmc_V.p) return;
+  const std::vector<uint32_t> V = sobol_directions();
+  std::vector<uint32_t> lo(static_cast<size_t>(QMC_DMAX) * 64);
+  for (int d = 0; d < QMC_DMAX; ++d)
+    for (int l = 0; l < 64; ++l) {
+      uint32_t x = 0;
+      for (int k = 0; k < 6; ++k)
+        if ((l >> k) & 1) x ^= V[k * QMC_DMAX + d];
+      lo[static_cast<size_t>(d) * 64 + l] = x;
+    }
+  c->qmc_V.reserve(V.size() * sizeof(uint32_t));
+  c->qmc_lo.reserve(lo.size() * sizeof(uint32_t));
+  HIPCHECK(hipMemcpyAsync(c->qmc_V.p, V.data(), V.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->qmc_lo.p, lo.data(), lo.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+// Digital shift of replicate `rep`: splitmix64 seeded with
+// seed ^ rep * 0x9E3779B97F4A7C15, the low 32 bits of one output per
+// coordinate.
+void qmc_shift(uint64_t seed, uint64_t rep, int dims, uint32_t *out) {
+  uint64_t st = seed ^ (rep * 0x9E3779B97F4A7C15ull);
+  for (int d = 0; d < dims; ++d) out[d] = static_cast<uint32_t>(splitmix64_host(st) & 0xFFFFFFFFull);
+}
+
 void validate(const myql_model *m, const int32_t *schemes, const myql_instrument *in, const myql_mc_config *cfg) {
   if (!m || !schemes || !in || !cfg) throw Fail(MYQL_E_INVALID_ARG, "null argument");
   if (m->n_factors < 0 || m->n_factors > 2) throw Fail(MYQL_E_INVALID_ARG, "n_factors must be 0, 1 or 2");
@@ -590,6 +746,21 @@ void validate(const myql_model *m, const int32_t *schemes, const myql_instrument
   if (cfg->num_paths < 1 || cfg->time_steps < 1) throw Fail(MYQL_E_INVALID_ARG, "num_paths and time_steps must be >= 1");
   if (cfg->time_steps > (1ull << 30)) throw Fail(MYQL_E_INVALID_ARG, "time_steps too large");
   if (cfg->precision != MYQL_FP64 && cfg->precision != MYQL_FP32) throw Fail(MYQL_E_INVALID_ARG, "bad precision");
+  if (cfg->stream_layout < MYQL_STREAMS_JUMP || cfg->stream_layout > MYQL_STREAMS_SOBOL)
+    throw Fail(MYQL_E_INVALID_ARG, "bad stream layout");
+  if (cfg->stream_layout == MYQL_STREAMS_SOBOL) {
+    for (int i = 0; i < m->n_factors; ++i)
+      if (schemes[i] == MYQL_SCHEME_EXACT)
+        throw Fail(MYQL_E_UNSUPPORTED, "quasi-Monte Carlo: SchemeExact draws an unbounded number of uniforms");
+    if (m->jump_kind == MYQL_JUMP_KOU)
+      throw Fail(MYQL_E_UNSUPPORTED, "quasi-Monte Carlo: Kou jumps draw an unbounded number of uniforms");
+    if (qmc_dims(*m, schemes, *in, cfg->time_steps) > QMC_DMAX)
+      throw Fail(MYQL_E_UNSUPPORTED, "quasi-Monte Carlo: more than 1,024 coordinates per path");
+    const uint64_t R = qmc_reps(*cfg);
+    if (R < 2) throw Fail(MYQL_E_INVALID_ARG, "quasi-Monte Carlo: the standard error needs at least 2 replicates");
+    if (cfg->num_paths < R) throw Fail(MYQL_E_INVALID_ARG, "quasi-Monte Carlo: fewer paths than replicates");
+    if (cfg->num_paths / R > (1ull << 30)) throw Fail(MYQL_E_INVALID_ARG, "quasi-Monte Carlo: over 2^30 points per replicate");
+  }
 }
 
 // Stepper constants of one variant (ASVJstepper.hpp:85-117, init_cir_impl
@@ -689,6 +860,7 @@ struct JobSpec {
   double S0, r, q;
   myql_mc_result *res;
   int k0, nk;  // strike chunk [k0, k0+nk) of the instrument
+  int qmc_rep = -1;  // MYQL_STREAMS_SOBOL: the replicate (digital shift) this job prices
 };
 
 // True when the two calls simulate identical paths: same model, schemes,
@@ -698,6 +870,7 @@ struct JobSpec {
 bool same_paths(const JobSpec &x, const JobSpec &y) {
   const myql_mc_config &a = *x.cfg, &b = *y.cfg;
   const int nf = x.model->n_factors;
+  if (x.qmc_rep != y.qmc_rep) return false;
   if (std::memcmp(x.model, y.model, sizeof(myql_model)) != 0) return false;
   for (int f = 0; f < nf; ++f)
     if (x.schemes[f] != y.schemes[f]) return false;
@@ -712,6 +885,7 @@ bool same_paths(const JobSpec &x, const JobSpec &y) {
 struct BuiltJob {
   int nf, nv;        // kernel instantiation
   bool pd;
+  bool qmc;          // randomized quasi-Monte Carlo points
   int n_slots;       // work units
   uint64_t cost;     // per-unit cost estimate
   int nk;
@@ -914,7 +1088,17 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
   J.M = M;
   J.S = S;
   J.steps = steps;
-  J.streams = stream_states(c, cfg.seed, cfg.stream_layout, S);
+  if (cfg.stream_layout == MYQL_STREAMS_SOBOL) {
+    // the replicate's shift is placed by run_jobs (qmc_shift: an offset
+    // into the call's shift buffer until then)
+    qmc_tables(c);
+    J.streams = nullptr;
+    J.qmc_V = c->qmc_V.as<uint32_t>();
+    J.qmc_lo = c->qmc_lo.as<uint32_t>();
+    J.qmc_dims = static_cast<int>(qmc_dims(M0, js.schemes, in, steps));
+  } else {
+    J.streams = stream_states(c, cfg.seed, cfg.stream_layout, S);
+  }
 
   // strikes of this chunk
   J.nk = js.nk;
@@ -927,6 +1111,7 @@ BuiltJob build_job(myql_ctx *c, const JobSpec &js, Job<R> &J, std::vector<double
   b.nf = nf;
   b.nv = nv;
   b.pd = pd;
+  b.qmc = cfg.stream_layout == MYQL_STREAMS_SOBOL;
   b.n_slots = static_cast<int>((S + BLOCK - 1) / BLOCK);
   const uint64_t per_stream = (M + S - 1) / S;
   b.cost = per_stream * steps * static_cast<uint64_t>(nsim) * (nf == 0 ? 1 : nf) + per_stream * js.nk;
@@ -969,19 +1154,43 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
     }
   }
 
+  // digital shifts of the quasi-Monte Carlo jobs' replicates
+  {
+    size_t words = 0;
+    for (size_t i = 0; i < jobs.size(); ++i)
+      if (built[i].qmc) words += static_cast<size_t>(jobs[i].qmc_dims);
+    if (words) {
+      std::vector<uint32_t> sh(words);
+      size_t off = 0;
+      std::vector<size_t> at(jobs.size(), 0);
+      for (size_t i = 0; i < jobs.size(); ++i) {
+        if (!built[i].qmc) continue;
+        at[i] = off;
+        qmc_shift(built[i].spec.cfg->seed, static_cast<uint64_t>(built[i].spec.qmc_rep), jobs[i].qmc_dims,
+                  sh.data() + off);
+        off += static_cast<size_t>(jobs[i].qmc_dims);
+      }
+      uint32_t *d_sh = static_cast<uint32_t *>(dev_mem(c, c->qmc_shift, words * sizeof(uint32_t)));
+      HIPCHECK(hipMemcpyAsync(d_sh, sh.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      for (size_t i = 0; i < jobs.size(); ++i)
+        if (built[i].qmc) jobs[i].qmc_shift = d_sh + at[i];
+    }
+  }
+
   // kernel groups (one persistent launch each) and their launch shapes:
   // the tuned shape where the autotuner has chosen one, else the default
-  using GKey = std::tuple<int, int, bool>;
+  using GKey = std::tuple<int, int, bool, bool>;  // nf, nv, pd, qmc
   const int prec = sizeof(R) == 4 ? MYQL_FP32 : MYQL_FP64;
   bool autotune = false;
   for (const BuiltJob &b : built) autotune = autotune || (b.spec.cfg->flags & MYQL_FLAG_AUTOTUNE);
   if (c->noalloc) autotune = false;  // trial launches would need buffers of their own
   auto shape_of = [&](const GKey &k) -> std::pair<McShape, int> {
+    if (std::get<3>(k)) return {MC_QMC_SHAPE, 4};
     auto it = c->tuned.find(std::make_tuple(prec, std::get<0>(k), std::get<1>(k), std::get<2>(k)));
     if (it != c->tuned.end()) return {it->second.shape, it->second.gridf};
     return {MC_DEFAULT_SHAPE, 4};
   };
-  auto gkey = [&](size_t i) { return GKey{built[i].nf, built[i].nv, built[i].pd}; };
+  auto gkey = [&](size_t i) { return GKey{built[i].nf, built[i].nv, built[i].pd, built[i].qmc}; };
   auto slots_for = [&](size_t i, int blk) { return static_cast<int>((jobs[i].S + blk - 1) / blk); };
 
   // slots and sums layout; partials are laid out for the smallest block so
@@ -1069,6 +1278,7 @@ void run_jobs(myql_ctx *c, const std::vector<JobSpec> &specs, double &device_ms)
   if (autotune) {
     for (auto &kv : members) {
       const GKey &k = kv.first;
+      if (std::get<3>(k)) continue;  // one compiled shape
       const auto tkey = std::make_tuple(prec, std::get<0>(k), std::get<1>(k), std::get<2>(k));
       if (c->tuned.count(tkey)) continue;
       const int n_shapes = mc_tune_shape_count(std::get<0>(k), std::get<1>(k), std::get<2>(k));
@@ -2137,11 +2347,26 @@ int fail(myql_ctx *c, const Fail &e) {
 }
 
 // Split jobs into strike chunks of at most 64 and run them by precision.
+// A quasi-Monte Carlo call's replicates: each prices num_paths / R points
+// under its own digital shift into its own result arrays; the caller's
+// result gets their means and standard errors.
+struct QmcRep {
+  myql_mc_config cfg;
+  myql_mc_result res;
+  std::vector<std::vector<double>> store;
+};
+
+// value / standard-error field pairs of a result
+using ResField = double *myql_mc_result::*;
+const std::pair<ResField, ResField> kResPairs[] = {
+    {&myql_mc_result::price, &myql_mc_result::price_std_err}, {&myql_mc_result::delta, &myql_mc_result::delta_std_err},
+    {&myql_mc_result::gamma, &myql_mc_result::gamma_std_err}, {&myql_mc_result::theta, &myql_mc_result::theta_std_err},
+    {&myql_mc_result::rho, &myql_mc_result::rho_std_err},     {&myql_mc_result::charm, &myql_mc_result::charm_std_err}};
+
 void run_specs(myql_ctx *c, const std::vector<JobSpec> &in_specs) {
   std::vector<JobSpec> s32, s64;
-  for (const JobSpec &s : in_specs) {
-    validate(s.model, s.schemes, s.instr, s.cfg);
-    if (!s.res) throw Fail(MYQL_E_INVALID_ARG, "null result");
+  std::vector<std::vector<std::unique_ptr<QmcRep>>> reps(in_specs.size());
+  auto push = [&](const JobSpec &s) {
     const int nk = static_cast<int>(s.instr->n_strikes);
     for (int k0 = 0; k0 < nk; k0 += STRIKE_CHUNK) {
       JobSpec t = s;
@@ -2149,12 +2374,88 @@ void run_specs(myql_ctx *c, const std::vector<JobSpec> &in_specs) {
       t.nk = std::min(STRIKE_CHUNK, nk - k0);
       (s.cfg->precision == MYQL_FP32 ? s32 : s64).push_back(t);
     }
+  };
+  for (size_t si = 0; si < in_specs.size(); ++si) {
+    const JobSpec &s = in_specs[si];
+    validate(s.model, s.schemes, s.instr, s.cfg);
+    if (!s.res) throw Fail(MYQL_E_INVALID_ARG, "null result");
     s.res->device_ms = 0.0;
+    if (s.cfg->stream_layout != MYQL_STREAMS_SOBOL) {
+      push(s);
+      continue;
+    }
+    const uint64_t R = qmc_reps(*s.cfg);
+    const size_t nk = static_cast<size_t>(s.instr->n_strikes);
+    for (uint64_t r = 0; r < R; ++r) {
+      auto q = std::make_unique<QmcRep>();
+      q->cfg = *s.cfg;
+      q->cfg.num_paths = s.cfg->num_paths / R;
+      q->res = myql_mc_result{};
+      auto give = [&](double *const &user, double *&mine) {
+        if (!user) return;
+        q->store.emplace_back(nk, 0.0);
+        mine = q->store.back().data();
+      };
+      for (const auto &pr : kResPairs) {
+        give(s.res->*pr.first, q->res.*pr.first);
+        give(s.res->*pr.second, q->res.*pr.second);
+      }
+      for (int f = 0; f < 2; ++f) {
+        give(s.res->vega[f], q->res.vega[f]);
+        give(s.res->vega_std_err[f], q->res.vega_std_err[f]);
+        give(s.res->vanna[f], q->res.vanna[f]);
+        give(s.res->vanna_std_err[f], q->res.vanna_std_err[f]);
+      }
+      JobSpec t = s;
+      t.cfg = &q->cfg;
+      t.res = &q->res;
+      t.qmc_rep = static_cast<int>(r);
+      reps[si].push_back(std::move(q));
+      push(t);
+    }
   }
   double ms32 = 0.0, ms64 = 0.0;
   if (!s32.empty()) run_jobs<float>(c, s32, ms32);
   if (!s64.empty()) run_jobs<double>(c, s64, ms64);
-  for (const JobSpec &s : in_specs) s.res->device_ms = ms32 + ms64;
+  for (size_t si = 0; si < in_specs.size(); ++si) {
+    const JobSpec &s = in_specs[si];
+    s.res->device_ms = ms32 + ms64;
+    if (reps[si].empty()) continue;
+    // replicate means, and their standard deviation / sqrt(R) as the error
+    const auto &rs = reps[si];
+    const double R = static_cast<double>(rs.size());
+    const size_t nk = static_cast<size_t>(s.instr->n_strikes);
+    auto combine_arr = [&](double *val, double *err, auto get) {
+      if (!val) return;
+      for (size_t k = 0; k < nk; ++k) {
+        double m = 0.0;
+        for (const auto &q : rs) m += get(*q)[k];
+        m /= R;
+        double v = 0.0;
+        for (const auto &q : rs) {
+          const double d = get(*q)[k] - m;
+          v += d * d;
+        }
+        val[k] = m;
+        if (err) err[k] = rs.size() > 1 ? std::sqrt(v / (R - 1.0) / R) : 0.0;
+      }
+    };
+    // the fields the Greek mode produces (the others are left alone, as in
+    // a pseudo-random call): price; delta, gamma; the Full Greeks
+    const int n_pairs = s.mode == MYQL_GREEK_FULL ? 6 : (s.mode == MYQL_GREEK_ESSENTIAL ? 3 : 1);
+    for (int p = 0; p < n_pairs; ++p) {
+      const auto &pr = kResPairs[p];
+      combine_arr(s.res->*pr.first, s.res->*pr.second, [&](const QmcRep &q) { return q.res.*pr.first; });
+    }
+    if (s.mode == MYQL_GREEK_FULL)
+      for (int f = 0; f < 2; ++f) {
+        combine_arr(s.res->vega[f], s.res->vega_std_err[f], [&](const QmcRep &q) { return q.res.vega[f]; });
+        combine_arr(s.res->vanna[f], s.res->vanna_std_err[f], [&](const QmcRep &q) { return q.res.vanna[f]; });
+      }
+    uint64_t steps = 0;
+    for (const auto &q : rs) steps += q->res.total_steps;
+    s.res->total_steps = steps;
+  }
 }
 
 }  // namespace
@@ -2437,6 +2738,9 @@ int myql_mc_calculate_noalloc(myql_ctx *c, const myql_model *model, const int32_
     HIPCHECK(hipSetDevice(c->device));
     if (!c->scratch) throw Fail(MYQL_E_HIP, "myql_mc_calculate_noalloc: no scratch address for this device");
     if (greek_mode < MYQL_GREEK_NONE || greek_mode > MYQL_GREEK_FULL) throw Fail(MYQL_E_INVALID_ARG, "bad greek mode");
+    if (cfg && cfg->stream_layout == MYQL_STREAMS_SOBOL)
+      throw Fail(MYQL_E_UNSUPPORTED, "myql_mc_calculate_noalloc: quasi-Monte Carlo tables are allocated (use "
+                                     "myql_mc_calculate)");
     struct Mode {
       myql_ctx *c;
       explicit Mode(myql_ctx *x) : c(x) {
@@ -2686,6 +2990,35 @@ int myql_debug_poisson_counts(myql_ctx *c, const double *lambda, int32_t n_slots
                          du.as<double>(), n, dout.as<int32_t>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(out, dout.p, n * n_slots * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
+}
+
+int myql_debug_qmc_points(myql_ctx *c, uint64_t seed, uint64_t replicate, int32_t dims, uint64_t n_points,
+                          uint64_t n_streams, uint32_t *out) {
+  if (!c || !out || dims < 1 || dims > QMC_DMAX || n_points == 0 || n_streams == 0 || n_streams % 64 != 0 ||
+      n_points > (1ull << 24))
+    return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    qmc_tables(c);
+    std::vector<uint32_t> sh(static_cast<size_t>(dims));
+    qmc_shift(seed, replicate, dims, sh.data());
+    DevBuf dsh, dout;
+    dsh.reserve(sh.size() * sizeof(uint32_t));
+    dout.reserve(n_points * static_cast<size_t>(dims) * sizeof(uint32_t));
+    HIPCHECK(hipMemcpyAsync(dsh.p, sh.data(), sh.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    const uint64_t n_waves = n_streams / 64, n_it = (n_points + n_streams - 1) / n_streams;
+    const unsigned nb = static_cast<unsigned>((n_waves + BLOCK / 64 - 1) / (BLOCK / 64));
+    hipLaunchKernelGGL(debug_qmc_kernel, dim3(nb), dim3(BLOCK), 0, c->stream, c->qmc_V.as<uint32_t>(),
+                       c->qmc_lo.as<uint32_t>(), dsh.as<uint32_t>(), static_cast<int>(dims), n_waves, n_it, n_points,
+                       dout.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, dout.p, n_points * static_cast<size_t>(dims) * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     return MYQL_OK;
   } catch (const Fail &e) {

@@ -91,6 +91,14 @@ struct Job {
   // never; the extended-rows default already serves every row it holds)
   int nci_rows_acc[2];
   double nci_u_acc, nci_u_acc_hi;
+  // randomized QMC (MYQL_STREAMS_SOBOL): Sobol' direction integers
+  // V[k][d] (k = 0..31, QMC_DMAX coordinates each), the lower-6-bit table
+  // lo[d][l], this replicate's digital shift (qmc_dims words), and the
+  // coordinates a path may draw (0: pseudo-random streams)
+  const uint32_t *qmc_V;
+  const uint32_t *qmc_lo;
+  const uint32_t *qmc_shift;
+  int qmc_dims;
   const double *strikes;
   int nk, nkp;                     // strikes in this job, padded to a power of 2 (<= 64)
   double *partial;                 // per unit: 2*NSTAT x nk sums

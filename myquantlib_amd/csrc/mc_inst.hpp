@@ -24,8 +24,20 @@ template <>
 hipError_t launch_mc<MYQL_R>(int nf, int nv, bool pd, McShape shape, const Job<MYQL_R> *jobs, const Unit *units,
                              int n_units, unsigned *counter, double *accg, size_t smem, int grid,
                              hipStream_t stream) {
+#define MYQL_LAUNCH_Q(NF_, NV_, PD_, B_, U_)                                                                  \
+  if (nf == NF_ && nv == NV_ && pd == PD_) {                                                                 \
+    hipLaunchKernelGGL((mc_paths_kernel_qmc<MYQL_R, NF_, NV_, PD_>), dim3(grid), dim3(BLOCK), smem, stream,   \
+                       jobs, units, n_units, counter, accg);                                                 \
+    return hipGetLastError();                                                                                \
+  }
+  if (shape.qmc) {
+    MYQL_MC_KERNELS(MYQL_LAUNCH_Q, BLOCK, 4)
+    return hipErrorInvalidValue;
+  }
+#undef MYQL_LAUNCH_Q
 #define MYQL_LAUNCH_K(KERNEL, OCC_, NF_, NV_, PD_, B_, U_)                                                   \
-  if (nf == NF_ && nv == NV_ && pd == PD_ && shape.blk == B_ && shape.unr == U_ && shape.occ == OCC_) {       \
+  if (!shape.qmc && nf == NF_ && nv == NV_ && pd == PD_ && shape.blk == B_ && shape.unr == U_ &&              \
+      shape.occ == OCC_) {                                                                                   \
     if (smem > 64 * 1024)                                                                                    \
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(KERNEL<MYQL_R, NF_, NV_, PD_, B_, U_>),        \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));         \
@@ -46,8 +58,21 @@ hipError_t launch_mc<MYQL_R>(int nf, int nv, bool pd, McShape shape, const Job<M
 template <>
 int mc_blocks_per_cu<MYQL_R>(int nf, int nv, bool pd, McShape shape, size_t smem) {
   int nb = 0;
+#define MYQL_OCC_Q(NF_, NV_, PD_, B_, U_)                                                                     \
+  if (nf == NF_ && nv == NV_ && pd == PD_) {                                                                  \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mc_paths_kernel_qmc<MYQL_R, NF_, NV_, PD_>, BLOCK,   \
+                                                     smem) != hipSuccess)                                     \
+      nb = 1;                                                                                                 \
+    return nb > 0 ? nb : 1;                                                                                   \
+  }
+  if (shape.qmc) {
+    MYQL_MC_KERNELS(MYQL_OCC_Q, BLOCK, 4)
+    return 1;
+  }
+#undef MYQL_OCC_Q
 #define MYQL_OCC_K(KERNEL, OCC_, NF_, NV_, PD_, B_, U_)                                                       \
-  if (nf == NF_ && nv == NV_ && pd == PD_ && shape.blk == B_ && shape.unr == U_ && shape.occ == OCC_) {        \
+  if (!shape.qmc && nf == NF_ && nv == NV_ && pd == PD_ && shape.blk == B_ && shape.unr == U_ &&               \
+      shape.occ == OCC_) {                                                                                    \
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, KERNEL<MYQL_R, NF_, NV_, PD_, B_, U_>, B_, smem) !=   \
         hipSuccess)                                                                                            \
       nb = 1;                                                                                                  \

@@ -201,8 +201,8 @@ __device__ __forceinline__ void finalize_stage(const Job<R> *__restrict__ J, con
 }
 
 // ---- gamma variates of SchemeExact (VolSchemes.hpp:129-203) ---------------
-template <typename R>
-__device__ static R gamma_draw(double alpha_base, int N, Rng &g) {
+template <typename R, typename Gen>
+__device__ static R gamma_draw(double alpha_base, int N, Gen &g) {
   const double alpha = alpha_base + N;
   if (alpha > 1.0) {
     const double dd = alpha - 1.0 / 3.0;
@@ -245,9 +245,9 @@ __device__ static R gamma_draw(double alpha_base, int N, Rng &g) {
 }
 
 // ---- one lane-local log jump (JumpPolicies.hpp:55-135) ---------------------
-template <typename R>
+template <typename R, typename Gen>
 __device__ __forceinline__ R log_jump(const Job<R> *__restrict__ J, const VarParams<R> *__restrict__ P,
-                                      Rng &g) {
+                                      Gen &g) {
   const int n = jump_count(P->jmean, P->jexpm, uniform(g));
   if (n == 0) return P->jdrift;
   if (J->jump_kind == 1) {
@@ -266,9 +266,9 @@ __device__ __forceinline__ R log_jump(const Job<R> *__restrict__ J, const VarPar
 // European end-of-path jump for NV lockstep slots. All slots share the
 // Poisson uniform; slots needing more draws read them from a copy of the
 // stream, and the stream advances by what the base path (slot 0) consumed.
-template <typename R, int NV, bool TD = false, int KT = 0>
+template <typename R, int NV, bool TD = false, int KT = 0, typename Gen>
 __device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
-                                          const VarParams<R> *__restrict__ const *P, R *x, Rng &g,
+                                          const VarParams<R> *__restrict__ const *P, R *x, Gen &g,
                                           double *xt = nullptr) {
   if (J->jump_kind == 0) return;
   const double ua = uniform(g);
@@ -283,7 +283,7 @@ __device__ __forceinline__ void euro_jump(const Job<R> *__restrict__ J,
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = 0;
   if (nmax > 0) {
-    Rng t = g;
+    Gen t = g;
     if (J->jump_kind == 1) {
       const R Z = normcdfinv<R>(uniform(t));
 #pragma unroll
@@ -530,10 +530,10 @@ __device__ __forceinline__ double nci_row64(const double *__restrict__ tab, int 
 // ---- one variance factor step for all NV slots (VolSchemes.hpp) -----------
 // TD: slots KT, KT + 1 are stepped in FP64 from vt into vnt (their entries
 // of v / vn are not used).
-template <typename R, int NV, int NFX, bool TD = false, int KT = 0>
+template <typename R, int NV, int NFX, bool TD = false, int KT = 0, typename Gen>
 __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
                                               const VarParams<R> *__restrict__ const *P, int f,
-                                              R (&v)[NV][NFX], R (&vn)[NV][NFX], Rng &g,
+                                              R (&v)[NV][NFX], R (&vn)[NV][NFX], Gen &g,
                                               double (*vt)[NFX] = nullptr, double (*vnt)[NFX] = nullptr,
                                               int *n_out = nullptr) {
   const int sch = J->scheme[f];
@@ -693,8 +693,8 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
 // Simulate one variant group for the lane's current sample and stage it.
 // PD selects path-dependent trackers; European kernels carry no tracker
 // state at all.
-template <typename R, int NF, int NV, bool PD, int BLK = BLOCK>
-__device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfirst, Rng &g,
+template <typename R, int NF, int NV, bool PD, int BLK = BLOCK, typename Gen>
+__device__ __forceinline__ void sim_group(const Job<R> *__restrict__ J, int gfirst, Gen &g,
                                           R *__restrict__ stage, int tid) {
   constexpr int NFX = NF > 0 ? NF : 1;
   const VarParams<R> *__restrict__ P[NV];
@@ -1088,7 +1088,17 @@ __device__ __forceinline__ void unit_partials(const Job<R> *__restrict__ J, cons
 // Persistent work-queue kernel. `counter` must be zero at launch; `accg`
 // holds 2*NSTAT*BLK doubles per work unit of BLK streams; UNR unrolls the
 // strike loop over staged samples.
-template <typename R, int NF, int NV, bool PD, int BLK, int UNR>
+//
+// QMC (randomized quasi-Monte Carlo): lane s of the job (wave w = s / 64,
+// lane l = s % 64) prices at iteration it the point c = (gray(h) << 6) | l of
+// the digitally shifted Sobol' sequence, h = w n_it + it, valid while
+// 64 h + l < M; the paths draw the point's coordinates in order (Qrng). The
+// wave keeps, for every coordinate d < qmc_dims, hi[d] = shift[d] ^ the XOR
+// of V[k][d] over the set bits k >= 6 of c in LDS: built at the unit's
+// first point, then one XOR per coordinate per iteration (gray(h) and
+// gray(h - 1) differ in bit ctz(h) only). The lane's lower bits come from
+// qmc_lo[d][l], the XOR of V[k][d] over the set bits k < 6 of l.
+template <typename R, int NF, int NV, bool PD, int BLK, int UNR, bool QMC = false>
 __device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, const Unit *__restrict__ units,
                                               int n_units, unsigned *__restrict__ counter,
                                               double *__restrict__ accg_all) {
@@ -1096,6 +1106,7 @@ __device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, c
   __shared__ int s_unit;
   __shared__ int s_valid[BLK];
   __shared__ double s_red[BLK];
+  __shared__ uint32_t s_qhi[QMC ? (BLK / 64) * QMC_DMAX : 1];
   R *stage = reinterpret_cast<R *>(smem);
   const int tid = threadIdx.x;
 
@@ -1113,23 +1124,54 @@ __device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, c
     const uint64_t S = J->S, M = J->M;
     const uint64_t s = static_cast<uint64_t>(blk) * BLK + tid;
     const bool active = s < S;
-    Rng g;
-    if (active) {
-      g.s0 = J->streams[4 * s + 0];
-      g.s1 = J->streams[4 * s + 1];
-      g.s2 = J->streams[4 * s + 2];
-      g.s3 = J->streams[4 * s + 3];
-    } else {
-      g.s0 = g.s1 = g.s2 = g.s3 = 0;
-    }
     const uint64_t q = M / S, rem = M % S;
-    const uint64_t count = active ? q + (s < rem ? 1 : 0) : 0;
-    const uint64_t n_it = q + (static_cast<uint64_t>(blk) * BLK < rem ? 1 : 0);
+    uint64_t count = 0, n_it = 0, h0 = 0;
+    Rng g;
+    uint32_t *qhi = s_qhi + (QMC ? (tid >> 6) * QMC_DMAX : 0);
+    if constexpr (!QMC) {
+      if (active) {
+        g.s0 = J->streams[4 * s + 0];
+        g.s1 = J->streams[4 * s + 1];
+        g.s2 = J->streams[4 * s + 2];
+        g.s3 = J->streams[4 * s + 3];
+      } else {
+        g.s0 = g.s1 = g.s2 = g.s3 = 0;
+      }
+      count = active ? q + (s < rem ? 1 : 0) : 0;
+      n_it = q + (static_cast<uint64_t>(blk) * BLK < rem ? 1 : 0);
+    } else {
+      n_it = q + (rem ? 1 : 0);
+      h0 = (s >> 6) * n_it;
+      qmc_hi_init(qhi, J->qmc_V, J->qmc_shift, J->qmc_dims, h0, tid & 63);
+      wave_sync();
+    }
 
     for (uint64_t it = 0; it < n_it; ++it) {
-      const bool valid = it < count;
+      bool valid;
+      if constexpr (QMC) {
+        const uint64_t h = h0 + it;
+        if (it > 0) {
+          qmc_hi_step(qhi, J->qmc_V, J->qmc_dims, h, tid & 63);
+          wave_sync();
+        }
+        valid = active && 64 * h + static_cast<uint64_t>(tid & 63) < M;
+      } else {
+        valid = it < count;
+      }
       if (valid) {
-        if constexpr (NV > 1) {
+        if constexpr (QMC) {
+          Qrng qg{qhi, J->qmc_lo + (tid & 63), 0};
+          if constexpr (NV > 1) {
+            sim_group<R, NF, NV, PD, BLK>(J, 0, qg, stage, tid);
+          } else {
+            const Qrng snap = qg;
+            sim_group<R, NF, 1, PD, BLK>(J, 0, qg, stage, tid);
+            for (int gi = 1; gi < J->n_groups; ++gi) {
+              Qrng gg = snap;
+              sim_group<R, NF, 1, PD, BLK>(J, J->group_first[gi], gg, stage, tid);
+            }
+          }
+        } else if constexpr (NV > 1) {
           // lockstep: one group, the stream advances as the base path does
           sim_group<R, NF, NV, PD, BLK>(J, 0, g, stage, tid);
         } else {
@@ -1175,6 +1217,16 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
 mc_paths_kernel_occ4(const Job<R> *__restrict__ jobs, const Unit *__restrict__ units, int n_units,
                      unsigned *__restrict__ counter, double *__restrict__ accg_all) {
   mc_paths_body<R, NF, NV, PD, BLK, UNR>(jobs, units, n_units, counter, accg_all);
+}
+
+// The same kernel drawing randomized quasi-Monte Carlo points (one launch
+// shape: 256-lane blocks, strike loop unrolled by 4).
+template <typename R, int NF, int NV, bool PD>
+__global__ void __launch_bounds__(BLOCK) mc_paths_kernel_qmc(const Job<R> *__restrict__ jobs,
+                                                             const Unit *__restrict__ units, int n_units,
+                                                             unsigned *__restrict__ counter,
+                                                             double *__restrict__ accg_all) {
+  mc_paths_body<R, NF, NV, PD, BLOCK, 4, true>(jobs, units, n_units, counter, accg_all);
 }
 
 }  // namespace myql

@@ -19,10 +19,14 @@ bool mc_supported(int nf, int nv, bool pd);
 // (the strip workloads, whose strike loop dominates the staging phase) also
 // with the loop not unrolled and, at 64-stream units, in the four-wave
 // variant. The autotuner picks among them.
+// qmc = 1: the kernel drawing randomized quasi-Monte Carlo points
+// (mc_paths_kernel_qmc), compiled at the default shape only.
 struct McShape {
   int blk, unr, occ;
+  int qmc = 0;
 };
 constexpr McShape MC_DEFAULT_SHAPE{BLOCK, 4, 0};
+constexpr McShape MC_QMC_SHAPE{BLOCK, 4, 0, 1};
 constexpr McShape MC_TUNE_SHAPES[8] = {{256, 4, 0}, {128, 4, 0}, {64, 4, 0}, {256, 1, 0},
                                        {128, 1, 0}, {64, 1, 0},  {64, 4, 4}, {64, 1, 4}};
 // number of leading MC_TUNE_SHAPES compiled for a kernel

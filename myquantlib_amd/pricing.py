@@ -105,9 +105,9 @@ class InstrumentSpec:
 def mc_config(num_paths: int, time_steps: int, seed: int = 42, fd_bump: float = 1e-4,
               vol_bump: float = 1e-4, t_bump: float = 1e-4, r_bump: float = 1e-4,
               precision: int = capi.FP64, stream_layout: int = capi.STREAMS_JUMP,
-              num_streams: int = 0, flags: int = 0) -> MCConfig:
+              num_streams: int = 0, flags: int = 0, qmc_replicates: int = 0) -> MCConfig:
     return MCConfig(num_paths, time_steps, seed, fd_bump, vol_bump, t_bump, r_bump, precision,
-                    stream_layout, num_streams, flags)
+                    stream_layout, num_streams, flags, qmc_replicates)
 
 
 def cv_config(train_paths: int = 1 << 20, train_seed: int = 7, steps: int = 2000, batch: int = 8192,
@@ -416,6 +416,15 @@ class Engine:
         rc = self.lib.myql_debug_poisson_counts(self.ctx, lam.ctypes.data_as(pd), lam.shape[1],
                                                 u.ctypes.data_as(pd), u.size,
                                                 out.ctypes.data_as(C.POINTER(C.c_int32)))
+        self._check(rc)
+        return out
+
+    def qmc_points(self, seed: int, replicate: int, dims: int, n_points: int, n_streams: int) -> np.ndarray:
+        """The randomized QMC coordinates the path kernels draw, as uint32
+        (n_points, dims)."""
+        out = np.zeros((n_points, dims), dtype=np.uint32)
+        rc = self.lib.myql_debug_qmc_points(self.ctx, seed, replicate, dims, n_points, n_streams,
+                                            out.ctypes.data_as(C.POINTER(C.c_uint32)))
         self._check(rc)
         return out
 

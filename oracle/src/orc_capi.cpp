@@ -300,6 +300,8 @@ int oracle_mc_calculate(const myql_model *model, const int32_t *schemes, const m
     validate(model, schemes, instr);
     if (!cfg || !res) throw Error(MYQL_E_INVALID_ARG, "null argument");
     if (cfg->num_paths < 1 || cfg->time_steps < 1) throw Error(MYQL_E_INVALID_ARG, "num_paths and time_steps must be >= 1");
+    // quasi-Monte Carlo points are checked through oracle/sobol.py, not here
+    if (cfg->stream_layout == MYQL_STREAMS_SOBOL) throw Error(MYQL_E_UNSUPPORTED, "the oracle draws pseudo-random streams only");
     const myql_model &M0 = *model;
     Instr I;
     I.in = *instr;
