@@ -197,6 +197,42 @@ def cv_leg(eng, eval_paths: int, bates_kou: bool, payoff: int = 0, points: int =
             "options_using_cv": used, "options": int(vr.size), "train_loss": net.final_loss}
 
 
+def qmc_leg(eng, paths: int, po32_ms_to, reps: int = 16):
+    """Randomized quasi-Monte Carlo on the price-only strip (GreekMode None,
+    FP32 paths, the strip's model and maturities): `paths` Sobol' points per
+    maturity in `reps` digitally shifted replicates (MYQL_STREAMS_SOBOL;
+    3 coordinates per step, 602 at T = 2), against pseudo-random FP32 on
+    the same number of paths; ms to 1e-4 relative SE as for the other legs
+    (the replicates' standard error for QMC)."""
+    from myquantlib_amd import capi
+    Jq = jobs(paths, capi.FP32, mode=capi.GREEK_NONE, streams=min(paths // reps, STRIP_STREAMS))
+    for j in Jq:
+        j[4].stream_layout = capi.STREAMS_SOBOL
+        j[4].qmc_replicates = reps
+    Jp = jobs(paths, capi.FP32, mode=capi.GREEK_NONE, streams=min(paths, STRIP_STREAMS))
+    eng.mc_batch(Jq)
+    t0 = time.perf_counter()
+    rq = eng.mc_batch(Jq)
+    tq = 1e3 * (time.perf_counter() - t0)
+    eng.mc_batch(Jp)
+    t0 = time.perf_counter()
+    rp = eng.mc_batch(Jp)
+    tp = 1e3 * (time.perf_counter() - t0)
+    mq, wq = ms_to_target(rq, tq)
+    mp, wp = ms_to_target(rp, tp)
+    ratios = []
+    for a, b in zip(rq, rp):
+        ok = np.asarray(b["price"]) >= 0.1
+        ratios.extend((np.asarray(b["price_std_err"]) / np.asarray(a["price_std_err"]))[ok])
+    out = {"paths_per_maturity": paths, "replicates": reps, "qmc_fp32_ms": tq, "plain_fp32_ms": tp,
+           "qmc_fp32_ms_to_1e-4": mq, "plain_fp32_ms_to_1e-4": mp, "qmc_worst_rel_se": wq, "plain_worst_rel_se": wp,
+           "se_ratio_median": float(np.median(ratios)), "se_ratio_min": float(np.min(ratios)),
+           "speedup_vs_plain_fp32": mp / mq, "full_strip_plain_fp32_ms_to_1e-4": po32_ms_to}
+    print(f"# qmc leg: {tq:.0f} ms (plain {tp:.0f} ms), median SE ratio {out['se_ratio_median']:.2f}, "
+          f"ms to 1e-4: qmc {mq:.3g}, plain {mp:.3g} ({mp / mq:.2f}x)", file=sys.stderr, flush=True)
+    return out
+
+
 def path_steps(paths: int) -> int:
     return sum(paths * int(round(100 * T)) * VARIANTS for T in MATURITIES)
 
@@ -564,6 +600,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the price-only and control-variate legs")
     ap.add_argument("--no-mlmc", action="store_true", help="skip the multilevel leg")
+    ap.add_argument("--no-qmc", action="store_true", help="skip the quasi-Monte Carlo leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Double-Heston leg")
     ap.add_argument("--no-fourier", action="store_true", help="skip the Gil-Pelaez leg")
     ap.add_argument("--no-ref-rows", action="store_true",
@@ -665,6 +702,7 @@ def main():
         cv["bates_strip"] = cv_leg(eng, cv_paths, bates_kou=False)
         cv["bates_kou_nv_strip"] = cv_leg(eng, cv_paths, bates_kou=True)
         cv["bates_kou_nv_smoothed_digitals"] = cv_leg(eng, cv_paths, bates_kou=True, payoff=capi.PAYOFF_CASH)
+    qmc = None if (args.no_cv or args.no_qmc) else qmc_leg(eng, min(paths, 1 << 22), po32)
 
     mlmc = None if args.no_mlmc else mlmc_leg(eng)
     batched = None if args.no_batched else batched_leg(eng, flags=flags)
@@ -707,6 +745,7 @@ def main():
             "learned_cv_speedup_vs_plain_fp64": cv["bates_strip"]["speedup_vs_plain_fp64"] if cv else None,
             "full_greeks_fp32": t32, "full_greeks_fp64": ms64},
         "learned_cv": cv or None,
+        "qmc": qmc,
         "mlmc": mlmc,
         "batched_double_heston": batched,
         "fourier": fourier_leg,
