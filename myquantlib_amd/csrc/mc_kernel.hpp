@@ -883,13 +883,16 @@ __device__ static __noinline__ void eval3_pd(const Job<R> *__restrict__ J, const
 }
 
 // base / up / dn payoffs of staged variant v at sample s (calculate_to_buffer;
-// European.hpp:40-74 scales the terminal spot by (S0 +- h)/S0)
-template <typename R, int BLK = BLOCK>
+// European.hpp:40-74 scales the terminal spot by (S0 +- h)/S0). PD = false:
+// the kernel prices Europeans only (no tracker, no out-of-line call).
+template <typename R, int BLK = BLOCK, bool PD = true>
 __device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__restrict__ stage, int v,
                                       int s, double K, bool bumps, double &p, double &pu, double &pd) {
-  if (J->tracker != 0) {
-    eval3_pd<R, BLK>(J, stage, v, s, K, bumps, p, pu, pd);
-    return;
+  if constexpr (PD) {
+    if (J->tracker != 0) {
+      eval3_pd<R, BLK>(J, stage, v, s, K, bumps, p, pu, pd);
+      return;
+    }
   }
   const double S = static_cast<double>(stage[(F_SPOT * MAXV + v) * BLK + s]);
   const bool smooth = J->mode != 0;
@@ -923,8 +926,21 @@ __device__ __forceinline__ void eval3_spot(const Job<R> *__restrict__ J, double 
   pd = payoff(J, smooth, S * J->mult_dn, K);
 }
 
+// End of strike group [s0, s0 + nkp)'s valid samples. A batch's valid
+// samples form a prefix of each wavefront's lanes (a lane's sample count
+// falls as its stream index rises; a quasi-Monte Carlo wave's points are
+// valid up to a lane), and the group's samples are lanes of this lane's own
+// wavefront, so its valid samples are [s0, valid_end). Every lane of the
+// wavefront must make the call.
+__device__ __forceinline__ int valid_end(const int *__restrict__ valid, int tid, int s0, int nkp) {
+  const uint64_t vm = __ballot(valid[tid] != 0);
+  const uint64_t gm = nkp == 64 ? vm : (vm >> (s0 & 63)) & ((uint64_t{1} << nkp) - 1);
+  return s0 + __popcll(gm);
+}
+
 // REL: the maturity-bumped variants are staged in FP64 form (t_slots_fp64).
-template <typename R, int BLK, int UNR, bool REL = false>
+// PD: as in eval3.
+template <typename R, int BLK, int UNR, bool REL = false, bool PD = true>
 __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R *__restrict__ stage,
                                            const int *__restrict__ valid, int tid, double *__restrict__ accg,
                                            bool first) {
@@ -943,15 +959,14 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   const bool ess = mode != 0, full = mode == 2;
   const bool live = j < J->nk;
   const double K = live ? J->strikes[j] : 0.0;
-  const int s0 = grp * nkp, s1 = grp * nkp + nkp;
+  const int s0 = grp * nkp, s1 = valid_end(valid, tid, s0, nkp);
   {
     double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg = 0, sg2 = 0;
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
-        if (!valid[s]) continue;
         double p, pu = 0, pd = 0;
-        eval3<R, BLK>(J, stage, V_BASE, s, K, ess, p, pu, pd);
+        eval3<R, BLK, PD>(J, stage, V_BASE, s, K, ess, p, pu, pd);
         sp += p;
         sp2 += p * p;
         if (ess) {
@@ -980,10 +995,9 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
-        if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
-        eval3<R, BLK>(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
-        eval3<R, BLK>(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
+        eval3<R, BLK, PD>(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
+        eval3<R, BLK, PD>(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
         const double vg = (ap - bp) * J->inv2dv * J->cv[f];
         const double vn = ((au - ad) - (bu - bd)) * (J->inv2dv * J->inv2h) * J->cv[f];
         sv += vg;
@@ -1008,7 +1022,6 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
-        if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
         if constexpr (REL) {
           const double Su = static_cast<double>(stage[(F_SPOT * MAXV + V_TUP) * BLK + s]) +
@@ -1017,8 +1030,8 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
           eval3_spot(J, Su, K, ap, au, ad);
           eval3_spot(J, Sd, K, bp, bu, bd);
         } else {
-          eval3<R, BLK>(J, stage, V_TUP, s, K, true, ap, au, ad);
-          eval3<R, BLK>(J, stage, V_TDN, s, K, true, bp, bu, bd);
+          eval3<R, BLK, PD>(J, stage, V_TUP, s, K, true, ap, au, ad);
+          eval3<R, BLK, PD>(J, stage, V_TDN, s, K, true, bp, bu, bd);
         }
         const double th = (ap * J->dfTu - bp * J->dfTd) * J->inv2dT;
         const double ch = ((au - ad) * J->dfTu - (bu - bd) * J->dfTd) * (J->inv2dT * J->inv2h);
@@ -1038,7 +1051,6 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
-        if (!valid[s]) continue;
         double ap, bp, dum0, dum1;
         if (J->derive_r) {
           // European: the rate-bumped paths are the base path shifted by
@@ -1047,13 +1059,165 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
           ap = payoff(J, mode != 0, S * J->erb_up, K);
           bp = payoff(J, mode != 0, S * J->erb_dn, K);
         } else {
-          eval3<R, BLK>(J, stage, V_RUP, s, K, false, ap, dum0, dum1);
-          eval3<R, BLK>(J, stage, V_RDN, s, K, false, bp, dum0, dum1);
+          eval3<R, BLK, PD>(J, stage, V_RUP, s, K, false, ap, dum0, dum1);
+          eval3<R, BLK, PD>(J, stage, V_RDN, s, K, false, bp, dum0, dum1);
         }
         const double rh = (ap * J->dfru - bp * J->dfrd) * J->inv2dr;
         sr += rh;
         sr2 += rh * rh;
       }
+    }
+    flush<BLK>(accg, 2 * ST_RHO, tid, first, sr);
+    flush<BLK>(accg, 2 * ST_RHO + 1, tid, first, sr2);
+  }
+}
+
+// accumulate() for European vanilla calls and puts, the strips' common case,
+// with every job field it reads loaded once per batch and the payoff written
+// out. With sg = +1 (call) or -1 (put), sg (S m - K) = fma(sg S, m, -sg K)
+// exactly (negation is exact and rounding symmetric), so each payoff is one
+// FMA (or subtraction) and a max, the same number payoff() returns.
+template <typename R, int BLK, int UNR, bool REL>
+__device__ __forceinline__ void accumulate_vanilla(const Job<R> *__restrict__ J, const R *__restrict__ stage,
+                                                   const int *__restrict__ valid, int tid,
+                                                   double *__restrict__ accg, bool first) {
+  int zero = 0;  // as in accumulate: keeps the strike and slot addresses in the batch
+  asm volatile("" : "+s"(zero));
+  tid += zero;
+  const int nkp = J->nkp;
+  const int j = tid & (nkp - 1);
+  const int s0 = (tid / nkp) * nkp, s1 = valid_end(valid, tid, s0, nkp);
+  const int mode = J->mode;
+  const bool ess = mode != 0, full = mode == 2;
+  const bool live = j < J->nk;
+  const double sg = J->call ? 1.0 : -1.0;
+  const double Ks = live ? sg * J->strikes[j] : 0.0;
+  const int n = live ? s1 - s0 : 0;
+  const double mu = J->mult_up, md = J->mult_dn;
+  auto spot = [&](int v, int s) { return sg * static_cast<double>(stage[(F_SPOT * MAXV + v) * BLK + s]); };
+  auto pay = [&](double Ss) {
+    const double d = Ss - Ks;
+    return d > 0.0 ? d : 0.0;
+  };
+  auto pay_m = [&](double Ss, double m) {
+    const double d = fma(Ss, m, -Ks);
+    return d > 0.0 ? d : 0.0;
+  };
+  if (!ess) {
+    double sp = 0, sp2 = 0;
+#pragma unroll UNR
+    for (int i = 0; i < n; ++i) {
+      const double p = pay(spot(V_BASE, s0 + i));
+      sp += p;
+      sp2 += p * p;
+    }
+    flush<BLK>(accg, 2 * ST_PRICE, tid, first, sp);
+    flush<BLK>(accg, 2 * ST_PRICE + 1, tid, first, sp2);
+    return;
+  }
+  {
+    const double inv2h = J->inv2h, invh2 = J->invh2;
+    double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg2 = 0, sgm = 0;
+#pragma unroll UNR
+    for (int i = 0; i < n; ++i) {
+      const double Ss = spot(V_BASE, s0 + i);
+      const double p = pay(Ss), pu = pay_m(Ss, mu), pd = pay_m(Ss, md);
+      sp += p;
+      sp2 += p * p;
+      const double d = (pu - pd) * inv2h;
+      const double gm = (pu - p * 2.0 + pd) * invh2;
+      sd += d;
+      sd2 += d * d;
+      sgm += gm;
+      sg2 += gm * gm;
+    }
+    flush<BLK>(accg, 2 * ST_PRICE, tid, first, sp);
+    flush<BLK>(accg, 2 * ST_PRICE + 1, tid, first, sp2);
+    flush<BLK>(accg, 2 * ST_DELTA, tid, first, sd);
+    flush<BLK>(accg, 2 * ST_DELTA + 1, tid, first, sd2);
+    flush<BLK>(accg, 2 * ST_GAMMA, tid, first, sgm);
+    flush<BLK>(accg, 2 * ST_GAMMA + 1, tid, first, sg2);
+  }
+  if (!full) return;
+#pragma unroll 1
+  for (int f = 0; f < J->nvega; ++f) {
+    const double cvg = J->inv2dv, cvn = J->inv2dv * J->inv2h, cf = J->cv[f];
+    double sv = 0, sv2 = 0, sn = 0, sn2 = 0;
+#pragma unroll UNR
+    for (int i = 0; i < n; ++i) {
+      const double A = spot(V_V0UP + 2 * f, s0 + i), B = spot(V_V0DN + 2 * f, s0 + i);
+      const double ap = pay(A), au = pay_m(A, mu), ad = pay_m(A, md);
+      const double bp = pay(B), bu = pay_m(B, mu), bd = pay_m(B, md);
+      const double vg = (ap - bp) * cvg * cf;
+      const double vn = ((au - ad) - (bu - bd)) * cvn * cf;
+      sv += vg;
+      sv2 += vg * vg;
+      sn += vn;
+      sn2 += vn * vn;
+    }
+    flush<BLK>(accg, 2 * (ST_VEGA0 + f), tid, first, sv);
+    flush<BLK>(accg, 2 * (ST_VEGA0 + f) + 1, tid, first, sv2);
+    flush<BLK>(accg, 2 * (ST_VANNA0 + f), tid, first, sn);
+    flush<BLK>(accg, 2 * (ST_VANNA0 + f) + 1, tid, first, sn2);
+  }
+  if (J->nvega < 2) {
+    flush<BLK>(accg, 2 * ST_VEGA1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VEGA1 + 1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VANNA1, tid, first, 0.0);
+    flush<BLK>(accg, 2 * ST_VANNA1 + 1, tid, first, 0.0);
+  }
+  {
+    const double dfTu = J->dfTu, dfTd = J->dfTd, inv2dT = J->inv2dT, cch = J->inv2dT * J->inv2h;
+    double st = 0, st2 = 0, sc = 0, sc2 = 0;
+#pragma unroll UNR
+    for (int i = 0; i < n; ++i) {
+      const int s = s0 + i;
+      double A, B;
+      if constexpr (REL) {
+        const double Su = static_cast<double>(stage[(F_SPOT * MAXV + V_TUP) * BLK + s]) +
+                          static_cast<double>(stage[(F_SPOT * MAXV + V_RUP) * BLK + s]);
+        A = sg * Su;
+        B = sg * (Su + Su * static_cast<double>(stage[(F_SPOT * MAXV + V_TDN) * BLK + s]));
+      } else {
+        A = spot(V_TUP, s);
+        B = spot(V_TDN, s);
+      }
+      const double ap = pay(A), au = pay_m(A, mu), ad = pay_m(A, md);
+      const double bp = pay(B), bu = pay_m(B, mu), bd = pay_m(B, md);
+      const double th = (ap * dfTu - bp * dfTd) * inv2dT;
+      const double ch = ((au - ad) * dfTu - (bu - bd) * dfTd) * cch;
+      st += th;
+      st2 += th * th;
+      sc += ch;
+      sc2 += ch * ch;
+    }
+    flush<BLK>(accg, 2 * ST_THETA, tid, first, st);
+    flush<BLK>(accg, 2 * ST_THETA + 1, tid, first, st2);
+    flush<BLK>(accg, 2 * ST_CHARM, tid, first, sc);
+    flush<BLK>(accg, 2 * ST_CHARM + 1, tid, first, sc2);
+  }
+  {
+    const double dfru = J->dfru, dfrd = J->dfrd, inv2dr = J->inv2dr;
+    const double eu = J->erb_up, ed = J->erb_dn;
+    const bool derive = J->derive_r != 0;
+    double sr = 0, sr2 = 0;
+#pragma unroll UNR
+    for (int i = 0; i < n; ++i) {
+      const int s = s0 + i;
+      double ap, bp;
+      if (derive) {
+        // the rate-bumped paths are the base path shifted by +-r_bump*T in
+        // log space (only C0 of the log step depends on r)
+        const double Ss = spot(V_BASE, s);
+        ap = pay_m(Ss, eu);
+        bp = pay_m(Ss, ed);
+      } else {
+        ap = pay(spot(V_RUP, s));
+        bp = pay(spot(V_RDN, s));
+      }
+      const double rh = (ap * dfru - bp * dfrd) * inv2dr;
+      sr += rh;
+      sr2 += rh * rh;
     }
     flush<BLK>(accg, 2 * ST_RHO, tid, first, sr);
     flush<BLK>(accg, 2 * ST_RHO + 1, tid, first, sr2);
@@ -1186,8 +1350,15 @@ __device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, c
       }
       s_valid[tid] = valid ? 1 : 0;
       __syncthreads();
-      accumulate<R, BLK, UNR, t_slots_fp64<R, NV, (NF > 0 ? NF : 1), PD>()>(J, stage, s_valid, tid, accg,
-                                                                          it == 0);
+      constexpr bool REL = t_slots_fp64<R, NV, (NF > 0 ? NF : 1), PD>();
+      bool done = false;
+      if constexpr (!PD) {
+        if (J->payoff == 0) {
+          accumulate_vanilla<R, BLK, UNR, REL>(J, stage, s_valid, tid, accg, it == 0);
+          done = true;
+        }
+      }
+      if (!done) accumulate<R, BLK, UNR, REL, PD>(J, stage, s_valid, tid, accg, it == 0);
       if (J->view)
         accumulate<R, BLK, UNR>(jobs + (J->view - 1), stage, s_valid, tid,
                                 J->view_accg + static_cast<size_t>(u) * (2 * NSTAT * BLK), it == 0);
