@@ -11,6 +11,20 @@
 
 namespace myql {
 
+// A pointer into device (global) memory read out of a parameter block is
+// generic to the compiler, which then loads through it with flat
+// instructions. Those also count against the scalar loads' wait counter, so
+// every later wait for a job field would wait for a table gather in flight
+// too; reads through gmem(p) are global loads, which do not.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+  return (const __attribute__((address_space(1))) T *)(p);
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem_w(T *p) {
+  return (__attribute__((address_space(1))) T *)(p);
+}
+
 struct Rng {
   uint64_t s0, s1, s2, s3;
 };

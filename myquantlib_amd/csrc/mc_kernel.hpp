@@ -504,12 +504,17 @@ __device__ __forceinline__ void poisson_counts_lockstep(const Lam &lams, double 
 template <typename T, typename R>
 __device__ __forceinline__ T nci_tail_rows(const Job<R> *__restrict__ J, const T *__restrict__ tail, int N,
                                            double u2) {
-  double tp = (-log1p(-u2) - J->tail_w0) * J->tail_inv_hw;
+  // u2 passes through an opaque copy: the tail is read for u2 >= 0.999 only,
+  // and without it the compiler forms this FP64 logarithm (~140 VALU
+  // instructions) on every step, ahead of the branches that read it
+  double w = u2;
+  asm volatile("" : "+v"(w));
+  double tp = (-log1p(-w) - J->tail_w0) * J->tail_inv_hw;
   if (tp < 0.0) tp = 0.0;
   int j = static_cast<int>(tp);
   if (j >= J->tail_pts - 1) j = J->tail_pts - 2;
   const T tt = static_cast<T>(tp - static_cast<double>(j)), tt2 = tt * tt, tt3 = tt2 * tt;
-  const T *row = tail + 2 * (static_cast<size_t>(N) * J->tail_pts + j);
+  const auto row = gmem(tail + 2 * (static_cast<size_t>(N) * J->tail_pts + j));
   return (T(2) * tt3 - T(3) * tt2 + T(1)) * row[0] + (tt3 - T(2) * tt2 + tt) * row[1] +
          (T(-2) * tt3 + T(3) * tt2) * row[2] + (tt3 - tt2) * row[3];
 }
@@ -522,7 +527,7 @@ __device__ __forceinline__ R nci_tail_value(const Job<R> *__restrict__ J, int f,
 // Row N of the FP64 main table at grid interval i, fraction t
 __device__ __forceinline__ double nci_row64(const double *__restrict__ tab, int grid, int N, int i, double t) {
   const double t2 = t * t, t3 = t2 * t;
-  const double *row = tab + 2 * (static_cast<size_t>(N) * grid + i);
+  const auto row = gmem(tab + 2 * (static_cast<size_t>(N) * grid + i));
   return (2.0 * t3 - 3.0 * t2 + 1.0) * row[0] + (t3 - 2.0 * t2 + t) * row[1] + (-2.0 * t3 + 3.0 * t2) * row[2] +
          (t3 - t2) * row[3];
 }
@@ -620,7 +625,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
         y0d = nci_row64(J->nci64[f], J->nci_grid, Ns[0], i, td);
         y0 = static_cast<R>(y0d);
       } else {
-        const R *row = tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i);
+        const auto row = gmem(tab + 2 * (static_cast<size_t>(Ns[0]) * J->nci_grid + i));
         y0 = h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3];
       }
     }
@@ -638,7 +643,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       if (tl0) yt0 = nci_tail_value<R>(J, f, Ns[0], u2);
       if (ex0) {
         const double dof = P[0]->Kd[f][1] + 2.0 * Ns[0];
-        const double lg = (Ns[0] < J->nci_lg_n) ? J->nci_lgam[f][Ns[0]] : lgamma(0.5 * dof);
+        const double lg = (Ns[0] < J->nci_lg_n) ? gmem(J->nci_lgam[f])[Ns[0]] : lgamma(0.5 * dof);
         q0 = chi2_quantile_lg(dof, u2, lg);
       }
     }
@@ -646,7 +651,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
     auto exact_q = [&](int k, int N) {
       if (ex0 && N == Ns[0] && P[k]->Kd[f][1] == P[0]->Kd[f][1]) return q0;
       const double dof = P[k]->Kd[f][1] + 2.0 * N;
-      const double lg = (N < J->nci_lg_n) ? J->nci_lgam[f][N] : lgamma(0.5 * dof);
+      const double lg = (N < J->nci_lg_n) ? gmem(J->nci_lgam[f])[N] : lgamma(0.5 * dof);
       return chi2_quantile_lg(dof, u2, lg);
     };
     // The maturity-bumped slots of FP32 kernels read the FP64 rows, so their
@@ -671,7 +676,7 @@ __device__ __forceinline__ void evolve_factor(const Job<R> *__restrict__ J,
       if (main0 && N == Ns[0] && tab_k) {
         put(k, y0);
       } else if (tab_k && u2 < 0.999) {
-        const R *row = tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i);
+        const auto row = gmem(tab + 2 * (static_cast<size_t>(N) * J->nci_grid + i));
         put(k, h00 * row[0] + h10 * row[1] + h01 * row[2] + h11 * row[3]);
       } else if (tab_k && u2 >= 0.999 && J->nci_tail[f] != nullptr) {
         put(k, (tl0 && N == Ns[0]) ? yt0 : nci_tail_value<R>(J, f, N, u2));
@@ -912,7 +917,7 @@ __device__ __forceinline__ void eval3(const Job<R> *__restrict__ J, const R *__r
 // (accg[a*BLK + tid]), so the path simulation holds no accumulators.
 template <int BLK>
 __device__ __forceinline__ void flush(double *__restrict__ accg, int a, int tid, bool first, double x) {
-  double *slot = accg + a * BLK + tid;
+  const auto slot = gmem_w(accg + a * BLK + tid);
   *slot = first ? x : *slot + x;
 }
 
@@ -958,7 +963,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   const int mode = J->mode;
   const bool ess = mode != 0, full = mode == 2;
   const bool live = j < J->nk;
-  const double K = live ? J->strikes[j] : 0.0;
+  const double K = live ? gmem(J->strikes)[j] : 0.0;
   const int s0 = grp * nkp, s1 = valid_end(valid, tid, s0, nkp);
   {
     double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg = 0, sg2 = 0;
@@ -1091,7 +1096,7 @@ __device__ __forceinline__ void accumulate_vanilla(const Job<R> *__restrict__ J,
   const bool ess = mode != 0, full = mode == 2;
   const bool live = j < J->nk;
   const double sg = J->call ? 1.0 : -1.0;
-  const double Ks = live ? sg * J->strikes[j] : 0.0;
+  const double Ks = live ? sg * gmem(J->strikes)[j] : 0.0;
   const int n = live ? s1 - s0 : 0;
   const double mu = J->mult_up, md = J->mult_dn;
   auto spot = [&](int v, int s) { return sg * static_cast<double>(stage[(F_SPOT * MAXV + v) * BLK + s]); };
@@ -1236,14 +1241,14 @@ __device__ __forceinline__ void unit_partials(const Job<R> *__restrict__ J, cons
   const int na = mode == 2 ? 2 * NSTAT : (mode == 1 ? 2 * (ST_GAMMA + 1) : 2);
   double *out = J->partial + static_cast<size_t>(J->first_slot + blk) * (2 * NSTAT) * J->nk;
   for (int a = 0; a < 2 * NSTAT; ++a) {
-    double x = (a < na && any) ? accg[a * BLK + tid] : 0.0;
+    double x = (a < na && any) ? gmem(accg)[a * BLK + tid] : 0.0;
     for (int off = nkp; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
     s_red[tid] = x;
     __syncthreads();
     if (wave == 0 && lane < J->nk) {
       double t = 0.0;
       for (int w = 0; w < BLK / 64; ++w) t += s_red[w * 64 + lane];
-      out[a * J->nk + lane] = t;
+      gmem_w(out)[a * J->nk + lane] = t;
     }
     __syncthreads();
   }
@@ -1294,10 +1299,10 @@ __device__ __forceinline__ void mc_paths_body(const Job<R> *__restrict__ jobs, c
     uint32_t *qhi = s_qhi + (QMC ? (tid >> 6) * QMC_DMAX : 0);
     if constexpr (!QMC) {
       if (active) {
-        g.s0 = J->streams[4 * s + 0];
-        g.s1 = J->streams[4 * s + 1];
-        g.s2 = J->streams[4 * s + 2];
-        g.s3 = J->streams[4 * s + 3];
+        g.s0 = gmem(J->streams)[4 * s + 0];
+        g.s1 = gmem(J->streams)[4 * s + 1];
+        g.s2 = gmem(J->streams)[4 * s + 2];
+        g.s3 = gmem(J->streams)[4 * s + 3];
       } else {
         g.s0 = g.s1 = g.s2 = g.s3 = 0;
       }
