@@ -473,6 +473,58 @@ __device__ __forceinline__ int giles_asym32(double lam, float W, float &S, float
   return -1;
 }
 
+// Giles' tail branch (lam > 4, |W| >= 3) in FP32: Newton's iteration for
+// the R with eta(R) = W / sqrt(lam), eta(R) = sign(R - 1) sqrt(2 (1 - R +
+// R log R)), then S and Del as the FP64 method forms them. Returns floor(S)
+// when S clears 10, the integers and the correction band [X, X + 2 Del) by
+// tol, else -1 with S set for the caller's S < 10 - tol test (the bottom-up
+// sum). tol = 1e-3 + 1e-5 lam is 12x the largest |S32 - S64| found with W
+// from the FP32 normal quantile and every FP32 logarithm, square root and
+// reciprocal perturbed by up to 2 ulp, over lam in (4, 3200] and |W| in
+// [3, 9] where S64 >= 9 (tests/test_poisson_tail32.py). Two cases need no
+// iteration, both S64 < 10: eta > -sqrt(2) for every R > 0, so below
+// W / sqrt(lam) = -1.3 the method's iteration has no root (or one below
+// R = 0.03) and its S stays below 2; and for W < 0 (R < 1) its S is below
+// lam R + 0.4875 + 0.033 (the log-ratio term lies in [1/3, 0.4875] there),
+// under 10 for lam <= 9.4. An iteration that has not converged to 1e-6
+// (relative) after 12 steps is left to the FP64 method.
+__device__ __forceinline__ int giles_tail32(double lam, float W, float &S, float &tol) {
+  const float lf = static_cast<float>(lam);
+  constexpr float kLn2 = 0.69314718055994531f;
+  tol = 1e-3f + 1e-5f * lf;
+  const float s = W * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(lf));
+  S = 0.0f;
+  if (s < -1.3f || (W < 0.0f && lf <= 9.4f)) return -1;
+  float R = fmaxf(1.0f + s, 0.1f);
+  bool conv = false;
+  for (int it = 0; it < 12; ++it) {
+    const float Tl = kLn2 * __builtin_amdgcn_logf(R);
+    const float Rp = R;
+    float S2 = __builtin_amdgcn_sqrtf(2.0f * ((1.0f - R) + R * Tl));
+    if (R < 1.0f) S2 = -S2;
+    R = Rp - (S2 - s) * S2 * __builtin_amdgcn_rcpf(Tl);
+    if (R < 0.1f * Rp) R = 0.1f * Rp;
+    if (fabsf(R - Rp) <= 1e-6f * R) {
+      conv = true;
+      break;
+    }
+  }
+  if (!conv) {
+    S = 10.0f;
+    return -1;
+  }
+  const float Tl = kLn2 * __builtin_amdgcn_logf(R);
+  const float a = __builtin_amdgcn_sqrtf(2.0f * R * ((1.0f - R) + R * Tl));
+  S = lf * R + kLn2 * __builtin_amdgcn_logf(a * __builtin_amdgcn_rcpf(fabsf(R - 1.0f))) * __builtin_amdgcn_rcpf(Tl);
+  S = S - 0.0218f * __builtin_amdgcn_rcpf(S + 0.065f * lf);
+  const float Del = 0.01f * __builtin_amdgcn_rcpf(S);
+  S = S + Del;
+  const float X = floorf(S);
+  const float fr = S - X;
+  if (S >= 10.0f + tol && fr >= 2.0f * Del + tol && fr <= 1.0f - tol) return static_cast<int>(X);
+  return -1;
+}
+
 // The bottom-up sum's count in FP32, or -1 when U lies within 3e-5 of F(n)
 // or F(n-1) (or no count below 48 reaches U). The loop steps all active
 // lanes together, so j is wave-uniform and 1/j a scalar load.
@@ -506,7 +558,10 @@ __device__ __forceinline__ int pois_sum32(double lam, double U, PoisMargin *mg) 
 
 // The count Giles' method returns for (lam, U), 0 <= U < 1, from the FP32
 // first pass, or -1 (undecided). W = giles_w32(U), read when lam > 4. mg
-// receives the margins bumped_count reads.
+// receives the margins bumped_count reads. TAIL: also the tail branch
+// (giles_tail32); without it |W| >= 3 is left undecided, so that the lockstep
+// slots of such a sample go to the wave-packed pass together.
+template <bool TAIL = false>
 __device__ __forceinline__ int poisson_first32(double lam, double U, float W, PoisMargin *mg) {
   if (mg) {
     mg->valid = false;
@@ -515,8 +570,18 @@ __device__ __forceinline__ int poisson_first32(double lam, double U, float W, Po
   }
   if (U <= 0.0) return 0;
   if (lam > 4.0) {
-    if (!(fabsf(W) < 2.999f)) return -1;
     float S, Del, tol;
+    if constexpr (TAIL) {
+      // the method's branch is decided on FP64 W (|W| < 3): W32 is within
+      // 1e-5 of it
+      if (fabsf(W) > 3.001f) {
+        const int N = giles_tail32(lam, W, S, tol);
+        if (N >= 0) return N;
+        if (!(S < 10.0f - tol)) return -1;
+        return pois_sum32(lam, U, mg);
+      }
+    }
+    if (!(fabsf(W) < 2.999f)) return -1;
     const int N = giles_asym32(lam, W, S, Del, tol);
     if (N >= 0) {
       if (mg) {

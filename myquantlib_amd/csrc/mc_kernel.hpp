@@ -361,7 +361,14 @@ __device__ __forceinline__ void giles_rare(const Lam &lams, double U, unsigned r
       }
     }
     wave_sync();
-    if (rank < total - base) s_n[wave][rank] = static_cast<int>(poissinv(s_lam[wave][rank], s_u[wave][rank]));
+    if (rank < total - base) {
+      // the FP32 first pass with its tail branch on the item's own
+      // intensity, then the FP64 method where that is undecided
+      const double lam = s_lam[wave][rank], u = s_u[wave][rank];
+      int n = poisson_first32<true>(lam, u, lam > 4.0 ? giles_w32(u) : 0.0f, nullptr);
+      if (n < 0) n = static_cast<int>(poissinv(lam, u));
+      s_n[wave][rank] = n;
+    }
     wave_sync();
 #pragma unroll
     for (int k = 0; k < NV; ++k) {

@@ -116,6 +116,25 @@ def test_poisson_first_pass_counts_equal_giles(engine, oracle):
         uu = _near_cdf_uniforms(l0, rng, 40)
         lams.append(np.full(uu.size, l0) * np.exp(rng.uniform(-0.05, 0.05, uu.size)))
         us.append(uu)
+    # Giles' tail branch (|W| >= 3, run in FP32 by the wave-packed pass):
+    # uniforms down to 1e-19 and up to 1 - 1e-15, and placed near the CDF
+    # values 3-8 standard deviations out
+    for l0 in (4.5, 9.0, 9.8, 15.0, 40.0, 150.0, 900.0, 2500.0):
+        ll = np.full(400, l0) * np.exp(rng.uniform(-0.05, 0.05, 400))
+        uu = np.concatenate([10 ** rng.uniform(-19, -2.9, 200), 1.0 - 10 ** rng.uniform(-15, -2.9, 200)])
+        lams.append(ll)
+        us.append(uu)
+        sd = np.sqrt(l0)
+        hi = np.arange(int(l0 + 3 * sd), int(l0 + 8 * sd) + 2)
+        lo = np.arange(max(0, int(l0 - 8 * sd)), max(1, int(l0 - 3 * sd)))
+        d = 10.0 ** rng.uniform(-7, -4, (hi.size + lo.size, 6)) * np.where(
+            rng.uniform(size=(hi.size + lo.size, 6)) < 0.5, -1, 1)
+        # relative offsets from the tail probability on either side
+        uu = np.concatenate([(1.0 - stats.poisson.sf(hi, l0)[:, None] * (1.0 + d[:hi.size])).ravel(),
+                             (stats.poisson.cdf(lo, l0)[:, None] * (1.0 + d[hi.size:])).ravel()])
+        uu = uu[(uu > 0) & (uu < 1)]
+        lams.append(np.full(uu.size, l0))
+        us.append(uu)
     lam, u = np.concatenate(lams), np.concatenate(us)
     ref = np.array([oracle.poissinvcdf(a, b) for a, b in zip(lam, u)]).astype(np.int32)
     got = engine.poisson_counts(lam, u)[:, 0]
