@@ -481,7 +481,7 @@ __device__ __forceinline__ int giles_asym32(double lam, float W, float &S, float
 // sum). tol = 1e-3 + 1e-5 lam is 12x the largest |S32 - S64| found with W
 // from the FP32 normal quantile and every FP32 logarithm, square root and
 // reciprocal perturbed by up to 2 ulp, over lam in (4, 3200] and |W| in
-// [3, 9] where S64 >= 9 (tests/test_poisson_tail32.py). Two cases need no
+// [3, 9] where S64 >= 9 (tests/test_poisson_fp32_pass.py). Two cases need no
 // iteration, both S64 < 10: eta > -sqrt(2) for every R > 0, so below
 // W / sqrt(lam) = -1.3 the method's iteration has no root (or one below
 // R = 0.03) and its S stays below 2; and for W < 0 (R < 1) its S is below
@@ -525,6 +525,74 @@ __device__ __forceinline__ int giles_tail32(double lam, float W, float &S, float
   return -1;
 }
 
+// Giles' correction near an integer boundary (S > 10 in [X, X + 2 Del),
+// 0.5 lam < X < 2 lam) in FP32: Temme's expansion Sm of the CDF at X - 1 as
+// the FP64 method forms it (the kTemmeB1 / kTemmeB0 chains), compared with
+// U (X < lam) or -V (X >= lam) to choose X - 1 or X. The FP32 form takes
+// r - 1 = (X - lam) / lam from FP64 and 1 - r + r log r from its series in
+// r - 1 below |r - 1| = 1/4, so eta carries no cancellation. Returns the
+// count where Sm clears U (or -V) by 4e-6, 5.9x the largest |Sm32 - Sm64|
+// found with every FP32 transcendental perturbed by up to 2 ulp (erfc by
+// 4) over lam in (5.5, 3200] (tests/test_poisson_fp32_pass.py); else -1.
+__device__ __forceinline__ int giles_temme32(double lam, double U, int X) {
+  constexpr float kLn2 = 0.69314718055994531f;
+  const double Xd = static_cast<double>(X);
+  const float d = static_cast<float>((Xd - lam) * (1.0 / lam));
+  const float r = 1.0f + d;
+  float g;
+  if (fabsf(d) < 0.25f) {
+    // sum over k >= 2 of (-1)^k d^k / (k (k - 1)), to k = 14
+    float p = 1.0f / 182.0f;
+    p = p * d - 1.0f / 156.0f;
+    p = p * d + 1.0f / 132.0f;
+    p = p * d - 1.0f / 110.0f;
+    p = p * d + 1.0f / 90.0f;
+    p = p * d - 1.0f / 72.0f;
+    p = p * d + 1.0f / 56.0f;
+    p = p * d - 1.0f / 42.0f;
+    p = p * d + 1.0f / 30.0f;
+    p = p * d - 1.0f / 20.0f;
+    p = p * d + 1.0f / 12.0f;
+    p = p * d - 1.0f / 6.0f;
+    p = p * d + 1.0f / 2.0f;
+    g = d * d * p;
+  } else {
+    g = r * (kLn2 * __builtin_amdgcn_logf(r)) - d;
+  }
+  float eta = __builtin_amdgcn_sqrtf(2.0f * g * __builtin_amdgcn_rcpf(r));
+  if (Xd > lam) eta = -eta;
+  const float Xf = static_cast<float>(X);
+  const float xi = __builtin_amdgcn_rcpf(Xf);
+  float B1 = static_cast<float>(kTemmeB1[0]), Sm = B1, B0 = static_cast<float>(kTemmeB0[0]);
+  Sm = B0 + Sm * eta;
+  float mult = 25.0f;
+#pragma unroll
+  for (int j = 1; j < 13; ++j) {
+    B1 = static_cast<float>(kTemmeB1[j]) + mult * B1 * xi;
+    Sm = B1 + Sm * eta;
+    mult -= 1.0f;
+    B0 = static_cast<float>(kTemmeB0[j]) + mult * B0 * xi;
+    Sm = B0 + Sm * eta;
+    mult -= 1.0f;
+  }
+  Sm = Sm * __builtin_amdgcn_rcpf(1.0f + B1 * xi);
+  Sm = Sm * expf(-0.5f * Xf * eta * eta) *
+       __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(6.28318530717958648f * Xf));
+  const float z = eta * __builtin_amdgcn_sqrtf(0.5f * Xf);
+  constexpr double kTol = 4e-6;
+  if (Xd < lam) {
+    const double c = static_cast<double>(Sm + 0.5f * erfcf(z));
+    if (c > U + kTol) return X - 1;
+    if (c < U - kTol) return X;
+  } else {
+    const double c = static_cast<double>(Sm - 0.5f * erfcf(-z));
+    const double mV = U - 1.0;
+    if (c > mV + kTol) return X - 1;
+    if (c < mV - kTol) return X;
+  }
+  return -1;
+}
+
 // The bottom-up sum's count in FP32, or -1 when U lies within 3e-5 of F(n)
 // or F(n-1) (or no count below 48 reaches U). The loop steps all active
 // lanes together, so j is wave-uniform and 1/j a scalar load.
@@ -559,8 +627,9 @@ __device__ __forceinline__ int pois_sum32(double lam, double U, PoisMargin *mg) 
 // The count Giles' method returns for (lam, U), 0 <= U < 1, from the FP32
 // first pass, or -1 (undecided). W = giles_w32(U), read when lam > 4. mg
 // receives the margins bumped_count reads. TAIL: also the tail branch
-// (giles_tail32); without it |W| >= 3 is left undecided, so that the lockstep
-// slots of such a sample go to the wave-packed pass together.
+// (giles_tail32) and the Temme correction (giles_temme32); without them
+// those cases are left undecided, so that the lockstep slots of such a
+// sample go to the wave-packed pass together.
 template <bool TAIL = false>
 __device__ __forceinline__ int poisson_first32(double lam, double U, float W, PoisMargin *mg) {
   if (mg) {
@@ -591,6 +660,17 @@ __device__ __forceinline__ int poisson_first32(double lam, double U, float W, Po
         mg->s_tol = tol;
       }
       return N;
+    }
+    if constexpr (TAIL) {
+      // S64 > 10 inside the correction band, with X = floor(S64) and the
+      // band's edges cleared by tol: the Temme branch, which the FP64 method
+      // takes for 0.5 lam < X < 2 lam
+      const float X = floorf(S), fr = S - X;
+      if (S >= 11.0f + tol && fr >= tol && fr <= 2.0f * Del - tol) {
+        const double Xd = static_cast<double>(X);
+        if (Xd > 0.5 * lam && Xd < 2.0 * lam) return giles_temme32(lam, U, static_cast<int>(X));
+        return -1;
+      }
     }
     if (!(S < 10.0f - tol)) return -1;
   }

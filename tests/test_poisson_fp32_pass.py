@@ -1,4 +1,5 @@
-"""Error bound of the path kernels' FP32 pass over Giles' tail branch.
+"""Error bounds of the path kernels' FP32 passes over two branches of Giles'
+inverse Poisson CDF: the tail branch and the Temme correction.
 
 Giles' inverse Poisson CDF (reference math/Numerics.hpp:49-272) takes, for
 lam > 4 and |W| >= 3 (W = Phi^-1(U)), a Newton iteration for R with
@@ -18,8 +19,18 @@ and checks |S32 - S64| < tol / 8 wherever S64 >= 9, and that the two cases
 the device sends to the bottom-up sum without iterating (W / sqrt(lam) <
 -1.3; W < 0 with lam <= 9.4) have S64 < 10 - tol. The counts themselves are
 compared with the oracle's inversion on the GPU (test_gpu_parity.py).
+
+The Temme correction (S > 10 within [X, X + 2 Del) of an integer X, 0.5 lam
+< X < 2 lam) chooses X - 1 where Temme's expansion Sm of the CDF at X - 1
+exceeds U (for X < lam; Sm - 1 against U - 1 above). The device forms Sm in
+FP32 (giles_temme32) and decides only where it clears U by 4e-6; the second
+test checks |Sm32 - Sm64| < 4e-6 / 4 with the same perturbations (erfc by 4
+ulp).
 """
+import math
+
 import numpy as np
+from scipy.special import erfc
 
 C = [-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00, -2.549732539343734e+00,
      4.374664141464968e+00, 2.938163982698783e+00]
@@ -135,3 +146,74 @@ def test_tail_pass_error_below_an_eighth_of_tol():
     assert np.all(conv[big])
     err = np.abs(S32[big].astype(np.float64) - S64k[big])
     assert np.max(err / tolk[big]) < 0.125, np.max(err / tolk[big])
+
+
+B1 = [8.0995211567045583e-16, -5.1391118342426808e-16, -1.3923887224181616e-13, 1.1004392031956284e-13,
+      2.4361948020667402e-11, -2.5514193994946487e-11, -4.3820360184533521e-09, 6.7078535434015332e-09,
+      8.2967113409530833e-07, -2.1854485106799979e-06, -0.00017875514403292177, 0.0011574074074074078,
+      0.083333333333333329]
+B0 = [-1.9752288294349411e-15, 2.8534893807047458e-14, 3.3717632624009806e-13, -5.0276692801141763e-12,
+      -5.8307721325504166e-11, 9.1476995822367933e-10, 1.0261809784240299e-08, -1.7665952736826086e-07,
+      -1.8540622107151585e-06, 3.9192631785224383e-05, 0.00035273368606701921, -0.014814814814814815,
+      -0.33333333333333331]
+
+
+def _temme64(lam, X):
+    """The method's Sm (+ or - the erfc term) in FP64."""
+    ilam, Xi = 1.0 / lam, 1.0 / X
+    eta = X * ilam
+    eta = np.sqrt(2.0 * (1.0 - eta + eta * np.log(eta)) / eta)
+    eta = np.where(X > lam, -eta, eta)
+    b1, b0 = B1[0], B0[0]
+    sm = b0 + b1 * eta
+    mult = 25.0
+    for j in range(1, 13):
+        b1 = B1[j] + mult * b1 * Xi
+        sm = b1 + sm * eta
+        mult -= 1.0
+        b0 = B0[j] + mult * b0 * Xi
+        sm = b0 + sm * eta
+        mult -= 1.0
+    sm = sm / (1.0 + b1 * Xi) * np.exp(-0.5 * X * eta * eta) / np.sqrt(2.0 * math.pi * X)
+    return np.where(X < lam, sm + 0.5 * erfc(eta * np.sqrt(0.5 * X)), sm - 0.5 * erfc(-eta * np.sqrt(0.5 * X)))
+
+
+def _temme32(lam, X, rng):
+    d = ((X - lam) * (1.0 / lam)).astype(F32)
+    r = (F32(1.0) + d).astype(F32)
+    p = F32(1.0 / 182.0)
+    for k in range(13, 1, -1):
+        p = (p * d + F32((-1) ** k / (k * (k - 1)))).astype(F32)
+    g = np.where(np.abs(d) < F32(0.25), d * d * p, r * _noisy(np.log(r), rng) - d).astype(F32)
+    eta = _noisy(np.sqrt(F32(2.0) * g * _noisy(F32(1.0) / r, rng)), rng)
+    eta = np.where(X > lam, -eta, eta).astype(F32)
+    Xf = X.astype(F32)
+    xi = _noisy(F32(1.0) / Xf, rng)
+    b1, b0 = F32(B1[0]), F32(B0[0])
+    sm = (b0 + b1 * eta).astype(F32)
+    mult = F32(25.0)
+    for j in range(1, 13):
+        b1 = (F32(B1[j]) + mult * b1 * xi).astype(F32)
+        sm = (b1 + sm * eta).astype(F32)
+        mult = F32(mult - 1)
+        b0 = (F32(B0[j]) + mult * b0 * xi).astype(F32)
+        sm = (b0 + sm * eta).astype(F32)
+        mult = F32(mult - 1)
+    sm = (sm * _noisy(F32(1.0) / (F32(1.0) + b1 * xi), rng)).astype(F32)
+    sm = (sm * _noisy(np.exp(F32(-0.5) * Xf * eta * eta), rng) *
+          _noisy(F32(1.0) / _noisy(np.sqrt(F32(2.0 * math.pi) * Xf), rng), rng)).astype(F32)
+    z = (eta * _noisy(np.sqrt(F32(0.5) * Xf), rng)).astype(F32)
+    ep = _noisy(erfc(z.astype(np.float64)).astype(F32), rng, 4)
+    em = _noisy(erfc(-z.astype(np.float64)).astype(F32), rng, 4)
+    return np.where(X < lam, sm + F32(0.5) * ep, sm - F32(0.5) * em).astype(F32)
+
+
+def test_temme_pass_error_below_a_quarter_of_tol():
+    rng = np.random.default_rng(5)
+    lam = np.concatenate([rng.uniform(5.5, 30.0, 100000), 10 ** rng.uniform(1.4, 3.5, 100000)])
+    X = np.floor(lam + rng.uniform(-3.2, 3.2, lam.size) * np.sqrt(lam))
+    keep = (X >= 11) & (X > 0.5 * lam) & (X < 2.0 * lam)
+    lam, X = lam[keep], X[keep]
+    assert lam.size > 150000
+    err = np.abs(_temme32(lam, X, rng).astype(np.float64) - _temme64(lam, X))
+    assert np.max(err) < 1e-6, np.max(err)
