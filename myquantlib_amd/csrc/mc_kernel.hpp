@@ -333,6 +333,17 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 // lanes one item each, and read back by their owners. The wave sits through
 // the rare code once per 64 items instead of once per slot. Must be entered
 // by every active lane of the wave (a wave-uniform condition).
+//
+// poisson_fp32_item: one packed item's FP32 first pass on its own intensity,
+// with the tail branch and the Temme correction. The single-slot paths (the
+// multilevel kernel's steps among them) call it out of line: inlined into
+// that kernel's step loop, its code made the compiler turn every job-field
+// read of the loop into a vector load (694 -> 1301 ms for the Heston
+// up-and-out run, which never takes this path). The lockstep path inlines
+// it (5 ms of 371 on the Bates strip).
+__device__ static __noinline__ int poisson_fp32_item(double lam, double u) {
+  return poisson_first32<true>(lam, u, lam > 4.0 ? giles_w32(u) : 0.0f, nullptr);
+}
 template <int NV, typename Lam>
 __device__ __forceinline__ void giles_rare(const Lam &lams, double U, unsigned rare, int (&Ns)[NV]) {
   constexpr int W64 = 64;
@@ -365,7 +376,9 @@ __device__ __forceinline__ void giles_rare(const Lam &lams, double U, unsigned r
       // the FP32 first pass with its tail branch on the item's own
       // intensity, then the FP64 method where that is undecided
       const double lam = s_lam[wave][rank], u = s_u[wave][rank];
-      int n = poisson_first32<true>(lam, u, lam > 4.0 ? giles_w32(u) : 0.0f, nullptr);
+      int n;
+      if constexpr (NV > 1) n = poisson_first32<true>(lam, u, lam > 4.0 ? giles_w32(u) : 0.0f, nullptr);
+      else n = poisson_fp32_item(lam, u);
       if (n < 0) n = static_cast<int>(poissinv(lam, u));
       s_n[wave][rank] = n;
     }
@@ -971,12 +984,16 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
   const bool ess = mode != 0, full = mode == 2;
   const bool live = j < J->nk;
   const double K = live ? gmem(J->strikes)[j] : 0.0;
-  const int s0 = grp * nkp, s1 = valid_end(valid, tid, s0, nkp);
+  // samples are checked one by one here: counting them once per batch
+  // (valid_end) took the batched Double-Heston kernel from 127 to 134 VGPRs,
+  // one wave per SIMD fewer
+  const int s0 = grp * nkp, s1 = grp * nkp + nkp;
   {
     double sp = 0, sp2 = 0, sd = 0, sd2 = 0, sg = 0, sg2 = 0;
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
         double p, pu = 0, pd = 0;
         eval3<R, BLK, PD>(J, stage, V_BASE, s, K, ess, p, pu, pd);
         sp += p;
@@ -1007,6 +1024,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
         eval3<R, BLK, PD>(J, stage, V_V0UP + 2 * f, s, K, true, ap, au, ad);
         eval3<R, BLK, PD>(J, stage, V_V0DN + 2 * f, s, K, true, bp, bu, bd);
@@ -1034,6 +1052,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
         double ap, au, ad, bp, bu, bd;
         if constexpr (REL) {
           const double Su = static_cast<double>(stage[(F_SPOT * MAXV + V_TUP) * BLK + s]) +
@@ -1063,6 +1082,7 @@ __device__ __forceinline__ void accumulate(const Job<R> *__restrict__ J, const R
     if (live) {
 #pragma unroll UNR
       for (int s = s0; s < s1; ++s) {
+        if (!valid[s]) continue;
         double ap, bp, dum0, dum1;
         if (J->derive_r) {
           // European: the rate-bumped paths are the base path shifted by
