@@ -531,9 +531,10 @@ __device__ __forceinline__ int giles_tail32(double lam, float W, float &S, float
 // U (X < lam) or -V (X >= lam) to choose X - 1 or X. The FP32 form takes
 // r - 1 = (X - lam) / lam from FP64 and 1 - r + r log r from its series in
 // r - 1 below |r - 1| = 1/4, so eta carries no cancellation. Returns the
-// count where Sm clears U (or -V) by 4e-6, 5.9x the largest |Sm32 - Sm64|
+// count where Sm clears U (or -V) by 4e-6, 6x the largest |Sm32 - Sm64|
 // found with every FP32 transcendental perturbed by up to 2 ulp (erfc by
-// 4) over lam in (5.5, 3200] (tests/test_poisson_fp32_pass.py); else -1.
+// 4) over X >= 10, lam in (5, 3200] (tests/test_poisson_fp32_pass.py); else
+// -1.
 __device__ __forceinline__ int giles_temme32(double lam, double U, int X) {
   constexpr float kLn2 = 0.69314718055994531f;
   const double Xd = static_cast<double>(X);
@@ -664,12 +665,15 @@ __device__ __forceinline__ int poisson_first32(double lam, double U, float W, Po
     if constexpr (TAIL) {
       // S64 > 10 inside the correction band, with X = floor(S64) and the
       // band's edges cleared by tol: the Temme branch, which the FP64 method
-      // takes for 0.5 lam < X < 2 lam
+      // takes for 0.5 lam < X < 2 lam. At X = 10 a step down to 9 sends the
+      // method to its bottom-up sum, whose count pois_sum32 finds.
       const float X = floorf(S), fr = S - X;
-      if (S >= 11.0f + tol && fr >= tol && fr <= 2.0f * Del - tol) {
+      if (S >= 10.0f + tol && fr >= tol && fr <= 2.0f * Del - tol) {
         const double Xd = static_cast<double>(X);
-        if (Xd > 0.5 * lam && Xd < 2.0 * lam) return giles_temme32(lam, U, static_cast<int>(X));
-        return -1;
+        if (!(Xd > 0.5 * lam && Xd < 2.0 * lam)) return -1;
+        const int n = giles_temme32(lam, U, static_cast<int>(X));
+        if (n == 9) return pois_sum32(lam, U, nullptr);
+        return n;
       }
     }
     if (!(S < 10.0f - tol)) return -1;

@@ -20,8 +20,8 @@ the device sends to the bottom-up sum without iterating (W / sqrt(lam) <
 -1.3; W < 0 with lam <= 9.4) have S64 < 10 - tol. The counts themselves are
 compared with the oracle's inversion on the GPU (test_gpu_parity.py).
 
-The Temme correction (S > 10 within [X, X + 2 Del) of an integer X, 0.5 lam
-< X < 2 lam) chooses X - 1 where Temme's expansion Sm of the CDF at X - 1
+The Temme correction (S > 10 within [X, X + 2 Del) of an integer X >= 10,
+0.5 lam < X < 2 lam) chooses X - 1 where Temme's expansion Sm of the CDF at X - 1
 exceeds U (for X < lam; Sm - 1 against U - 1 above). The device forms Sm in
 FP32 (giles_temme32) and decides only where it clears U by 4e-6; the second
 test checks |Sm32 - Sm64| < 4e-6 / 4 with the same perturbations (erfc by 4
@@ -210,9 +210,9 @@ def _temme32(lam, X, rng):
 
 def test_temme_pass_error_below_a_quarter_of_tol():
     rng = np.random.default_rng(5)
-    lam = np.concatenate([rng.uniform(5.5, 30.0, 100000), 10 ** rng.uniform(1.4, 3.5, 100000)])
+    lam = np.concatenate([rng.uniform(5.0, 30.0, 100000), 10 ** rng.uniform(1.4, 3.5, 100000)])
     X = np.floor(lam + rng.uniform(-3.2, 3.2, lam.size) * np.sqrt(lam))
-    keep = (X >= 11) & (X > 0.5 * lam) & (X < 2.0 * lam)
+    keep = (X >= 10) & (X > 0.5 * lam) & (X < 2.0 * lam)
     lam, X = lam[keep], X[keep]
     assert lam.size > 150000
     err = np.abs(_temme32(lam, X, rng).astype(np.float64) - _temme64(lam, X))
