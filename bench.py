@@ -75,7 +75,7 @@ EXECUTED_VARIANTS = 5
 # HBM bytes per path-kernel launch (FETCH_SIZE + WRITE_SIZE from rocprofv3,
 # scaled by the same counters on a plain copy of known size), by
 # --paths-log2 of the measured run; None where not measured
-TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 6.05e11}
+TRAFFIC_BYTES_PER_LAUNCH: dict = {24: 4.14e11}
 
 
 def model():
