@@ -486,6 +486,18 @@ int myql_debug_cv_input_grad(myql_ctx *ctx, const float *params, const float *fe
                              int64_t n, float *out);
 int myql_debug_cv_param_grad(myql_ctx *ctx, const float *params, const float *feats,
                              const float *targets, int64_t n, float *grad, double *loss);
+/* The control variate's terminal-jump expectation E[P_k | S_T- = S]
+ * (Merton or Kou jumps over the instrument's maturity, strikes k of the
+ * instrument; vanilla and, in Greek mode NONE, sharp digital payoffs), at
+ * n given spots: tab[i * n_strikes + k] as the estimator reads it from its
+ * table on the log-spot grid [log S0 - 3, log S0 + 2], dir the same summed
+ * directly, ddir the derivative in log S of its n >= 1 part.
+ * *table_on = 0 (outputs untouched) where the estimator goes without the
+ * term (smoothed digitals, jump counts the table does not cover). No
+ * counterpart in the reference. */
+int myql_debug_cv_jump_expectation(myql_ctx *ctx, const myql_model *model, const myql_instrument *instrument,
+                                   int32_t greek_mode, double S0, const double *spots, int64_t n, double *tab,
+                                   double *dir, double *ddir, int32_t *table_on);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,7 @@ EXPORTED = [
     "myql_debug_poissinv", "myql_debug_poisson_counts", "myql_debug_qmc_points", "myql_debug_chi2_quantile", "myql_debug_path_payoffs", "myql_debug_set_launch_shape",
     "myql_debug_cv_input_grad",
     "myql_debug_cv_param_grad",
+    "myql_debug_cv_jump_expectation",
 ]
 
 
@@ -209,4 +210,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.myql_debug_cv_input_grad.argtypes = [vp, pf, pf, C.c_int64, pf]
     lib.myql_debug_cv_param_grad.restype = C.c_int
     lib.myql_debug_cv_param_grad.argtypes = [vp, pf, pf, pf, C.c_int64, pf, _PD]
+    lib.myql_debug_cv_jump_expectation.restype = C.c_int
+    lib.myql_debug_cv_jump_expectation.argtypes = [vp, vp, vp, C.c_int32, C.c_double, _PD, C.c_int64, _PD, _PD, _PD,
+                                                   C.POINTER(C.c_int32)]
     return lib

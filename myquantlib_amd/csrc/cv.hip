@@ -36,6 +36,14 @@ hipError_t launch_cv_rb_table(const CvSet &set, double *tab, hipStream_t stream)
   return hipGetLastError();
 }
 
+hipError_t launch_cv_debug_rb(const CvSet &set, const double *spots, int64_t n, double *tab, double *dir,
+                              double *ddir, hipStream_t stream) {
+  const int64_t m = n * set.nk;
+  hipLaunchKernelGGL(cv_debug_rb_kernel, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0, stream, set, spots,
+                     n, tab, dir, ddir);
+  return hipGetLastError();
+}
+
 hipError_t launch_cv_moments(const CvSet &set, double *part, int blocks, hipStream_t stream) {
   hipLaunchKernelGGL(cv_moments_kernel, dim3(blocks, set.nk), dim3(256), 0, stream, set, part);
   return hipGetLastError();

@@ -12,7 +12,8 @@
 //   M_A = sum_j e^{r(T-t_j)} du_k/dx (t_j, X_j) A_j
 //   M_V = sum_j e^{r(T-t_j)} sum_f (du_k/dx C2_f + du_k/dv_f)(t_j, X_j) V_f,j
 //   M_J = e^{r(T-t_last)} du_k/dx (t_last, X_last) (e^J - 1)
-//   D   = P_k - E[P_k | S_T-]   (Merton; vanilla and sharp digital payoffs)
+//   D   = P_k - E[P_k | S_T-]   (Merton or Kou; vanilla and sharp digital
+//                                payoffs)
 // A_j and V_f,j are the block sums of the stepper's own increments whose
 // conditional mean is exactly zero: the stock's independent noise
 // sqrt(C3 (v + v')) G, and each factor's v' - E[v' | v] (closed form for NV
@@ -39,6 +40,7 @@
 //                       products (fitting), summed in FP64 by the strike's
 //                       lane
 //   cv_reduce_kernel    fixed-order sum of the waves' sums
+//   cv_debug_rb_kernel  E[P_k | S_T-] at given spots (debug entry)
 //   cv_train_kernel     FP32 forward/backward of 32 sampled rows per block
 //   cv_adam_kernel      gradient reduction (fixed order) and Adam update
 #pragma once
@@ -87,9 +89,109 @@ __device__ __forceinline__ void rb_term(const CvSet &c, double S, double K, doub
   }
 }
 
+// Kou terminal jump J = A - B: A ~ Gamma(j, e1) the sum of j up jumps, B ~
+// Gamma(m, e2) of m down jumps, j and m independent Poisson counts (a
+// compound Poisson process thins into independent up and down parts).
+// For integer shapes, conditioning on B and expanding (B + t)^i gives, at
+// t >= 0, with a = e1 / (e1 + e2), b = e2 / (e1 + e2),
+// Q_r(u) = e^{-u} sum_{k<r} u^k / k! (the upper regularised gamma function
+// at integer shape) and u = e1 t:
+//   P(A - B > t) = b^m sum_{l<j} C(m-1+l, l) a^l Q_{j-l}(u)
+//   f_{A-B}(t)   = e1 b^m sum_{l<j} C(m-1+l, l) a^l e^{-u} u^{j-1-l} / (j-1-l)!
+// (the negative binomial weights: l down-events among the first
+// events of the race of the two exponential clocks). Summed over j, m with
+// weights wu_j cu^j, wd_m cd^m; the (0, 0) term is zero at t >= 0.
+__device__ void kou_upper(const double *wu, int nu, double cu, const double *wd, int nd, double cd, double e1,
+                          double e2, double t, double &P, double &f) {
+  double pk[CV_RB_TERMS], Q[CV_RB_TERMS + 1], nb[CV_RB_TERMS];
+  const double u = e1 * t;
+  double term = exp(-u), q = 0.0;
+  for (int r = 0; r < nu; ++r) {
+    pk[r] = term;
+    Q[r] = q;
+    q += term;
+    term *= u / static_cast<double>(r + 1);
+  }
+  Q[nu] = q;
+  const double a = e1 / (e1 + e2), b = e2 / (e1 + e2);
+  double bm = 1.0, cdm = 1.0;
+  P = 0.0;
+  f = 0.0;
+  for (int m = 0; m < nd; ++m) {
+    nb[0] = bm;
+    for (int l = 0; l + 1 < nu; ++l) nb[l + 1] = nb[l] * (static_cast<double>(m + l) / static_cast<double>(l + 1)) * a;
+    double sp = 0.0, sf = 0.0, cj = 1.0;
+    for (int j = 1; j < nu; ++j) {
+      cj *= cu;
+      double s1 = 0.0, s2 = 0.0;
+      for (int l = 0; l < j; ++l) {
+        s1 += nb[l] * Q[j - l];
+        s2 += nb[l] * pk[j - 1 - l];
+      }
+      sp += wu[j] * cj * s1;
+      sf += wu[j] * cj * s2;
+    }
+    P += wd[m] * cdm * sp;
+    f += wd[m] * cdm * sf;
+    bm *= b;
+    cdm *= cd;
+  }
+  f *= e1;
+}
+
+// n >= 1 part of P(J > t) and of the density of J at t (tilted = false), or
+// of E[e^J 1{J > t}] (tilted = true: e^{A - B} turns Gamma(j, e1) into
+// (e1 / (e1 - 1))^j Gamma(j, e1 - 1) and Gamma(m, e2) into
+// (e2 / (e2 + 1))^m Gamma(m, e2 + 1)); t < 0 from the mirrored sum,
+// P(A - B > t) = 1 - P(B - A >= -t)
+__device__ void kou_tail(const CvSet &c, bool tilted, double t, double &P, double &f) {
+  const double e1 = tilted ? c.kou_e1 - 1.0 : c.kou_e1, e2 = tilted ? c.kou_e2 + 1.0 : c.kou_e2;
+  const double cu = tilted ? c.kou_e1 / e1 : 1.0, cd = tilted ? c.kou_e2 / e2 : 1.0;
+  if (t >= 0.0) {
+    kou_upper(c.rb_p, c.rb_n, cu, c.rb_s, c.rb_nd, cd, e1, e2, t, P, f);
+  } else {
+    double Pm;
+    kou_upper(c.rb_s, c.rb_nd, cd, c.rb_p, c.rb_n, cu, e2, e1, -t, Pm, f);
+    P = (tilted ? c.kou_m1 : c.kou_mass) - Pm;
+  }
+}
+
+// n >= 1 part of E[P(S e^{m0 + J})] under Kou jumps and its derivative in
+// log S, for x = log(S / K) + m0: with T0 = P(J > -x), T1 = E[e^J 1{J > -x}]
+// and f the density of J at -x, the call is K (e^x T1 - T0) with derivative
+// K e^x T1, the cash digital T0 with derivative f, the asset digital
+// K e^x T1 with derivative K e^x T1 + K f; puts by parity with the n >= 1
+// mass and E[e^J 1{n >= 1}]
+__device__ __noinline__ void kou_sum(const CvSet &c, double S, double K, double &g, double &dg) {
+  const double x = log(S / K) + c.rb_m0;
+  double T0, f, T1, f1;
+  kou_tail(c, false, -x, T0, f);
+  kou_tail(c, true, -x, T1, f1);
+  const double F = S * exp(c.rb_m0);
+  if (c.payoff == 0) {
+    if (c.call) {
+      g = F * T1 - K * T0;
+      dg = F * T1;
+    } else {
+      g = K * (c.kou_mass - T0) - F * (c.kou_m1 - T1);
+      dg = -F * (c.kou_m1 - T1);
+    }
+  } else if (c.payoff == 1) {
+    g = c.call ? T0 : c.kou_mass - T0;
+    dg = c.call ? f : -f;
+  } else {
+    g = c.call ? F * T1 : F * (c.kou_m1 - T1);
+    dg = c.call ? F * T1 + K * f : F * (c.kou_m1 - T1) - K * f;
+  }
+}
+
 // n >= 1 part of E[P_k | S_T- = S] and its derivative in log S, summed
 // directly
 __device__ __forceinline__ void rb_sum(const CvSet &c, double S, double K, double &g, double &dg) {
+  if (c.rb_kou) {
+    kou_sum(c, S, K, g, dg);
+    return;
+  }
   g = 0.0;
   dg = 0.0;
   for (int n = 0; n < c.rb_n; ++n) {
@@ -307,6 +409,23 @@ __global__ void cv_rb_table_kernel(CvSet set, double *__restrict__ tab) {
   rb_sum(set, exp(y), set.strikes[k], g, dg);
   tab[2 * idx] = g;
   tab[2 * idx + 1] = set.rb_h * dg;
+}
+
+// E[P_k | S_T- = spots[i]] per (spot i, strike k): from the table as the
+// estimator reads it (tab), and summed directly (dir; with its derivative in
+// log S of the n >= 1 part, ddir)
+__global__ void cv_debug_rb_kernel(CvSet set, const double *__restrict__ spots, int64_t n, double *__restrict__ tab,
+                                   double *__restrict__ dir, double *__restrict__ ddir) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n * set.nk) return;
+  const int64_t i = idx / set.nk;
+  const int k = static_cast<int>(idx % set.nk);
+  const double S = spots[i], K = set.strikes[k], S0 = S * exp(set.rb_m0);
+  tab[idx] = rb_expect(set, k, S, log(S), S0, K);
+  double g, dg;
+  rb_sum(set, S, K, g, dg);
+  dir[idx] = set.rb_p0 * cv_payoff(set, S0, K) + g;
+  ddir[idx] = dg;
 }
 
 // discounted (terminal-jump-expected) payoff of path p, strike k: the

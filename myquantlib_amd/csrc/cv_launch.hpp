@@ -31,6 +31,8 @@ template <typename R>
 hipError_t launch_nci_rowmean(const R *tab, int rows, int grid, double dx, double d, double *mean,
                               hipStream_t stream);
 hipError_t launch_cv_rb_table(const CvSet &set, double *tab, hipStream_t stream);
+hipError_t launch_cv_debug_rb(const CvSet &set, const double *spots, int64_t n, double *tab, double *dir,
+                              double *ddir, hipStream_t stream);
 hipError_t launch_cv_moments(const CvSet &set, double *part, int blocks, hipStream_t stream);
 // mode 0: price only, 1: with plain delta and gamma, 2: fitting sums; each
 // task's `sums` receives nk x ns sums (ns = 4, 8 or 21); tasks: device

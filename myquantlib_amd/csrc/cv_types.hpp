@@ -68,7 +68,12 @@ struct CvSet {
   const double *rb;
   double rb_y0, rb_h, rb_inv_h;
   double rb_p0, rb_m0;
-  int rb_n;
+  // Merton: n-th term rb_p = Poisson weight, J_n ~ N(rb_m, rb_s^2).
+  // Kou (rb_kou): the up and down jumps are independent Poisson counts;
+  // rb_p[j] (j < rb_n) and rb_s[m] (m < rb_nd) their weights, the n >= 1
+  // part's mass and E[e^J 1{n >= 1}] in kou_mass, kou_m1
+  int rb_n, rb_kou, rb_nd;
+  double kou_e1, kou_e2, kou_mass, kou_m1;
   double rb_p[CV_RB_TERMS], rb_m[CV_RB_TERMS], rb_s[CV_RB_TERMS];
 };
 

@@ -1801,6 +1801,88 @@ struct CvRecord {
   DevBuf rec, st, sm, strikes, rb, mu, isd;
 };
 
+// The set's terminal-jump expectation E[P_k | S_T-] (vanilla and sharp
+// digital payoffs under Merton or Kou jumps; S.payoff, S.call, S.smooth,
+// S.strikes and S.nk set): its parameters and, on the log-spot grid around
+// S0, its table, filled on the context stream into `rb`.
+void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, CvSet &S, DevBuf &rb) {
+  const int nk = S.nk;
+  // Merton terminal jump: E[P | S_T-] in closed form for vanilla and sharp
+  // digital payoffs: J = m0 + n mu + sqrt(n) delta Z, n ~ Poisson(lambda T),
+  // m0 the drift compensation (JumpPolicies.hpp:55-135)
+  // Kou terminal jump (JumpPolicies.hpp:55-135): up and down jump counts are
+  // independent Poisson(lambda T p), Poisson(lambda T (1 - p)); sums of
+  // exponentials, so E[P | S_T-] has a closed form at integer shapes
+  S.rb_on = ((m.jump_kind == MYQL_JUMP_MERTON || m.jump_kind == MYQL_JUMP_KOU) && !S.smooth) ? 1 : 0;
+  S.rb_kou = 0;
+  S.rb_nd = 0;
+  if (S.rb_on && m.jump_kind == MYQL_JUMP_KOU) {
+    const double lam = m.jump[0], p = m.jump[1], e1 = m.jump[2], e2 = m.jump[3];
+    const double kap = (p * e1) / (e1 - 1.0) + ((1.0 - p) * e2) / (e2 + 1.0) - 1.0;
+    S.rb_kou = 1;
+    S.kou_e1 = e1;
+    S.kou_e2 = e2;
+    S.rb_m0 = -lam * kap * T;
+    S.rb_p0 = std::exp(-lam * T);
+    // Poisson weights of one side until their mass is covered
+    auto weights = [](double mean, double *w, int &n) {
+      double pn = std::exp(-mean), mass = 0.0;
+      n = 0;
+      while (n < CV_RB_TERMS && mass < 1.0 - 1e-17) {
+        w[n] = pn;
+        mass += pn;
+        ++n;
+        pn *= mean / n;
+      }
+      return mass;
+    };
+    const double mu = weights(lam * T * p, S.rb_p, S.rb_n), md = weights(lam * T * (1.0 - p), S.rb_s, S.rb_nd);
+    double su = 0.0, sd = 0.0, cu = 1.0, cd = 1.0;
+    for (int j = 0; j < S.rb_n; ++j, cu *= e1 / (e1 - 1.0)) su += S.rb_p[j] * cu;
+    for (int j = 0; j < S.rb_nd; ++j, cd *= e2 / (e2 + 1.0)) sd += S.rb_s[j] * cd;
+    S.kou_mass = mu * md - S.rb_p0;
+    S.kou_m1 = su * sd - S.rb_p0;
+    // the n >= 1 sums stay unbiased only where both sides' mass is covered
+    // and e^J has a mean (e1 > 1)
+    const double mass = mu * md;
+    if (!(e1 > 1.0) || mass < 1.0 - 1e-14) S.rb_on = 0;
+    else {
+      S.rb_y0 = std::log(S0) - 3.0;
+      S.rb_h = 5.0 / (CV_RB_PTS - 1);
+      S.rb_inv_h = 1.0 / S.rb_h;
+      rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
+      S.rb = rb.as<double>();
+      HIPCHECK(launch_cv_rb_table(S, rb.as<double>(), c->stream));
+    }
+  } else if (S.rb_on) {
+    const double lt = m.jump[0] * T, mu = m.jump[1], dl = m.jump[2];
+    S.rb_m0 = -m.jump[0] * (std::exp(mu + 0.5 * dl * dl) - 1.0) * T;
+    S.rb_p0 = std::exp(-lt);
+    double pn = S.rb_p0, mass = S.rb_p0;
+    S.rb_n = 0;
+    for (int n = 1; n <= CV_RB_TERMS && mass < 1.0 - 1e-17; ++n) {
+      pn *= lt / n;
+      mass += pn;
+      S.rb_p[S.rb_n] = pn;
+      S.rb_m[S.rb_n] = S.rb_m0 + n * mu;
+      S.rb_s[S.rb_n] = std::sqrt(static_cast<double>(n)) * dl;
+      ++S.rb_n;
+    }
+    // counts beyond the tabulated terms would bias E[P | S_T-]: then the
+    // estimator goes without that term (D = 0), and stays unbiased
+    if (mass < 1.0 - 1e-14) {
+      S.rb_on = 0;
+    } else {
+      S.rb_y0 = std::log(S0) - 3.0;
+      S.rb_h = 5.0 / (CV_RB_PTS - 1);
+      S.rb_inv_h = 1.0 / S.rb_h;
+      rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
+      S.rb = rb.as<double>();
+      HIPCHECK(launch_cv_rb_table(S, rb.as<double>(), c->stream));
+    }
+  }
+}
+
 // Set up the recording of the base paths of job `jb` under config `cfg`
 // (paths, seed, stream layout, precision) at `points` control points: the
 // job, the buffers and the set's terminal-jump expectation tables. The paths
@@ -1863,37 +1945,7 @@ void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, in
     out.sm.reserve(S.Ms * sizeof(double));
     S.Sm = out.sm.as<double>();
   }
-  // Merton terminal jump: E[P | S_T-] in closed form for vanilla and sharp
-  // digital payoffs: J = m0 + n mu + sqrt(n) delta Z, n ~ Poisson(lambda T),
-  // m0 the drift compensation (JumpPolicies.hpp:55-135)
-  S.rb_on = (m.jump_kind == MYQL_JUMP_MERTON && !S.smooth) ? 1 : 0;
-  if (S.rb_on) {
-    const double lt = m.jump[0] * T, mu = m.jump[1], dl = m.jump[2];
-    S.rb_m0 = -m.jump[0] * (std::exp(mu + 0.5 * dl * dl) - 1.0) * T;
-    S.rb_p0 = std::exp(-lt);
-    double pn = S.rb_p0, mass = S.rb_p0;
-    S.rb_n = 0;
-    for (int n = 1; n <= CV_RB_TERMS && mass < 1.0 - 1e-17; ++n) {
-      pn *= lt / n;
-      mass += pn;
-      S.rb_p[S.rb_n] = pn;
-      S.rb_m[S.rb_n] = S.rb_m0 + n * mu;
-      S.rb_s[S.rb_n] = std::sqrt(static_cast<double>(n)) * dl;
-      ++S.rb_n;
-    }
-    // counts beyond the tabulated terms would bias E[P | S_T-]: then the
-    // estimator goes without that term (D = 0), and stays unbiased
-    if (mass < 1.0 - 1e-14) {
-      S.rb_on = 0;
-    } else {
-      S.rb_y0 = std::log(jb.S0) - 3.0;
-      S.rb_h = 5.0 / (CV_RB_PTS - 1);
-      S.rb_inv_h = 1.0 / S.rb_h;
-      out.rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
-      S.rb = out.rb.as<double>();
-      HIPCHECK(launch_cv_rb_table(S, out.rb.as<double>(), c->stream));
-    }
-  }
+  cv_terminal_jump(c, m, T, jb.S0, S, out.rb);
 }
 
 // Device copies of sets (kept alive by the caller until it synchronises).
@@ -3054,6 +3106,51 @@ int myql_debug_set_launch_shape(myql_ctx *c, int32_t precision, int32_t n_factor
 int myql_debug_chi2_quantile(myql_ctx *c, const double *dof, const double *p, int64_t n, double *out) {
   if (!dof || !p || !out) return MYQL_E_INVALID_ARG;
   return debug_map(c, 2, dof, p, n, out, nullptr);
+}
+
+int myql_debug_cv_jump_expectation(myql_ctx *c, const myql_model *model, const myql_instrument *instrument,
+                                   int32_t greek_mode, double S0, const double *spots, int64_t n, double *tab,
+                                   double *dir, double *ddir, int32_t *table_on) {
+  if (!c) return MYQL_E_INVALID_ARG;
+  try {
+    set_device(c);
+    if (!model || !instrument || !spots || !tab || !dir || !ddir || !table_on || n < 0 || !(S0 > 0.0))
+      throw Fail(MYQL_E_INVALID_ARG, "null argument, negative count or non-positive S0");
+    if (model->jump_kind != MYQL_JUMP_MERTON && model->jump_kind != MYQL_JUMP_KOU)
+      throw Fail(MYQL_E_INVALID_ARG, "model has no jumps");
+    if (instrument->n_strikes < 1 || instrument->n_strikes > 64 || !instrument->strikes || !(instrument->maturity > 0.0))
+      throw Fail(MYQL_E_INVALID_ARG, "instrument needs 1..64 strikes and a positive maturity");
+    for (int64_t i = 0; i < n; ++i)
+      if (!(spots[i] > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "spots must be positive");
+    const int nk = static_cast<int>(instrument->n_strikes);
+    CvSet S{};
+    DevBuf ks, rb, d_sp, d_out;
+    ks.reserve(nk * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(ks.p, instrument->strikes, nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    S.strikes = ks.as<double>();
+    S.nk = nk;
+    S.payoff = instrument->payoff;
+    S.call = instrument->option_type == MYQL_CALL ? 1 : 0;
+    S.smooth = (greek_mode != MYQL_GREEK_NONE && instrument->payoff != MYQL_PAYOFF_VANILLA) ? 1 : 0;
+    cv_terminal_jump(c, *model, instrument->maturity, S0, S, rb);
+    *table_on = S.rb_on;
+    if (S.rb_on && n > 0) {
+      const size_t m = static_cast<size_t>(n) * nk;
+      d_sp.reserve(n * sizeof(double));
+      d_out.reserve(3 * m * sizeof(double));
+      HIPCHECK(hipMemcpyAsync(d_sp.p, spots, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      double *o = d_out.as<double>();
+      HIPCHECK(launch_cv_debug_rb(S, d_sp.as<double>(), n, o, o + m, o + 2 * m, c->stream));
+      HIPCHECK(hipMemcpyAsync(tab, o, m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipMemcpyAsync(dir, o + m, m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipMemcpyAsync(ddir, o + 2 * m, m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->err.clear();
+    return MYQL_OK;
+  } catch (const Fail &e) {
+    return fail(c, e);
+  }
 }
 
 int myql_cv_train(myql_ctx *c, const myql_mc_job *jobs, int64_t n_jobs, const myql_cv_config *cfg, myql_cv_net *net) {

@@ -441,6 +441,20 @@ class Engine:
         self._check(rc)
         return tuple(out)
 
+    def cv_jump_expectation(self, model: "Model", instr: "InstrumentSpec", mode: int, S0: float, spots) -> tuple:
+        """The control variate's terminal-jump expectation E[P_k | S_T- = S]
+        at the given spots (myql_debug_cv_jump_expectation): (tab, dir, ddir),
+        each n_spots x n_strikes, or None where the estimator goes without it."""
+        spots = np.ascontiguousarray(spots, dtype=np.float64)
+        st = instr.struct()
+        out = [np.zeros((spots.size, st.n_strikes)) for _ in range(3)]
+        on = C.c_int32(0)
+        pd = C.POINTER(C.c_double)
+        self._check(self.lib.myql_debug_cv_jump_expectation(self.ctx, C.byref(model), C.byref(st), mode, S0,
+                                                            spots.ctypes.data_as(pd), spots.size,
+                                                            *[o.ctypes.data_as(pd) for o in out], C.byref(on)))
+        return tuple(out) if on.value else None
+
     def chi2_quantile(self, dof: np.ndarray, p: np.ndarray) -> np.ndarray:
         dof = np.ascontiguousarray(dof, dtype=np.float64)
         p = np.ascontiguousarray(p, dtype=np.float64)
