@@ -98,43 +98,24 @@ __device__ __forceinline__ void rb_term(const CvSet &c, double S, double K, doub
 // at integer shape) and u = e1 t:
 //   P(A - B > t) = b^m sum_{l<j} C(m-1+l, l) a^l Q_{j-l}(u)
 //   f_{A-B}(t)   = e1 b^m sum_{l<j} C(m-1+l, l) a^l e^{-u} u^{j-1-l} / (j-1-l)!
-// (the negative binomial weights: l down-events among the first
-// events of the race of the two exponential clocks). Summed over j, m with
-// weights wu_j cu^j, wd_m cd^m; the (0, 0) term is zero at t >= 0.
-__device__ void kou_upper(const double *wu, int nu, double cu, const double *wd, int nd, double cd, double e1,
-                          double e2, double t, double &P, double &f) {
-  double pk[CV_RB_TERMS], Q[CV_RB_TERMS + 1], nb[CV_RB_TERMS];
+// (negative binomial weights: l down-events among the first events of the
+// race of the two exponential clocks). Summed over j, m with weights
+// wu_j cu^j, wd_m cd^m, the sums regroup by r = j - l into
+//   P(t) = sum_{r>=1} G_r Q_r(u),  f(t) = e1 sum_{r>=1} G_r e^{-u} u^{r-1} / (r-1)!
+// with G_r = sum_l NB_l wu_{l+r} cu^{l+r} and NB_l = sum_m wd_m cd^m
+// C(m-1+l, l) a^l b^m independent of t (built on the host, kou_g); the
+// (0, 0) term is zero at t >= 0.
+__device__ __forceinline__ void kou_upper(const double *__restrict__ G, int n, double e1, double t, double &P,
+                                          double &f) {
   const double u = e1 * t;
   double term = exp(-u), q = 0.0;
-  for (int r = 0; r < nu; ++r) {
-    pk[r] = term;
-    Q[r] = q;
-    q += term;
-    term *= u / static_cast<double>(r + 1);
-  }
-  Q[nu] = q;
-  const double a = e1 / (e1 + e2), b = e2 / (e1 + e2);
-  double bm = 1.0, cdm = 1.0;
   P = 0.0;
   f = 0.0;
-  for (int m = 0; m < nd; ++m) {
-    nb[0] = bm;
-    for (int l = 0; l + 1 < nu; ++l) nb[l + 1] = nb[l] * (static_cast<double>(m + l) / static_cast<double>(l + 1)) * a;
-    double sp = 0.0, sf = 0.0, cj = 1.0;
-    for (int j = 1; j < nu; ++j) {
-      cj *= cu;
-      double s1 = 0.0, s2 = 0.0;
-      for (int l = 0; l < j; ++l) {
-        s1 += nb[l] * Q[j - l];
-        s2 += nb[l] * pk[j - 1 - l];
-      }
-      sp += wu[j] * cj * s1;
-      sf += wu[j] * cj * s2;
-    }
-    P += wd[m] * cdm * sp;
-    f += wd[m] * cdm * sf;
-    bm *= b;
-    cdm *= cd;
+  for (int r = 1; r < n; ++r) {
+    q += term;  // Q_r(u)
+    f += G[r] * term;
+    P += G[r] * q;
+    term *= u / static_cast<double>(r);
   }
   f *= e1;
 }
@@ -144,14 +125,13 @@ __device__ void kou_upper(const double *wu, int nu, double cu, const double *wd,
 // (e1 / (e1 - 1))^j Gamma(j, e1 - 1) and Gamma(m, e2) into
 // (e2 / (e2 + 1))^m Gamma(m, e2 + 1)); t < 0 from the mirrored sum,
 // P(A - B > t) = 1 - P(B - A >= -t)
-__device__ void kou_tail(const CvSet &c, bool tilted, double t, double &P, double &f) {
-  const double e1 = tilted ? c.kou_e1 - 1.0 : c.kou_e1, e2 = tilted ? c.kou_e2 + 1.0 : c.kou_e2;
-  const double cu = tilted ? c.kou_e1 / e1 : 1.0, cd = tilted ? c.kou_e2 / e2 : 1.0;
+__device__ __forceinline__ void kou_tail(const CvSet &c, bool tilted, double t, double &P, double &f) {
+  const int o = tilted ? 1 : 0;
   if (t >= 0.0) {
-    kou_upper(c.rb_p, c.rb_n, cu, c.rb_s, c.rb_nd, cd, e1, e2, t, P, f);
+    kou_upper(c.kou_g[o], c.rb_n, tilted ? c.kou_e1 - 1.0 : c.kou_e1, t, P, f);
   } else {
     double Pm;
-    kou_upper(c.rb_s, c.rb_nd, cd, c.rb_p, c.rb_n, cu, e2, e1, -t, Pm, f);
+    kou_upper(c.kou_g[2 + o], c.rb_nd, tilted ? c.kou_e2 + 1.0 : c.kou_e2, -t, Pm, f);
     P = (tilted ? c.kou_m1 : c.kou_mass) - Pm;
   }
 }
@@ -185,9 +165,79 @@ __device__ __noinline__ void kou_sum(const CvSet &c, double S, double K, double 
   }
 }
 
+// n >= 1 part of P(J > t) (tilted = false) or E[e^J 1{J > t}] (tilted =
+// true), J the jumps' sum without the compensation, and the density of J
+// at t (n >= 1 part): Kou in closed form (kou_tail); Merton from the normal
+// n-jump sums N(n mu, n delta^2)
+__device__ void jump_tail(const CvSet &c, bool tilted, double t, double &P, double &f) {
+  if (c.rb_kou) {
+    kou_tail(c, tilted, t, P, f);
+    return;
+  }
+  P = 0.0;
+  f = 0.0;
+  for (int n = 0; n < c.rb_n; ++n) {
+    const double m = c.rb_m[n] - c.rb_m0, s = c.rb_s[n];
+    if (tilted) P += c.rb_p[n] * exp(m + 0.5 * s * s) * norm_cdf((m + s * s - t) / s);
+    else P += c.rb_p[n] * norm_cdf((m - t) / s);
+    f += c.rb_p[n] * norm_pdf((t - m) / s) / s;
+  }
+}
+
+// n >= 1 part of E[P(S e^{m0 + J})] for an auto-smoothed digital,
+// P(s) = Phi(+-(s - K) / eps) (times s for the asset digital), and its
+// derivative in log S. Outside s in [K - 10 eps, K + 10 eps] the payoff is
+// the sharp one to 1e-23, so those parts come from the tails in closed form;
+// inside, Gauss-Legendre on 16 panels per side of the density's break at
+// J = 0 integrates the smoothed payoff against the density of J.
+__device__ __noinline__ void smooth_sum(const CvSet &c, double S, double K, double &g, double &dg) {
+  const double F = S * exp(c.rb_m0), eps = 0.70710678118654752440 / c.eps_scale;
+  const double sg = c.call ? 1.0 : -1.0;
+  const double sl = K - 10.0 * eps, sh = K + 10.0 * eps;
+  const double zh = log(sh / F), zl = sl > 0.0 ? log(sl / F) : zh - 60.0;
+  double P, f;
+  const bool asset = c.payoff == 2;
+  const double mass = asset ? c.kou_m1 : c.kou_mass;
+  if (c.call) {
+    jump_tail(c, asset, zh, P, f);
+    g = asset ? F * P : P;
+  } else {
+    jump_tail(c, asset, zl, P, f);
+    g = asset ? F * (mass - P) : mass - P;
+  }
+  dg = asset ? g : 0.0;
+  // the window, split at J = 0 where the Kou density breaks
+  double e[3] = {zl, zh, zh};
+  int ne = 1;
+  if (zl < 0.0 && 0.0 < zh) {
+    e[1] = 0.0;
+    ne = 2;
+  }
+  constexpr int NP = 16;
+  for (int q = 0; q < ne; ++q) {
+    const double a = e[q], h = (e[q + 1] - e[q]) / NP;
+    for (int pnl = 0; pnl < NP; ++pnl) {
+      const double mid = a + (pnl + 0.5) * h;
+      for (int i = 0; i < 16; ++i) {
+        const double z = mid + 0.5 * h * c.gl_x[i], wq = 0.5 * h * c.gl_w[i];
+        double Pz, fz;
+        jump_tail(c, false, z, Pz, fz);
+        const double s = F * exp(z), d = sg * (s - K) / eps;
+        const double w = norm_cdf(d), dw = sg * norm_pdf(d) * s / eps;  // dw / dlog s
+        g += wq * fz * (asset ? s * w : w);
+        dg += wq * fz * (asset ? s * w + s * dw : dw);
+      }
+    }
+  }
+}
+
 // n >= 1 part of E[P_k | S_T- = S] and its derivative in log S, summed
 // directly
 __device__ __forceinline__ void rb_sum(const CvSet &c, double S, double K, double &g, double &dg) {
+  if (c.smooth) {
+    smooth_sum(c, S, K, g, dg);
+    return;
+  }
   if (c.rb_kou) {
     kou_sum(c, S, K, g, dg);
     return;

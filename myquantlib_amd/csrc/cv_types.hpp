@@ -59,8 +59,9 @@ struct CvSet {
   double T, df;
   // training-target normalisation per strike: (e^{-r(T-t)} P~ - mu_k) * inv_sd_k
   const float *mu, *inv_sd;
-  // terminal-jump expectation E[P_k | S_T-] (Merton jumps; vanilla and sharp
-  // digital payoffs): p0 * P_k(S_T- e^{m0}) (no jump) plus, on y = log S_T-,
+  // terminal-jump expectation E[P_k | S_T-] (Merton or Kou jumps; vanilla,
+  // sharp and smoothed digital payoffs): p0 * P_k(S_T- e^{m0}) (no jump)
+  // plus, on y = log S_T-,
   // the n >= 1 part rb[(i*nk + k)*2 + {0,1}] = {g, h g'} at grid point i
   // (cubic Hermite; strikes adjacent, so a path's lookups for all strikes
   // are contiguous); outside the grid the n >= 1 sum directly
@@ -69,12 +70,17 @@ struct CvSet {
   double rb_y0, rb_h, rb_inv_h;
   double rb_p0, rb_m0;
   // Merton: n-th term rb_p = Poisson weight, J_n ~ N(rb_m, rb_s^2).
-  // Kou (rb_kou): the up and down jumps are independent Poisson counts;
-  // rb_p[j] (j < rb_n) and rb_s[m] (m < rb_nd) their weights, the n >= 1
-  // part's mass and E[e^J 1{n >= 1}] in kou_mass, kou_m1
+  // Kou (rb_kou): the series coefficients kou_g[2 o + tilted][r] of
+  // P(+-J > t) = sum_r G_r Q_r(e t) (orientation o = 0: up side first,
+  // 1: mirrored; r < rb_n resp. rb_nd). Both laws: the n >= 1 part's mass
+  // and E[e^J 1{n >= 1}] (J without m0) in kou_mass, kou_m1
   int rb_n, rb_kou, rb_nd;
   double kou_e1, kou_e2, kou_mass, kou_m1;
   double rb_p[CV_RB_TERMS], rb_m[CV_RB_TERMS], rb_s[CV_RB_TERMS];
+  double kou_g[4][CV_RB_TERMS];
+  // smoothed digitals: 16-point Gauss-Legendre nodes and weights on [-1, 1]
+  // for the expectation over the payoff's transition window
+  double gl_x[16], gl_w[16];
 };
 
 // Batched launches: task t owns blocks [first_block, next task's

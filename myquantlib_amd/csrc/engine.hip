@@ -1801,19 +1801,56 @@ struct CvRecord {
   DevBuf rec, st, sm, strikes, rb, mu, isd;
 };
 
-// The set's terminal-jump expectation E[P_k | S_T-] (vanilla and sharp
-// digital payoffs under Merton or Kou jumps; S.payoff, S.call, S.smooth,
-// S.strikes and S.nk set): its parameters and, on the log-spot grid around
-// S0, its table, filled on the context stream into `rb`.
-void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, CvSet &S, DevBuf &rb) {
+// The set's terminal-jump expectation E[P_k | S_T-] (vanilla, sharp and
+// smoothed digital payoffs under Merton or Kou jumps; S.payoff, S.call,
+// S.smooth, S.eps_scale, S.strikes and S.nk set; ks the strikes on the
+// host): its parameters and, on the log-spot grid around S0, its table,
+// filled on the context stream into `rb`.
+// 16-point Gauss-Legendre rule on [-1, 1]: roots of P_16 by Newton's
+// method from the Chebyshev-like first guesses, w = 2 / ((1 - x^2) P_16'(x)^2)
+static void gauss_legendre16(double *x, double *w) {
+  constexpr int n = 16;
+  auto legendre = [](double z, double &dp) {
+    double p1 = 1.0, p2 = 0.0;
+    for (int j = 1; j <= n; ++j) {
+      const double p3 = p2;
+      p2 = p1;
+      p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+    }
+    dp = n * (z * p1 - p2) / (z * z - 1.0);
+    return p1;
+  };
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5)), dp;
+    for (int it = 0; it < 100; ++it) {
+      const double dz = legendre(z, dp) / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    legendre(z, dp);
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, const double *ks, CvSet &S,
+                      DevBuf &rb) {
   const int nk = S.nk;
+  gauss_legendre16(S.gl_x, S.gl_w);
   // Merton terminal jump: E[P | S_T-] in closed form for vanilla and sharp
   // digital payoffs: J = m0 + n mu + sqrt(n) delta Z, n ~ Poisson(lambda T),
   // m0 the drift compensation (JumpPolicies.hpp:55-135)
   // Kou terminal jump (JumpPolicies.hpp:55-135): up and down jump counts are
   // independent Poisson(lambda T p), Poisson(lambda T (1 - p)); sums of
   // exponentials, so E[P | S_T-] has a closed form at integer shapes
-  S.rb_on = ((m.jump_kind == MYQL_JUMP_MERTON || m.jump_kind == MYQL_JUMP_KOU) && !S.smooth) ? 1 : 0;
+  S.rb_on = (m.jump_kind == MYQL_JUMP_MERTON || m.jump_kind == MYQL_JUMP_KOU) ? 1 : 0;
+  // smoothed digitals: the transition window [K - 10 eps, K + 10 eps] must
+  // lie at positive spots
+  if (S.smooth) {
+    const double eps = 0.70710678118654752440 / S.eps_scale;
+    for (int k = 0; k < nk; ++k)
+      if (!(ks[k] - 10.0 * eps > 0.0) || !std::isfinite(eps)) S.rb_on = 0;
+  }
   S.rb_kou = 0;
   S.rb_nd = 0;
   if (S.rb_on && m.jump_kind == MYQL_JUMP_KOU) {
@@ -1836,10 +1873,40 @@ void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, CvS
       }
       return mass;
     };
-    const double mu = weights(lam * T * p, S.rb_p, S.rb_n), md = weights(lam * T * (1.0 - p), S.rb_s, S.rb_nd);
+    double wu[CV_RB_TERMS], wd[CV_RB_TERMS];
+    const double mu = weights(lam * T * p, wu, S.rb_n), md = weights(lam * T * (1.0 - p), wd, S.rb_nd);
     double su = 0.0, sd = 0.0, cu = 1.0, cd = 1.0;
-    for (int j = 0; j < S.rb_n; ++j, cu *= e1 / (e1 - 1.0)) su += S.rb_p[j] * cu;
-    for (int j = 0; j < S.rb_nd; ++j, cd *= e2 / (e2 + 1.0)) sd += S.rb_s[j] * cd;
+    for (int j = 0; j < S.rb_n; ++j, cu *= e1 / (e1 - 1.0)) su += wu[j] * cu;
+    for (int j = 0; j < S.rb_nd; ++j, cd *= e2 / (e2 + 1.0)) sd += wd[j] * cd;
+    // series coefficients G_r = sum_l NB_l W_{l+r} of P(A - B > t) =
+    // sum_r G_r Q_r(eA t), W_j = wA_j cA^j, NB_l = sum_m wB_m cB^m
+    // C(m-1+l, l) a^l b^m, a = eA / (eA + eB), b = 1 - a; per orientation
+    // (A the up or the down side) and tilt
+    auto series = [](const double *wA, int nA, double cA, const double *wB, int nB, double cB, double eA,
+                     double eB, double *G) {
+      const double a = eA / (eA + eB), b = eB / (eA + eB);
+      double W[CV_RB_TERMS], NB[CV_RB_TERMS];
+      double cj = 1.0;
+      for (int j = 0; j < nA; ++j, cj *= cA) {
+        W[j] = wA[j] * cj;
+        NB[j] = 0.0;
+      }
+      double vm = 1.0;
+      for (int m = 0; m < nB; ++m, vm *= b * cB) {
+        double nb = wB[m] * vm;  // V_m b^m C(m - 1, 0)
+        for (int l = 0; l < nA; ++l) {
+          NB[l] += nb;
+          nb *= (static_cast<double>(m + l) / static_cast<double>(l + 1)) * a;
+        }
+      }
+      for (int r = 0; r < CV_RB_TERMS; ++r) G[r] = 0.0;
+      for (int r = 1; r < nA; ++r)
+        for (int l = 0; l + r < nA; ++l) G[r] += NB[l] * W[l + r];
+    };
+    series(wu, S.rb_n, 1.0, wd, S.rb_nd, 1.0, e1, e2, S.kou_g[0]);
+    series(wu, S.rb_n, e1 / (e1 - 1.0), wd, S.rb_nd, e2 / (e2 + 1.0), e1 - 1.0, e2 + 1.0, S.kou_g[1]);
+    series(wd, S.rb_nd, 1.0, wu, S.rb_n, 1.0, e2, e1, S.kou_g[2]);
+    series(wd, S.rb_nd, e2 / (e2 + 1.0), wu, S.rb_n, e1 / (e1 - 1.0), e2 + 1.0, e1 - 1.0, S.kou_g[3]);
     S.kou_mass = mu * md - S.rb_p0;
     S.kou_m1 = su * sd - S.rb_p0;
     // the n >= 1 sums stay unbiased only where both sides' mass is covered
@@ -1867,6 +1934,12 @@ void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, CvS
       S.rb_m[S.rb_n] = S.rb_m0 + n * mu;
       S.rb_s[S.rb_n] = std::sqrt(static_cast<double>(n)) * dl;
       ++S.rb_n;
+    }
+    S.kou_mass = 0.0;
+    S.kou_m1 = 0.0;
+    for (int n = 0; n < S.rb_n; ++n) {
+      S.kou_mass += S.rb_p[n];
+      S.kou_m1 += S.rb_p[n] * std::exp(S.rb_m[n] - S.rb_m0 + 0.5 * S.rb_s[n] * S.rb_s[n]);
     }
     // counts beyond the tabulated terms would bias E[P | S_T-]: then the
     // estimator goes without that term (D = 0), and stays unbiased
@@ -1945,7 +2018,7 @@ void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, in
     out.sm.reserve(S.Ms * sizeof(double));
     S.Sm = out.sm.as<double>();
   }
-  cv_terminal_jump(c, m, T, jb.S0, S, out.rb);
+  cv_terminal_jump(c, m, T, jb.S0, in.strikes, S, out.rb);
 }
 
 // Device copies of sets (kept alive by the caller until it synchronises).
@@ -3132,7 +3205,10 @@ int myql_debug_cv_jump_expectation(myql_ctx *c, const myql_model *model, const m
     S.payoff = instrument->payoff;
     S.call = instrument->option_type == MYQL_CALL ? 1 : 0;
     S.smooth = (greek_mode != MYQL_GREEK_NONE && instrument->payoff != MYQL_PAYOFF_VANILLA) ? 1 : 0;
-    cv_terminal_jump(c, *model, instrument->maturity, S0, S, rb);
+    if (S.smooth && !(instrument->payoff_eps > 0.0))
+      throw Fail(MYQL_E_INVALID_ARG, "a digital payoff in a Greek mode needs payoff_eps > 0 here");
+    S.eps_scale = S.smooth ? 0.70710678118654752440 / instrument->payoff_eps : 0.0;
+    cv_terminal_jump(c, *model, instrument->maturity, S0, instrument->strikes, S, rb);
     *table_on = S.rb_on;
     if (S.rb_on && n > 0) {
       const size_t m = static_cast<size_t>(n) * nk;

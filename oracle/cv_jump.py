@@ -11,6 +11,10 @@ S 1{S > K}).
 
   terminal_expectation(...)  E[P(S e^J)] (include_zero=False: the n >= 1
                              part only)
+  mc_terminal_expectation    Monte Carlo of the same (also for digitals
+                             smoothed with bandwidth eps, which the device
+                             integrates over the payoff's transition window
+                             and the GPU test checks against this)
 
 Kou jumps (up with probability p, Exp(eta1); down Exp(eta2)): the n >= 1
 part is summed over the independent up and down counts (j, m) with the
@@ -35,12 +39,15 @@ from scipy import integrate, special, stats
 VANILLA, CASH, ASSET = 0, 1, 2
 
 
-def _payoff(payoff: int, call: bool, S, K):
+def _payoff(payoff: int, call: bool, S, K, eps: float | None = None):
+    """Payoffs.hpp:14-69; eps > 0: digitals smoothed to Phi(+-(S - K) / eps)
+    (0.5 erfc(-diff / (sqrt(2) eps)), as the pricer smooths them in its Greek
+    modes)."""
     S = np.asarray(S, dtype=np.float64)
     d = S - K if call else K - S
     if payoff == VANILLA:
         return np.maximum(d, 0.0)
-    w = (d > 0.0).astype(np.float64)
+    w = special.ndtr(d / eps) if eps else (d > 0.0).astype(np.float64)
     return w if payoff == CASH else S * w
 
 
@@ -141,11 +148,11 @@ def terminal_expectation(payoff: int, call: bool, K: float, S: float, T: float, 
 
 
 def mc_terminal_expectation(payoff: int, call: bool, K: float, S: float, T: float, jump_kind: str, params,
-                            n: int, seed: int):
+                            n: int, seed: int, eps: float | None = None):
     """Monte Carlo estimate (mean, standard error) of E[P(S e^{m0 + J})]
     with the jumps drawn as the reference draws them: a Poisson count for
     the maturity, then per jump its side and size (Kou) or its normal size
-    (Merton)."""
+    (Merton). eps: smoothed digitals."""
     rng = np.random.default_rng(seed)
     lam = params[0]
     cnt = rng.poisson(lam * T, size=n)
@@ -160,5 +167,5 @@ def mc_terminal_expectation(payoff: int, call: bool, K: float, S: float, T: floa
         m0 = -lam * (math.exp(mu + 0.5 * dl * dl) - 1.0) * T
         size = rng.normal(mu, dl, idx.size)
     J = np.bincount(idx, weights=size, minlength=n)
-    v = _payoff(payoff, call, S * np.exp(m0 + J), K)
+    v = _payoff(payoff, call, S * np.exp(m0 + J), K, eps)
     return float(v.mean()), float(v.std(ddof=1) / math.sqrt(n))
