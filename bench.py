@@ -313,6 +313,30 @@ def cpu_baseline(target_s: float = 8.0):
                       f"({cpu_name})"}
 
 
+def cv_jump_tables(eng):
+    """The control variate's terminal-jump tables of the Bates-Kou strip alone
+    (2,048 log-spot points x 64 strikes per maturity; vanilla calls, and
+    cash digitals smoothed with the strip's automatic bandwidth at T = 1),
+    built through myql_debug_cv_jump_expectation, which builds the table
+    the pricing call builds and reads it back at one spot; wall ms per strip
+    of 8 tables, after one warm-up strip."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import InstrumentSpec
+    m = model_bates_kou()
+    out = {}
+    for name, payoff, mode, eps in (("vanilla", capi.PAYOFF_VANILLA, capi.GREEK_NONE, -1.0),
+                                    ("smoothed_cash_digital", capi.PAYOFF_CASH, capi.GREEK_ESSENTIAL, 2.0)):
+        ins = [InstrumentSpec(payoff=payoff, strikes=STRIKES, maturity=T, payoff_eps=eps) for T in MATURITIES]
+        for rep in range(2):
+            t0 = time.perf_counter()
+            for i in ins:
+                assert eng.cv_jump_expectation(m, i, mode, S0, np.array([S0])) is not None
+            ms = 1e3 * (time.perf_counter() - t0)
+        out[name + "_strip_ms"] = ms
+        print(f"# cv jump tables ({name}): {ms:.1f} ms per strip of {len(ins)}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
     """A plain device-to-device copy of known size: each launch reads and
     writes nbytes, so FETCH_SIZE and WRITE_SIZE of these dispatches give the
@@ -614,6 +638,9 @@ def main():
     ap.add_argument("--batched-by-scheme", action="store_true",
                     help="only time the batched Double-Heston sets split by their SchemeExact factor "
                          "count, print one JSON line and exit")
+    ap.add_argument("--cv-jump-tables", action="store_true",
+                    help="only build the control variate's terminal-jump tables of the Bates-Kou strip "
+                         "(vanilla and smoothed digitals), print one JSON line and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -631,6 +658,9 @@ def main():
     eng = Engine(0)
     if args.batched_by_scheme:
         print(json.dumps({"batched_by_scheme": batched_by_scheme(eng)}))
+        return
+    if args.cv_jump_tables:
+        print(json.dumps({"cv_jump_tables": cv_jump_tables(eng)}))
         return
     paths = 1 << args.paths_log2
     flags = 0 if (args.no_autotune or args.shape) else capi.FLAG_AUTOTUNE
