@@ -313,6 +313,42 @@ def cpu_baseline(target_s: float = 8.0):
                       f"({cpu_name})"}
 
 
+CV_LEGS = (("bates_strip", False, False), ("bates_kou_nv_strip", True, False),
+           ("bates_kou_nv_smoothed_digitals", True, True))
+
+
+def cv_nets(eng, path: str, save: bool):
+    """The three control-variate legs split in two processes, so that the
+    pricing kernels can be profiled apart from the training: save=True
+    trains each leg's network (as cv_leg does) and writes the myql_cv_net
+    struct to path.<i>; save=False reads them back and times the pricing
+    call alone (2^22 paths per maturity, after one warm-up call)."""
+    from myquantlib_amd import capi
+    from myquantlib_amd.pricing import cv_config
+    cfg = cv_config(train_paths=1 << 20, train_seed=7, steps=2000, batch=8192, lr=1e-3, points=16)
+    out = {}
+    for i, (name, kou, dig) in enumerate(CV_LEGS):
+        payoff = capi.PAYOFF_CASH if dig else capi.PAYOFF_VANILLA
+        mode = capi.GREEK_ESSENTIAL if dig else capi.GREEK_NONE
+        fn = f"{path}.{i}"
+        if save:
+            net = eng.cv_train(jobs(1024, capi.FP32, mode=mode, seed=7, bates_kou=kou, payoff=payoff), cfg)
+            with open(fn, "wb") as f:
+                f.write(bytes(net))
+            out[name] = {"train_ms": net.train_ms}
+        else:
+            with open(fn, "rb") as f:
+                net = capi.CVNet.from_buffer_copy(f.read())
+            J = jobs(1 << 22, capi.FP32, mode=mode, bates_kou=kou, payoff=payoff)
+            eng.mc_batch_cv(J, cfg, net)
+            t0 = time.perf_counter()
+            res = eng.mc_batch_cv(J, cfg, net)
+            out[name] = {"cv_ms": 1e3 * (time.perf_counter() - t0),
+                         "variance_ratio_median": float(np.median(np.concatenate([r["var_ratio"] for r in res])))}
+        print(f"# cv nets {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
+
+
 def cv_jump_tables(eng):
     """The control variate's terminal-jump tables of the Bates-Kou strip alone
     (2,048 log-spot points x 64 strikes per maturity; vanilla calls, and
@@ -638,6 +674,12 @@ def main():
     ap.add_argument("--batched-by-scheme", action="store_true",
                     help="only time the batched Double-Heston sets split by their SchemeExact factor "
                          "count, print one JSON line and exit")
+    ap.add_argument("--cv-net-save", default=None, metavar="PATH",
+                    help="only train the three control-variate legs' networks, write them to PATH.<i> "
+                         "and exit")
+    ap.add_argument("--cv-net-load", default=None, metavar="PATH",
+                    help="only price the three control-variate legs with the networks in PATH.<i> "
+                         "(no training), print one JSON line and exit")
     ap.add_argument("--cv-jump-tables", action="store_true",
                     help="only build the control variate's terminal-jump tables of the Bates-Kou strip "
                          "(vanilla and smoothed digitals), print one JSON line and exit")
@@ -658,6 +700,9 @@ def main():
     eng = Engine(0)
     if args.batched_by_scheme:
         print(json.dumps({"batched_by_scheme": batched_by_scheme(eng)}))
+        return
+    if args.cv_net_save or args.cv_net_load:
+        print(json.dumps({"cv_nets": cv_nets(eng, args.cv_net_save or args.cv_net_load, bool(args.cv_net_save))}))
         return
     if args.cv_jump_tables:
         print(json.dumps({"cv_jump_tables": cv_jump_tables(eng)}))
