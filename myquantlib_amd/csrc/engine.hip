@@ -395,6 +395,10 @@ struct myql_ctx {
   // by later calls) and launch arrays
   std::vector<std::unique_ptr<CvRecord>> cv_recs;
   DevBuf cv_jobs, cv_sets, cv_tasks, cv_part, cv_sums, cv_coef;
+  // terminal-jump tables E[P_k | S_T-], keyed by everything they depend on
+  // (jump law and parameters, maturity, S0, payoff, type, smoothing, strikes):
+  // a pricing call reuses the table its training call built
+  std::map<std::vector<double>, std::unique_ptr<DevBuf>> cv_rb;
   // randomized QMC: Sobol' direction integers [32][QMC_DMAX] and the
   // lower-6-bit table [QMC_DMAX][64] (built at the first such call), and
   // the digital shifts of a call's replicates
@@ -418,6 +422,7 @@ void set_device(myql_ctx *c) {
   HIPCHECK(hipSetDevice(c->device));
   if (c->streams.size() > 16) c->streams.clear();
   if (c->nci.size() > 32) c->nci.clear();
+  if (c->cv_rb.size() > 64) c->cv_rb.clear();
 }
 
 // Device memory of `bytes` for buffer b: b grown as needed, or, inside
@@ -1798,14 +1803,32 @@ CvNorm cv_norm(const myql_mc_job *jobs, int64_t n) {
 struct CvRecord {
   CvSet set{};
   std::vector<unsigned char> job;  // host copy of the Job<R> the paths run
-  DevBuf rec, st, sm, strikes, rb, mu, isd;
+  DevBuf rec, st, sm, strikes, mu, isd;
 };
 
 // The set's terminal-jump expectation E[P_k | S_T-] (vanilla, sharp and
 // smoothed digital payoffs under Merton or Kou jumps; S.payoff, S.call,
 // S.smooth, S.eps_scale, S.strikes and S.nk set; ks the strikes on the
-// host): its parameters and, on the log-spot grid around S0, its table,
-// filled on the context stream into `rb`.
+// host): its parameters and, on the log-spot grid around S0, its table
+// (cached in the context, built on the context stream when new).
+// The terminal-jump table of set S (parameters filled), from the context's
+// cache or built on the context stream into a new cache entry.
+const double *cv_rb_table(myql_ctx *c, const myql_model &m, double T, double S0, const double *ks,
+                          const CvSet &S) {
+  std::vector<double> key{static_cast<double>(m.jump_kind), m.jump[0], m.jump[1], m.jump[2], m.jump[3], T, S0,
+                          static_cast<double>(S.payoff), static_cast<double>(S.call),
+                          static_cast<double>(S.smooth), S.smooth ? S.eps_scale : 0.0};
+  key.insert(key.end(), ks, ks + S.nk);
+  auto it = c->cv_rb.find(key);
+  if (it != c->cv_rb.end()) return it->second->as<double>();
+  auto buf = std::make_unique<DevBuf>();
+  buf->reserve(static_cast<size_t>(S.nk) * CV_RB_PTS * 2 * sizeof(double));
+  HIPCHECK(launch_cv_rb_table(S, buf->as<double>(), c->stream));
+  const double *p = buf->as<double>();
+  c->cv_rb.emplace(std::move(key), std::move(buf));
+  return p;
+}
+
 // 16-point Gauss-Legendre rule on [-1, 1]: roots of P_16 by Newton's
 // method from the Chebyshev-like first guesses, w = 2 / ((1 - x^2) P_16'(x)^2)
 static void gauss_legendre16(double *x, double *w) {
@@ -1833,8 +1856,7 @@ static void gauss_legendre16(double *x, double *w) {
   }
 }
 
-void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, const double *ks, CvSet &S,
-                      DevBuf &rb) {
+void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, const double *ks, CvSet &S) {
   const int nk = S.nk;
   gauss_legendre16(S.gl_x, S.gl_w);
   // Merton terminal jump: E[P | S_T-] in closed form for vanilla and sharp
@@ -1917,9 +1939,7 @@ void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, con
       S.rb_y0 = std::log(S0) - 3.0;
       S.rb_h = 5.0 / (CV_RB_PTS - 1);
       S.rb_inv_h = 1.0 / S.rb_h;
-      rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
-      S.rb = rb.as<double>();
-      HIPCHECK(launch_cv_rb_table(S, rb.as<double>(), c->stream));
+      S.rb = cv_rb_table(c, m, T, S0, ks, S);
     }
   } else if (S.rb_on) {
     const double lt = m.jump[0] * T, mu = m.jump[1], dl = m.jump[2];
@@ -1949,9 +1969,7 @@ void cv_terminal_jump(myql_ctx *c, const myql_model &m, double T, double S0, con
       S.rb_y0 = std::log(S0) - 3.0;
       S.rb_h = 5.0 / (CV_RB_PTS - 1);
       S.rb_inv_h = 1.0 / S.rb_h;
-      rb.reserve(static_cast<size_t>(nk) * CV_RB_PTS * 2 * sizeof(double));
-      S.rb = rb.as<double>();
-      HIPCHECK(launch_cv_rb_table(S, rb.as<double>(), c->stream));
+      S.rb = cv_rb_table(c, m, T, S0, ks, S);
     }
   }
 }
@@ -2018,7 +2036,7 @@ void cv_record(myql_ctx *c, const myql_mc_job &jb, const myql_mc_config &cfg, in
     out.sm.reserve(S.Ms * sizeof(double));
     S.Sm = out.sm.as<double>();
   }
-  cv_terminal_jump(c, m, T, jb.S0, in.strikes, S, out.rb);
+  cv_terminal_jump(c, m, T, jb.S0, in.strikes, S);
 }
 
 // Device copies of sets (kept alive by the caller until it synchronises).
@@ -2809,6 +2827,7 @@ void myql_destroy(myql_ctx *c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   c->streams.clear();
   c->nci.clear();
+  c->cv_rb.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -3197,7 +3216,7 @@ int myql_debug_cv_jump_expectation(myql_ctx *c, const myql_model *model, const m
       if (!(spots[i] > 0.0)) throw Fail(MYQL_E_INVALID_ARG, "spots must be positive");
     const int nk = static_cast<int>(instrument->n_strikes);
     CvSet S{};
-    DevBuf ks, rb, d_sp, d_out;
+    DevBuf ks, d_sp, d_out;
     ks.reserve(nk * sizeof(double));
     HIPCHECK(hipMemcpyAsync(ks.p, instrument->strikes, nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
     S.strikes = ks.as<double>();
@@ -3208,7 +3227,7 @@ int myql_debug_cv_jump_expectation(myql_ctx *c, const myql_model *model, const m
     if (S.smooth && !(instrument->payoff_eps > 0.0))
       throw Fail(MYQL_E_INVALID_ARG, "a digital payoff in a Greek mode needs payoff_eps > 0 here");
     S.eps_scale = S.smooth ? 0.70710678118654752440 / instrument->payoff_eps : 0.0;
-    cv_terminal_jump(c, *model, instrument->maturity, S0, instrument->strikes, S, rb);
+    cv_terminal_jump(c, *model, instrument->maturity, S0, instrument->strikes, S);
     *table_on = S.rb_on;
     if (S.rb_on && n > 0) {
       const size_t m = static_cast<size_t>(n) * nk;
